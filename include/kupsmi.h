@@ -1,0 +1,136 @@
+/*
+ * kupsmi.h -- C ABI of libkupsmi.so, the MI355X (gfx950) implementation of
+ * kUPS's batched particle-interaction hot path.
+ *
+ * Conventions (all entry points):
+ *   - Every pointer argument except `ctx` and host scalars is a DEVICE pointer
+ *     owned by the caller.  Nothing here allocates on the hot path: scratch
+ *     lives in the context and only grows (kupsmi_ctx_reserve / first use).
+ *   - `stream` is a hipStream_t passed as void*; all work is enqueued on it
+ *     and no entry point synchronises the host.  Capacity overflows are
+ *     reported through device counters that the caller reads at its own sync
+ *     point (the reference's Capacity/fix_or_raise convention,
+ *     core/capacity.py:175-234, core/propagator.py:137-172).
+ *   - Return value: 0 ok, <0 error (see KUPSMI_ERR_*; kupsmi_last_error()
+ *     gives the message).  No C++ exception crosses the ABI.
+ *   - Systems are contiguous atom ranges given by a system-offset table
+ *     sys_off[n_sys + 1] (the kUPS particle Table argsorted by system,
+ *     cf. potential/common/graph.py:122-164).
+ *   - Unit cells: lower-triangular lattice, tril[n_sys][6] =
+ *     [L00, L10, L11, L20, L21, L22] in FP64 (core/unitcell.py:251-300).
+ *   - Positions, momenta, forces: FP32, row-major (n, 3) like kUPS arrays.
+ *     Energies and virials: FP64 per system.
+ *   - Units: eV, Angstrom, amu; time in kUPS internal units
+ *     (1 fs = core/constants.py FEMTO_SECOND).
+ */
+#ifndef KUPSMI_H
+#define KUPSMI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KUPSMI_OK 0
+#define KUPSMI_ERR_ARG (-1)
+#define KUPSMI_ERR_HIP (-2)
+#define KUPSMI_ERR_UNSUPPORTED (-3)
+
+/* largest system the LDS-resident kernels hold (atoms per system) */
+#define KUPSMI_LDS_MAX_ATOMS 4096
+
+typedef struct kupsmi_ctx kupsmi_ctx;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int kupsmi_version(void);
+/* Message of the last error returned on this thread ("" if none). */
+const char* kupsmi_last_error(void);
+
+/* Context: owns scratch on the current HIP device.  No counterpart in the
+ * reference (JAX/XLA owns buffers there); it is the handle the jax.ffi
+ * adapter would keep per device. */
+int kupsmi_ctx_create(kupsmi_ctx** out);
+int kupsmi_ctx_destroy(kupsmi_ctx* ctx);
+
+/*
+ * Full directed neighbor list of every system (lh = rh, no remap).
+ * Replaces NearestNeighborList.__call__(lh, None, systems, cutoffs)
+ * (core/neighborlist.py:233-255) as implemented by CellListNeighborList /
+ * DenseNearestNeighborList (985-1077, 902-982) via basic_neighborlist
+ * (743-786): same system, strict |d| < rc decided in FP64, shifts
+ * round-half-even(f_i - f_j) or the centred image grid for systems whose
+ * perpendicular width is below 2 rc (453-487, 595-624), pairs with equal
+ * exclusion id dropped on the minimum image only (731-735).
+ * Edges are written i-major: edges_ij[e] = (i, j), edges_shift[e] = (s0,s1,s2);
+ * at most `capacity` edges are written and *count (device int64) receives the
+ * required number (> capacity means: regrow and call again).
+ */
+int kupsmi_neighborlist(kupsmi_ctx* ctx, void* stream, int n_sys,
+                        const int32_t* sys_off, int n_atoms, const float* pos,
+                        const int32_t* exclusion, const double* tril,
+                        const double* cutoff, int64_t capacity,
+                        int32_t* edges_ij, int32_t* edges_shift, int64_t* count);
+
+/*
+ * Lennard-Jones energy, forces and virial of every system at the given
+ * positions.  Replaces the LJ Potential (potential/classical/lennard_jones.py:
+ * 149-178 energy, differentiated by potential/common/energy.py:263-355):
+ * per directed edge u = 4 eps (c6^2 - c6), c6 = (sigma^2/r^2)^3, hard cutoff
+ * r^2 < rc^2 without shift, E = sum / 2; F = -dE/dr; virial
+ * W = -sum_directed u'(r^2) d (x) d (stress = W / V, observables/stress.py).
+ * Pair arithmetic FP32, cutoff band re-decided in FP64, energies/virials
+ * accumulated in FP64.  sigma/eps: (n_species x n_species) mixed tables in
+ * FP64 (Lorentz-Berthelot, lennard_jones.py:125-141).
+ * Exclusion is by particle (i != j on the minimum image), as in md_lj.
+ * Systems must hold <= KUPSMI_LDS_MAX_ATOMS atoms.
+ * Outputs: forces (n_atoms x 3, f32), energy[n_sys], virial[n_sys x 9] (f64).
+ */
+int kupsmi_lj_forces(kupsmi_ctx* ctx, void* stream, int n_sys,
+                     const int32_t* sys_off, int n_atoms, const float* pos,
+                     const int32_t* species, int n_species, const double* sigma,
+                     const double* epsilon, const double* tril,
+                     const double* cutoff, float* forces, double* energy,
+                     double* virial);
+
+/*
+ * Batched LJ molecular dynamics: n_steps steps of every system in ONE
+ * persistent launch, each system resident in LDS (one workgroup per system,
+ * work queue largest-first).  Replaces the MD step propagators built by
+ * make_md_step_from_state (md/integrators.py:904-962):
+ *   integrator 0: velocity Verlet  B(dt/2) A(dt) F B(dt/2)      (273-317)
+ *   integrator 1: BAOAB Langevin   B(dt/2) A(dt/2) O A(dt/2) F B(dt/2) (405-453)
+ * A steps wrap positions to [-0.5, 0.5) fractional (104-141); O uses
+ * p <- c1 p + sqrt(kT (1 - c1^2)) sqrt(m) eta with eta from Philox4x32-10
+ * keyed by (seed, global atom index, step), see kupsmi_normals.
+ * The neighbor set is rebuilt every step (cell list in LDS), exactly the set
+ * of kupsmi_neighborlist with particle exclusion.
+ * In/out: pos, mom, forces (F at the current positions on entry: the
+ * reference initialises it to zero, md/data.py:188).
+ * Per system: mass per atom (f32), dt, kT (eV), gamma (1/internal time).
+ * Outputs: pot_energy[n_sys x n_steps] and kin_energy[n_sys x n_steps] (f64,
+ * after each step; either may be NULL), virial[n_sys x 9] of the last step
+ * (may be NULL).  step0 is the global index of the first step (RNG stream
+ * position).  n_steps == 0 only evaluates forces/energy at the input state.
+ */
+int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n_steps,
+                 int64_t step0, uint64_t seed, int n_sys, const int32_t* sys_off,
+                 int n_atoms, float* pos, float* mom, float* forces,
+                 const float* mass, const int32_t* species, int n_species,
+                 const double* sigma, const double* epsilon, const double* tril,
+                 const double* cutoff, const double* dt, const double* kT,
+                 const double* gamma, double* pot_energy, double* kin_energy,
+                 double* virial);
+
+/*
+ * The standard normals eta[n_atoms x 3] that kupsmi_md_lj's O step uses at
+ * global step `step` (Philox4x32-10, Salmon et al. SC'11, counter =
+ * (atom, step), key = seed; Box-Muller).  For parity tests of the stochastic
+ * step.  No counterpart in the reference (jax.random.normal there).
+ */
+int kupsmi_normals(void* stream, uint64_t seed, int64_t step, int n_atoms,
+                   float* eta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KUPSMI_H */

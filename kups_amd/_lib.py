@@ -1,0 +1,100 @@
+"""ctypes binding of libkupsmi.so (the HIP/gfx950 library).
+
+The library is the only compute path: if it is missing, or no GPU is
+visible, every operation raises.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libkupsmi.so"
+
+_lib = None
+_ctx: dict[int, C.c_void_p] = {}
+
+vp = C.c_void_p
+i32 = C.c_int32
+i64 = C.c_int64
+u64 = C.c_uint64
+
+_SIGNATURES = {
+    "kupsmi_version": ([], C.c_int),
+    "kupsmi_last_error": ([], C.c_char_p),
+    "kupsmi_ctx_create": ([C.POINTER(vp)], C.c_int),
+    "kupsmi_ctx_destroy": ([vp], C.c_int),
+    "kupsmi_neighborlist": ([vp, vp, i32, vp, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
+    "kupsmi_lj_forces": ([vp, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp], C.c_int),
+    "kupsmi_md_lj": (
+        [vp, vp, i32, i32, i64, u64, i32, vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp,
+         vp, vp, vp, vp, vp, vp],
+        C.c_int,
+    ),
+    "kupsmi_normals": ([vp, u64, i64, i32, vp], C.c_int),
+}
+
+
+class KupsmiError(RuntimeError):
+    """An entry point of libkupsmi returned an error."""
+
+
+def load():
+    """Load libkupsmi.so and declare its entry points (no GPU needed)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise KupsmiError(
+                f"{LIB_PATH} is missing: build it with `python -m kups_amd.build` "
+                "(there is no CPU fallback)"
+            )
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().kupsmi_last_error().decode(errors="replace")
+        raise KupsmiError(f"libkupsmi error {rc}: {msg}")
+
+
+def require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise KupsmiError("no GPU visible: the kups_amd kernels run only on the MI355X")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def ctx() -> C.c_void_p:
+    """The library context of the current device (created on first use)."""
+    dev = require_gpu().index
+    if dev not in _ctx:
+        h = vp()
+        check(load().kupsmi_ctx_create(C.byref(h)))
+        _ctx[dev] = h
+    return _ctx[dev]
+
+
+def stream_ptr(stream: torch.cuda.Stream | None = None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise KupsmiError("kups_amd kernels take device tensors")
+    if not t.is_contiguous():
+        raise KupsmiError("kups_amd kernels take contiguous tensors")
+    return t.data_ptr()

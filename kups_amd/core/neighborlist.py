@@ -1,0 +1,90 @@
+"""Neighbor lists with the reference's plugin interface.
+
+``CellListNeighborList`` / ``DenseNearestNeighborList`` mirror
+core/neighborlist.py:902-1077: ``nl(lh, rh, systems, cutoffs,
+rh_index_remap)`` returns the directed ``Edges`` of every system.  The pair
+set is decided on the GPU in FP64 by ``kupsmi_neighborlist`` with the
+reference's semantics (strict cutoff, image grid for narrow cells,
+minimum-image-only exclusion).  Capacities follow the reference: the edge
+buffer holds ``avg_edges * n`` entries, an overflow raises CapacityError or,
+with ``fix=True``, regrows to the next power of two and recomputes
+(core/capacity.py:175-234, core/propagator.py:137-172).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from .. import _lib
+from .batch import ParticleBatch, SystemBatch, system_offsets
+from .capacity import CapacityError, next_power_of_two
+
+
+@dataclass
+class Edges:
+    """Directed pairs: indices (E, 2) and integer image shifts (E, 1, 3);
+    d_ij = r_j - r_i + shift @ L (core/neighborlist.py:115-211)."""
+
+    indices: torch.Tensor
+    shifts: torch.Tensor
+
+    def __len__(self) -> int:
+        return self.indices.shape[0]
+
+    def difference_vectors(self, positions: torch.Tensor, system: torch.Tensor,
+                           systems: SystemBatch) -> torch.Tensor:
+        lat = systems.lattice()[system[self.indices[:, 0]].long()]
+        sh = self.shifts[:, 0].to(lat.dtype)
+        off = torch.einsum("ej,ejk->ek", sh, lat)
+        p = positions.to(lat.dtype)
+        return p[self.indices[:, 1].long()] - p[self.indices[:, 0].long()] + off
+
+
+@dataclass
+class DenseNearestNeighborList:
+    """All same-system pairs (core/neighborlist.py:902-982)."""
+
+    avg_edges: int = 64
+    fix: bool = True
+
+    def __call__(self, lh: ParticleBatch, rh: ParticleBatch | None, systems: SystemBatch,
+                 cutoffs: torch.Tensor, rh_index_remap=None) -> Edges:
+        if rh is not None or rh_index_remap is not None:
+            raise NotImplementedError(
+                "rh queries are served inside the GCMC kernels; the export takes rh=None")
+        lib = _lib.load()
+        n_sys = systems.n_sys
+        sys_off, _ = system_offsets(lh.system, n_sys)
+        excl = lh.exclusion
+        if excl is None:
+            excl = torch.arange(lh.n, dtype=torch.int32, device=lh.positions.device)
+        cut = torch.as_tensor(cutoffs, dtype=torch.float64, device=lh.positions.device)
+        cut = cut.expand(n_sys).contiguous() if cut.dim() == 0 or cut.numel() == 1 else cut.contiguous()
+        count = torch.zeros(1, dtype=torch.int64, device=lh.positions.device)
+        avg = self.avg_edges
+        while True:
+            cap = max(avg * lh.n, 1)
+            ij = torch.empty((cap, 2), dtype=torch.int32, device=lh.positions.device)
+            sh = torch.empty((cap, 3), dtype=torch.int32, device=lh.positions.device)
+            _lib.check(lib.kupsmi_neighborlist(
+                _lib.ctx(), _lib.stream_ptr(), n_sys, _lib.ptr(sys_off), lh.n,
+                _lib.ptr(lh.positions.contiguous()), _lib.ptr(excl.to(torch.int32).contiguous()),
+                _lib.ptr(systems.tril), _lib.ptr(cut), cap, _lib.ptr(ij), _lib.ptr(sh),
+                _lib.ptr(count)))
+            required = int(count.item())
+            if required <= cap:
+                self.avg_edges = avg
+                return Edges(ij[:required], sh[:required].reshape(-1, 1, 3))
+            if not self.fix:
+                raise CapacityError(
+                    f"neighbor list needs {required} edges, capacity is {cap}", required)
+            avg = next_power_of_two(-(-required // max(lh.n, 1)))
+
+
+@dataclass
+class CellListNeighborList(DenseNearestNeighborList):
+    """Cell-list neighbor list (core/neighborlist.py:985-1077).  Its pair set is
+    the dense set; the export decides every pair in FP64, the MD/GCMC kernels
+    use LDS cell lists internally."""

@@ -1,0 +1,188 @@
+// Shared device/host helpers of libkupsmi (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "../../include/kupsmi.h"
+
+namespace kupsmi {
+
+// ------------------------------------------------------------- errors ----
+void set_error(const std::string& msg);
+#define KUPSMI_HIP(call)                                                   \
+  do {                                                                     \
+    hipError_t _e = (call);                                                \
+    if (_e != hipSuccess) {                                                \
+      ::kupsmi::set_error(std::string(#call) + ": " + hipGetErrorString(_e)); \
+      return KUPSMI_ERR_HIP;                                               \
+    }                                                                      \
+  } while (0)
+#define KUPSMI_CHECK_LAUNCH() KUPSMI_HIP(hipGetLastError())
+#define KUPSMI_REQUIRE(cond, msg)                                          \
+  do {                                                                     \
+    if (!(cond)) {                                                         \
+      ::kupsmi::set_error(msg);                                            \
+      return KUPSMI_ERR_ARG;                                               \
+    }                                                                      \
+  } while (0)
+
+// ------------------------------------------------------------ context ----
+// Grow-only device scratch.  Allocation happens only when a call needs more
+// than any earlier call; steady-state calls allocate nothing.
+struct Scratch {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+}  // namespace kupsmi
+
+struct kupsmi_ctx {
+  int device = 0;
+  kupsmi::Scratch slots[8];
+};
+
+namespace kupsmi {
+// Returns a device buffer of at least `bytes` for scratch slot `slot`.
+int scratch(kupsmi_ctx* ctx, int slot, size_t bytes, void** out);
+
+// --------------------------------------------------------- unit cells ----
+// Per-system geometry derived on the device from tril and cutoff, the
+// quantities core/neighborlist.py:281-292 (cells per axis) and 453-459
+// (image counts) compute.
+struct SysGeom {
+  double L[6];    // L00 L10 L11 L20 L21 L22
+  double Li[6];   // inverse, same layout
+  float Lf[6];
+  float Lif[6];
+  double rc, rc2;
+  float rc2f;
+  int nc[3];      // cells per axis
+  int h[3];       // stencil half width per axis (1, or image half count)
+  int rep;        // 1 if the system needs periodic images (image grid)
+  int ncell;
+};
+
+__host__ __device__ inline void tril_inverse(const double* t, double* li) {
+  const double a = t[0], b = t[1], c = t[2], d = t[3], e = t[4], f = t[5];
+  li[0] = 1.0 / a;
+  li[1] = -b / (a * c);
+  li[2] = 1.0 / c;
+  li[3] = (b * e - c * d) / (a * c * f);
+  li[4] = -e / (c * f);
+  li[5] = 1.0 / f;
+}
+
+// x @ M for lower-triangular M in tril layout (core/utils/math.py:177-239).
+template <typename T, typename M>
+__host__ __device__ inline void tmul(const M* m, T x0, T x1, T x2, T& y0, T& y1, T& y2) {
+  y0 = x0 * (T)m[0] + x1 * (T)m[1] + x2 * (T)m[3];
+  y1 = x1 * (T)m[2] + x2 * (T)m[4];
+  y2 = x2 * (T)m[5];
+}
+
+// ncell_cap bounds cells per system (LDS tables).
+__host__ __device__ inline void make_geom(const double* tril, double rc, int max_nc_axis,
+                                          SysGeom& g) {
+  for (int k = 0; k < 6; ++k) {
+    g.L[k] = tril[k];
+    g.Lf[k] = (float)tril[k];
+  }
+  tril_inverse(tril, g.Li);
+  for (int k = 0; k < 6; ++k) g.Lif[k] = (float)g.Li[k];
+  const double* t = tril;
+  // rows a=(t0,0,0) b=(t1,t2,0) c=(t3,t4,t5); vol = t0 t2 t5
+  double vol = fabs(t[0] * t[2] * t[5]);
+  // |b x c|, |a x c|, |a x b|
+  double bc0 = t[2] * t[5], bc1 = -t[1] * t[5], bc2 = t[1] * t[4] - t[2] * t[3];
+  double ac1 = -t[0] * t[5], ac2 = t[0] * t[4];
+  double ab2 = t[0] * t[2];
+  double perp[3];
+  perp[0] = vol / sqrt(bc0 * bc0 + bc1 * bc1 + bc2 * bc2);
+  perp[1] = vol / sqrt(ac1 * ac1 + ac2 * ac2);
+  perp[2] = vol / fabs(ab2);
+  g.rc = rc;
+  g.rc2 = rc * rc;
+  g.rc2f = (float)g.rc2;
+  int img[3];
+  for (int a = 0; a < 3; ++a) {
+    int n = isfinite(rc) ? (int)ceil(2.0 * rc / perp[a]) : 1;
+    if (n < 1) n = 1;
+    if (n % 2 == 0) n += 1;
+    img[a] = n;
+  }
+  g.rep = img[0] * img[1] * img[2] > 1;
+  for (int a = 0; a < 3; ++a) {
+    if (g.rep) {
+      g.nc[a] = 1;
+      g.h[a] = (img[a] - 1) / 2;
+    } else {
+      int n = isfinite(rc) ? (int)(perp[a] / rc) : 1;
+      if (n < 1) n = 1;
+      if (n > max_nc_axis) n = max_nc_axis;
+      g.nc[a] = n;
+      g.h[a] = 1;
+    }
+  }
+  g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
+}
+
+// Fractional wrap to [-0.5, 0.5) (core/unitcell.py:139-171), returns the
+// wrapped fractional coordinate.
+__device__ inline float wrap_half(float f) {
+  float u = f + 0.5f;
+  u = u - floorf(u);
+  if (u >= 1.0f) u = 0.0f;  // guard the rounding of u - floor(u) to 1
+  return u - 0.5f;
+}
+
+__device__ inline int cell_index(float fw, int n) {
+  int b = (int)((fw + 0.5f) * (float)n);
+  return b < 0 ? 0 : (b >= n ? n - 1 : b);
+}
+
+// --------------------------------------------------------- Philox RNG ----
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based.
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)M0 * c[0];
+    uint64_t p1 = (uint64_t)M1 * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += W0; k1 += W1;
+  }
+}
+
+// Uniform in (0, 1): (x + 0.5) 2^-32.
+__host__ __device__ inline float u01(uint32_t x) {
+  return ((float)x + 0.5f) * 2.3283064365386963e-10f;
+}
+
+// Three standard normals for (atom, step) by Box-Muller.
+__device__ inline void normals3(uint64_t seed, uint32_t atom, int64_t step, float& n0,
+                                float& n1, float& n2) {
+  uint32_t c[4] = {atom, (uint32_t)step, (uint32_t)((uint64_t)step >> 32), 0x4B555053u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  float r0 = sqrtf(-2.0f * logf(u01(c[0])));
+  float r1 = sqrtf(-2.0f * logf(u01(c[2])));
+  float s0, c0, s1, c1;
+  sincospif(2.0f * u01(c[1]), &s0, &c0);
+  sincospif(2.0f * u01(c[3]), &s1, &c1);
+  n0 = r0 * c0;
+  n1 = r0 * s0;
+  n2 = r1 * c1;
+}
+
+// ---------------------------------------------------------- reductions ----
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_sumf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace kupsmi

@@ -1,0 +1,56 @@
+// Context, scratch and error plumbing of the C ABI.
+#include <string>
+#include "common.hpp"
+
+namespace kupsmi {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int scratch(kupsmi_ctx* ctx, int slot, size_t bytes, void** out) {
+  KUPSMI_REQUIRE(ctx != nullptr, "null context");
+  KUPSMI_REQUIRE(slot >= 0 && slot < 8, "bad scratch slot");
+  Scratch& s = ctx->slots[slot];
+  if (s.bytes < bytes) {
+    // Grow geometrically so a slowly growing workload reallocates rarely.
+    size_t nb = s.bytes * 2 > bytes ? s.bytes * 2 : bytes;
+    nb = (nb + 255) & ~(size_t)255;
+    if (s.ptr) KUPSMI_HIP(hipFree(s.ptr));
+    s.ptr = nullptr;
+    s.bytes = 0;
+    KUPSMI_HIP(hipMalloc(&s.ptr, nb));
+    s.bytes = nb;
+  }
+  *out = s.ptr;
+  return KUPSMI_OK;
+}
+
+}  // namespace kupsmi
+
+using namespace kupsmi;
+
+extern "C" int kupsmi_version(void) { return 100; }
+
+extern "C" const char* kupsmi_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int kupsmi_ctx_create(kupsmi_ctx** out) {
+  KUPSMI_REQUIRE(out != nullptr, "null output pointer");
+  kupsmi_ctx* c = new (std::nothrow) kupsmi_ctx();
+  KUPSMI_REQUIRE(c != nullptr, "out of host memory");
+  KUPSMI_HIP(hipGetDevice(&c->device));
+  *out = c;
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_ctx_destroy(kupsmi_ctx* ctx) {
+  if (!ctx) return KUPSMI_OK;
+  int prev = 0;
+  KUPSMI_HIP(hipGetDevice(&prev));
+  KUPSMI_HIP(hipSetDevice(ctx->device));
+  for (auto& s : ctx->slots)
+    if (s.ptr) (void)hipFree(s.ptr);
+  (void)hipSetDevice(prev);
+  delete ctx;
+  return KUPSMI_OK;
+}

@@ -1,0 +1,92 @@
+"""MD state of a batch of systems (application/md/data.py:148-218).
+
+Momenta are drawn from Maxwell-Boltzmann, p ~ N(0, sqrt(m kT)) per component,
+minus the mean momentum of the system (171-178); gradients start at zero
+(188), so the first half kick of the first step uses F = 0 unless the caller
+evaluates forces first.  Time step and friction are converted from fs and
+1/fs to internal units (195-205).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .._lib import require_gpu
+from ..core.batch import offsets_from_sizes
+from ..core.constants import BOLTZMANN_CONSTANT, FEMTO_SECOND
+
+
+@dataclass
+class MDState:
+    positions: torch.Tensor  # (n, 3) f32
+    momenta: torch.Tensor  # (n, 3) f32
+    forces: torch.Tensor  # (n, 3) f32, F at the current positions
+    masses: torch.Tensor  # (n,) f32
+    species: torch.Tensor  # (n,) int32
+    sys_off: torch.Tensor  # (n_sys + 1,) int32
+    sizes: np.ndarray  # host copy of the system sizes
+    tril: torch.Tensor  # (n_sys, 6) f64
+    cutoff: torch.Tensor  # (n_sys,) f64
+    time_step: torch.Tensor  # (n_sys,) f64, internal units
+    temperature: torch.Tensor  # (n_sys,) f64, K
+    friction: torch.Tensor  # (n_sys,) f64, 1 / internal time
+    step: int = 0
+    extras: dict = field(default_factory=dict)
+
+    @property
+    def n_sys(self) -> int:
+        return self.tril.shape[0]
+
+    @property
+    def n(self) -> int:
+        return self.positions.shape[0]
+
+    @property
+    def kT(self) -> torch.Tensor:
+        return self.temperature * BOLTZMANN_CONSTANT
+
+
+def maxwell_boltzmann_momenta(masses: np.ndarray, sizes, temperature, rng: np.random.Generator):
+    """p ~ N(0, sqrt(m kT)) minus the per-system mean (md/data.py:171-178)."""
+    masses = np.asarray(masses, np.float64)
+    temperature = np.broadcast_to(np.asarray(temperature, np.float64), (len(sizes),))
+    sys_of = np.repeat(np.arange(len(sizes)), sizes)
+    std = np.sqrt(masses * temperature[sys_of] * BOLTZMANN_CONSTANT)
+    p = rng.standard_normal((masses.size, 3)) * std[:, None]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for s in range(len(sizes)):
+        a, b = off[s], off[s + 1]
+        if b > a:
+            p[a:b] -= p[a:b].sum(axis=0) / (b - a)
+    return p
+
+
+def make_md_state(positions, masses, species, sizes, tril, cutoff, *, time_step_fs,
+                  temperature, friction_per_fs=0.0, momenta=None, device=None) -> MDState:
+    dev = device or require_gpu()
+    n_sys = len(sizes)
+    n = int(np.sum(sizes))
+
+    def per_sys(x):
+        return torch.as_tensor(np.broadcast_to(np.asarray(x, np.float64), (n_sys,)).copy(),
+                               dtype=torch.float64, device=dev)
+
+    if momenta is None:
+        momenta = np.zeros((n, 3))
+    return MDState(
+        positions=torch.as_tensor(np.asarray(positions, np.float32), device=dev).contiguous(),
+        momenta=torch.as_tensor(np.asarray(momenta, np.float32), device=dev).contiguous(),
+        forces=torch.zeros((n, 3), dtype=torch.float32, device=dev),
+        masses=torch.as_tensor(np.asarray(masses, np.float32), device=dev).contiguous(),
+        species=torch.as_tensor(np.asarray(species, np.int32), device=dev).contiguous(),
+        sys_off=offsets_from_sizes(sizes, dev),
+        sizes=np.asarray(sizes, np.int64),
+        tril=torch.as_tensor(np.asarray(tril, np.float64).reshape(n_sys, 6), device=dev).contiguous(),
+        cutoff=per_sys(cutoff),
+        time_step=per_sys(np.asarray(time_step_fs, np.float64) * FEMTO_SECOND),
+        temperature=per_sys(temperature),
+        friction=per_sys(np.asarray(friction_per_fs, np.float64) / FEMTO_SECOND),
+    )

@@ -1,0 +1,97 @@
+"""Fused MD propagators (md/integrators.py).
+
+``make_velocity_verlet_step`` and ``make_baoab_langevin_step`` return
+propagators with the reference's step order (273-317: B(dt/2) A(dt) F B(dt/2);
+405-453: B(dt/2) A(dt/2) O A(dt/2) F B(dt/2); positions wrapped after every A,
+104-141).  Where the reference composes MomentumStep/PositionStep/
+StochasticStep around a Potential, here one persistent HIP launch
+(``kupsmi_md_lj``) runs any number of steps of every system, each system
+resident in LDS with its cell list rebuilt every step.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from .. import _lib
+from ..potential.lennard_jones import LennardJonesParameters
+from .data import MDState
+
+VELOCITY_VERLET = 0
+BAOAB_LANGEVIN = 1
+
+
+@dataclass
+class MDObservables:
+    potential_energy: torch.Tensor  # (n_sys, n_steps) f64
+    kinetic_energy: torch.Tensor  # (n_sys, n_steps) f64
+    virial: torch.Tensor  # (n_sys, 3, 3) f64 at the last step
+
+
+@dataclass
+class FusedLJPropagator:
+    """n MD steps of every system of an MDState in one launch."""
+
+    params: LennardJonesParameters
+    integrator: int = VELOCITY_VERLET
+    seed: int = 0
+
+    def _tables(self, dev):
+        return (torch.as_tensor(self.params.sigma, dtype=torch.float64, device=dev).contiguous(),
+                torch.as_tensor(self.params.epsilon, dtype=torch.float64, device=dev).contiguous())
+
+    def evaluate_forces(self, state: MDState) -> tuple[torch.Tensor, torch.Tensor]:
+        """F, per-system energy and virial at the current positions (updates state.forces)."""
+        obs = self._launch(state, 0, record=True)
+        return obs.potential_energy[:, 0], obs.virial
+
+    def __call__(self, state: MDState, n_steps: int = 1, record: bool = True) -> MDObservables | None:
+        return self._launch(state, n_steps, record)
+
+    def _launch(self, state: MDState, n_steps: int, record: bool):
+        if state.sizes.size and state.sizes.max() > 4096:
+            raise ValueError("the fused MD kernel holds systems of at most 4096 atoms")
+        dev = state.positions.device
+        sig, eps = self._tables(dev)
+        n_rec = max(n_steps, 1)
+        pot = kin = vir = None
+        if record:
+            pot = torch.empty((state.n_sys, n_rec), dtype=torch.float64, device=dev)
+            kin = torch.empty((state.n_sys, n_rec), dtype=torch.float64, device=dev) if n_steps else None
+            vir = torch.empty((state.n_sys, 3, 3), dtype=torch.float64, device=dev)
+        kT = state.kT.contiguous()
+        _lib.check(_lib.load().kupsmi_md_lj(
+            _lib.ctx(), _lib.stream_ptr(), int(self.integrator), int(n_steps), int(state.step),
+            int(self.seed) & (2**64 - 1), state.n_sys, _lib.ptr(state.sys_off), state.n,
+            _lib.ptr(state.positions), _lib.ptr(state.momenta), _lib.ptr(state.forces),
+            _lib.ptr(state.masses), _lib.ptr(state.species), self.params.n_species,
+            _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(state.tril), _lib.ptr(state.cutoff),
+            _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction),
+            _lib.ptr(pot), _lib.ptr(kin), _lib.ptr(vir)))
+        state.step += n_steps
+        if not record:
+            return None
+        if kin is None:
+            kin = torch.zeros_like(pot)
+        return MDObservables(pot, kin, vir)
+
+
+def make_velocity_verlet_step(params: LennardJonesParameters, seed: int = 0) -> FusedLJPropagator:
+    """NVE velocity Verlet (md/integrators.py:273-317)."""
+    return FusedLJPropagator(params, VELOCITY_VERLET, seed)
+
+
+def make_baoab_langevin_step(params: LennardJonesParameters, seed: int = 0) -> FusedLJPropagator:
+    """NVT BAOAB Langevin (md/integrators.py:405-453); O step noise from Philox4x32-10."""
+    return FusedLJPropagator(params, BAOAB_LANGEVIN, seed)
+
+
+def normals(seed: int, step: int, n_atoms: int, device=None) -> torch.Tensor:
+    """The (n_atoms, 3) standard normals of the O step at global step ``step``."""
+    dev = device or _lib.require_gpu()
+    out = torch.empty((n_atoms, 3), dtype=torch.float32, device=dev)
+    _lib.check(_lib.load().kupsmi_normals(_lib.stream_ptr(), int(seed) & (2**64 - 1), int(step),
+                                          n_atoms, _lib.ptr(out)))
+    return out

@@ -1,0 +1,202 @@
+"""ctypes bindings of liboracle.so plus numpy restatements (test infrastructure).
+
+Every array argument is converted to a C-contiguous numpy array of the
+required dtype; positions and the like are FP64 here.  See kups_oracle.c for
+the reference citations of the compiled functions.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = C.CDLL(str(LIB_PATH))
+        _lib.orc_neighborlist.restype = C.c_long
+        _lib.orc_kvecs.restype = C.c_long
+        _lib.orc_coulomb_k.restype = C.c_double
+        _lib.orc_md_vv_f32.restype = C.c_int
+        _lib.orc_num_threads.restype = C.c_int
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def coulomb_k() -> float:
+    """HARTREE*BOHR in eV*Angstrom (potential/classical/ewald.py:82)."""
+    return lib().orc_coulomb_k()
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def neighborlist(pos, excl, sys_off, tril, cutoff):
+    """Directed pair set (i, j, shift) of every system; see orc_neighborlist."""
+    pos, excl, sys_off, tril, cutoff = _d(pos), _i(excl), _i(sys_off), _d(tril), _d(cutoff)
+    nsys = len(sys_off) - 1
+    L = lib()
+    dummy_ij = np.zeros(2, np.int32)
+    dummy_s = np.zeros(3, np.int32)
+    n = L.orc_neighborlist(_p(pos), _p(excl), nsys, _p(sys_off), _p(tril), _p(cutoff),
+                           C.c_long(0), _p(dummy_ij), _p(dummy_s))
+    ij = np.zeros((max(n, 1), 2), np.int32)
+    sh = np.zeros((max(n, 1), 3), np.int32)
+    n2 = L.orc_neighborlist(_p(pos), _p(excl), nsys, _p(sys_off), _p(tril), _p(cutoff),
+                            C.c_long(n), _p(ij), _p(sh))
+    assert n2 == n
+    return ij[:n], sh[:n]
+
+
+def lj_edges(pos, species, sys_off, tril, cutoff, sigma, eps, ij, shift):
+    """LJ energy/forces/virial over a directed edge list (lennard_jones.py:149-178)."""
+    pos, species, sys_off, tril, cutoff = _d(pos), _i(species), _i(sys_off), _d(tril), _d(cutoff)
+    sigma, eps, ij, shift = _d(sigma), _d(eps), _i(ij), _i(shift)
+    nsys = len(sys_off) - 1
+    e = np.zeros(nsys)
+    f = np.zeros_like(pos)
+    w = np.zeros((nsys, 9))
+    lib().orc_lj_edges(_p(pos), _p(species), nsys, _p(sys_off), _p(tril), _p(cutoff),
+                       sigma.shape[0], _p(sigma), _p(eps), C.c_long(len(ij)), _p(ij), _p(shift),
+                       _p(e), _p(f), _p(w))
+    return e, f, w.reshape(nsys, 3, 3)
+
+
+def lj_cells(pos, species, sys_off, tril, cutoff, sigma, eps):
+    """LJ energy/forces/virial by cell list, same pair set (particle exclusion)."""
+    pos, species, sys_off, tril, cutoff = _d(pos), _i(species), _i(sys_off), _d(tril), _d(cutoff)
+    sigma, eps = _d(sigma), _d(eps)
+    nsys = len(sys_off) - 1
+    e = np.zeros(nsys)
+    f = np.zeros_like(pos)
+    w = np.zeros((nsys, 9))
+    lib().orc_lj_cells(_p(pos), _p(species), nsys, _p(sys_off), _p(tril), _p(cutoff),
+                       sigma.shape[0], _p(sigma), _p(eps), _p(e), _p(f), _p(w))
+    return e, f, w.reshape(nsys, 3, 3)
+
+
+def morse_edges(pos, species, sys_off, tril, D, alpha, r0, ij, shift):
+    """Morse bond energy/forces/virial over explicit edges (morse.py:142-168)."""
+    pos, species, sys_off, tril = _d(pos), _i(species), _i(sys_off), _d(tril)
+    D, alpha, r0, ij, shift = _d(D), _d(alpha), _d(r0), _i(ij), _i(shift)
+    nsys = len(sys_off) - 1
+    e = np.zeros(nsys)
+    f = np.zeros_like(pos)
+    w = np.zeros((nsys, 9))
+    lib().orc_morse_edges(_p(pos), _p(species), nsys, _p(sys_off), _p(tril), D.shape[0],
+                          _p(D), _p(alpha), _p(r0), C.c_long(len(ij)), _p(ij), _p(shift),
+                          _p(e), _p(f), _p(w))
+    return e, f, w.reshape(nsys, 3, 3)
+
+
+def ewald_real_edges(pos, q, sys_off, tril, alpha, cutoff, ij, shift, exclusion_correction=False):
+    """Real-space Ewald (ewald.py:305-325) or the exclusion correction (328-346)."""
+    pos, q, sys_off, tril = _d(pos), _d(q), _i(sys_off), _d(tril)
+    alpha, cutoff, ij, shift = _d(alpha), _d(cutoff), _i(ij), _i(shift)
+    nsys = len(sys_off) - 1
+    e = np.zeros(nsys)
+    f = np.zeros_like(pos)
+    w = np.zeros((nsys, 9))
+    lib().orc_ewald_real_edges(_p(pos), _p(q), nsys, _p(sys_off), _p(tril), _p(alpha),
+                               _p(cutoff), C.c_long(len(ij)), _p(ij), _p(shift),
+                               int(bool(exclusion_correction)), _p(e), _p(f), _p(w))
+    return e, f, w.reshape(nsys, 3, 3)
+
+
+def ewald_recip(pos, q, sys_off, tril, alpha, kvec_n):
+    """Reciprocal-space Ewald (ewald.py:349-439, 585-601); kvec_n (nsys, nk, 3)."""
+    pos, q, sys_off, tril, alpha = _d(pos), _d(q), _i(sys_off), _d(tril), _d(alpha)
+    kvec_n = _i(kvec_n)
+    nsys = len(sys_off) - 1
+    nk = kvec_n.shape[1]
+    e = np.zeros(nsys)
+    f = np.zeros_like(pos)
+    w = np.zeros((nsys, 9))
+    sk = np.zeros((nsys, nk, 2))
+    lib().orc_ewald_recip(_p(pos), _p(q), nsys, _p(sys_off), _p(tril), _p(alpha), nk,
+                          _p(kvec_n), _p(e), _p(f), _p(w), _p(sk))
+    return e, f, w.reshape(nsys, 3, 3), sk
+
+
+def ewald_self(q, sys_off, alpha):
+    """Ewald self term -K alpha/sqrt(pi) sum q^2 (ewald.py:277-302)."""
+    q, sys_off, alpha = _d(q), _i(sys_off), _d(alpha)
+    nsys = len(sys_off) - 1
+    e = np.zeros(nsys)
+    lib().orc_ewald_self(_p(q), nsys, _p(sys_off), _p(alpha), _p(e))
+    return e
+
+
+def kvecs(tril, nmax, kmax=-1.0):
+    """Integer half-space k grid (ewald.py:1322-1339 recipe); kmax < 0 keeps the box."""
+    tril = _d(tril)
+    L = lib()
+    n = L.orc_kvecs(_p(tril), int(nmax), C.c_double(kmax), C.c_long(0), _p(np.zeros(3, np.int32)))
+    out = np.zeros((max(n, 1), 3), np.int32)
+    L.orc_kvecs(_p(tril), int(nmax), C.c_double(kmax), C.c_long(n), _p(out))
+    return out[:n]
+
+
+def md_step(integrator, pos, mom, mass, species, sys_off, tril, cutoff, sigma, eps, dt, kT,
+            gamma, eta, forces):
+    """One FP64 VV (0) / BAOAB (1) step in place; returns (energy, virial)."""
+    for a in (pos, mom, forces):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    mass, species, sys_off, tril, cutoff = _d(mass), _i(species), _i(sys_off), _d(tril), _d(cutoff)
+    sigma, eps, dt, kT, gamma, eta = _d(sigma), _d(eps), _d(dt), _d(kT), _d(gamma), _d(eta)
+    nsys = len(sys_off) - 1
+    e = np.zeros(nsys)
+    w = np.zeros((nsys, 9))
+    lib().orc_md_step(int(integrator), _p(pos), _p(mom), _p(mass), _p(species), nsys,
+                      _p(sys_off), _p(tril), _p(cutoff), sigma.shape[0], _p(sigma), _p(eps),
+                      _p(dt), _p(kT), _p(gamma), _p(eta), _p(forces), _p(e), _p(w))
+    return e, w.reshape(nsys, 3, 3)
+
+
+def md_vv_f32(n_steps, pos, mom, mass, species, sys_off, tril, cutoff, sigma, eps, dt, forces):
+    """n_steps FP32 velocity-Verlet LJ steps in place (the FP32 CPU baseline)."""
+    for a in (pos, mom, forces):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    mass, species, sys_off, tril, cutoff = _f(mass), _i(species), _i(sys_off), _d(tril), _d(cutoff)
+    sigma, eps, dt = _f(sigma), _f(eps), _d(dt)
+    nsys = len(sys_off) - 1
+    e = np.zeros(nsys)
+    rc = lib().orc_md_vv_f32(int(n_steps), _p(pos), _p(mom), _p(mass), _p(species), nsys,
+                             _p(sys_off), _p(tril), _p(cutoff), sigma.shape[0], _p(sigma),
+                             _p(eps), _p(dt), _p(forces), _p(e))
+    if rc != 0:
+        raise ValueError("orc_md_vv_f32 supports only systems that need no periodic images")
+    return e
