@@ -130,6 +130,59 @@ int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n_steps,
 int kupsmi_normals(void* stream, uint64_t seed, int64_t step, int n_atoms,
                    float* eta);
 
+/*
+ * LJ + Ewald real-space energies, forces and virials of systems of any size
+ * (<= 16384 atoms each), state in HBM.  Replaces, for one evaluation, the LJ
+ * energy (potential/classical/lennard_jones.py:149-178) and the Ewald
+ * short-range energy (potential/classical/ewald.py:305-325:
+ * K/2 sum_directed q_i q_j erfc(alpha r)/r [r < rc]) with their jax.vjp
+ * gradients (potential/common/energy.py:297-300).
+ * LJ is skipped when sigma/epsilon/lj_cutoff are NULL; Coulomb when
+ * alpha/coul_cutoff/charges are NULL.  lj_exclusion / coul_exclusion: ids
+ * whose equal pairs are dropped on the minimum image (NULL: particle id),
+ * so GCMC hosts can exclude host-host LJ while keeping host-host Coulomb
+ * (application/simulations/mcmc_rigid.py:156-165).
+ * energy: n_sys x 2 (LJ, real-space) FP64; virial n_sys x 9.  accumulate != 0
+ * adds into forces/energy/virial instead of overwriting.
+ */
+int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
+                       int n_atoms, int max_atoms_per_sys, const float* pos,
+                       const float* charges, const int32_t* species, int n_species,
+                       const double* sigma, const double* epsilon, const double* lj_cutoff,
+                       const int32_t* lj_exclusion, const double* alpha,
+                       const double* coul_cutoff, const int32_t* coul_exclusion,
+                       const double* tril, int accumulate, float* forces, double* energy,
+                       double* virial);
+
+/*
+ * Reciprocal-space Ewald energy, forces, virial and structure factors.
+ * Replaces ewald_long_range_energy (potential/classical/ewald.py:585-601,
+ * structure factor 394-439, prefactor 363-391) and its gradient.
+ * k grid: n in [0..mx] x [-my..my] x [-mz..mz] (row-major, n0 slowest), kept
+ * where |k| <= kmax[s] (kmax NULL or < 0: all), k = 0 excluded
+ * (kvecs_from_kmax, ewald.py:1322-1339).  mx, my, mz <= 12.
+ * structure_factor (optional): n_sys x nk x 2 FP64 output S(k).
+ */
+int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
+                       int n_atoms, int max_atoms_per_sys, const float* pos,
+                       const float* charges, const double* tril, const double* alpha,
+                       const double* kmax, int mx, int my, int mz, int accumulate,
+                       float* forces, double* energy, double* virial,
+                       double* structure_factor);
+
+/*
+ * Elementwise integrator phases of the HBM-resident MD path
+ * (md/integrators.py:162-402): phase 0 = B(dt/2) A(dt) (integrator 0) or
+ * B(dt/2) A(dt/2) O A(dt/2) (integrator 1) and the fractional wrap; phase 1 =
+ * B(dt/2) plus the per-system kinetic energy; phase 2 = kinetic energy only.
+ * The O step draws the same Philox4x32-10 normals as kupsmi_md_lj.
+ */
+int kupsmi_integrate(void* stream, int phase, int integrator, int n_sys,
+                     const int32_t* sys_off, int max_atoms_per_sys, float* pos, float* mom,
+                     const float* forces, const float* mass, const double* tril,
+                     const double* dt, const double* kT, const double* gamma, uint64_t seed,
+                     int64_t step, double* kin_energy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,19 @@ _SIGNATURES = {
         C.c_int,
     ),
     "kupsmi_normals": ([vp, u64, i64, i32, vp], C.c_int),
+    "kupsmi_pair_forces": (
+        [vp, vp, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp,
+         vp, vp],
+        C.c_int,
+    ),
+    "kupsmi_ewald_recip": (
+        [vp, vp, i32, vp, i32, i32, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp],
+        C.c_int,
+    ),
+    "kupsmi_integrate": (
+        [vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64, vp],
+        C.c_int,
+    ),
 }
 
 

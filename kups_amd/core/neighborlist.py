@@ -62,6 +62,9 @@ class DenseNearestNeighborList:
             excl = torch.arange(lh.n, dtype=torch.int32, device=lh.positions.device)
         cut = torch.as_tensor(cutoffs, dtype=torch.float64, device=lh.positions.device)
         cut = cut.expand(n_sys).contiguous() if cut.dim() == 0 or cut.numel() == 1 else cut.contiguous()
+        # converted inputs are bound to names so they outlive the launches
+        pos = lh.positions.to(torch.float32).contiguous()
+        excl = excl.to(torch.int32).contiguous()
         count = torch.zeros(1, dtype=torch.int64, device=lh.positions.device)
         avg = self.avg_edges
         while True:
@@ -70,7 +73,7 @@ class DenseNearestNeighborList:
             sh = torch.empty((cap, 3), dtype=torch.int32, device=lh.positions.device)
             _lib.check(lib.kupsmi_neighborlist(
                 _lib.ctx(), _lib.stream_ptr(), n_sys, _lib.ptr(sys_off), lh.n,
-                _lib.ptr(lh.positions.contiguous()), _lib.ptr(excl.to(torch.int32).contiguous()),
+                _lib.ptr(pos), _lib.ptr(excl),
                 _lib.ptr(systems.tril), _lib.ptr(cut), cap, _lib.ptr(ij), _lib.ptr(sh),
                 _lib.ptr(count)))
             required = int(count.item())

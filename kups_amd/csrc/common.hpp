@@ -79,9 +79,12 @@ __host__ __device__ inline void tmul(const M* m, T x0, T x1, T x2, T& y0, T& y1,
   y2 = x2 * (T)m[5];
 }
 
-// ncell_cap bounds cells per system (LDS tables).
+// max_nc_axis bounds the cells per axis (LDS tables).  Cells are at least
+// rc / sub wide and the stencil spans +-sub cells, so every pair within rc is
+// visited; sub = 1 is the classic 27-cell stencil, sub = 2 the 125-cell one
+// that halves the candidates of dense, long-cutoff systems.
 __host__ __device__ inline void make_geom(const double* tril, double rc, int max_nc_axis,
-                                          SysGeom& g) {
+                                          SysGeom& g, int sub = 1) {
   for (int k = 0; k < 6; ++k) {
     g.L[k] = tril[k];
     g.Lf[k] = (float)tril[k];
@@ -115,11 +118,11 @@ __host__ __device__ inline void make_geom(const double* tril, double rc, int max
       g.nc[a] = 1;
       g.h[a] = (img[a] - 1) / 2;
     } else {
-      int n = isfinite(rc) ? (int)(perp[a] / rc) : 1;
+      int n = isfinite(rc) ? (int)(sub * perp[a] / rc) : 1;
       if (n < 1) n = 1;
       if (n > max_nc_axis) n = max_nc_axis;
       g.nc[a] = n;
-      g.h[a] = 1;
+      g.h[a] = sub;
     }
   }
   g.ncell = g.nc[0] * g.nc[1] * g.nc[2];
@@ -132,6 +135,24 @@ __device__ inline float wrap_half(float f) {
   u = u - floorf(u);
   if (u >= 1.0f) u = 0.0f;  // guard the rounding of u - floor(u) to 1
   return u - 0.5f;
+}
+
+// Wrap a Cartesian position into the primary cell by whole lattice vectors:
+// n = floor(f + 1/2) per axis and r' = r - n @ L (in FP64, rounded once), so a
+// position already inside the cell is kept bit for bit.  Returns the wrapped
+// fractional coordinates in f0..f2 (for cell indices).
+__device__ inline void wrap_shift(const double* L, const float* Lif, float& x, float& y, float& z,
+                                  float& f0, float& f1, float& f2) {
+  tmul(Lif, x, y, z, f0, f1, f2);
+  const float n0 = floorf(f0 + 0.5f), n1 = floorf(f1 + 0.5f), n2 = floorf(f2 + 0.5f);
+  f0 -= n0; f1 -= n1; f2 -= n2;
+  if (n0 != 0.f || n1 != 0.f || n2 != 0.f) {
+    double s0, s1, s2;
+    tmul(L, (double)n0, (double)n1, (double)n2, s0, s1, s2);
+    x = (float)((double)x - s0);
+    y = (float)((double)y - s1);
+    z = (float)((double)z - s2);
+  }
 }
 
 __device__ inline int cell_index(float fw, int n) {

@@ -214,6 +214,8 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
     const int total = a.n_steps > 0 ? a.n_steps : 1;
     for (int st = 0; st < total; ++st) {
       const bool last = st == total - 1;
+      // every force loop of the previous step has finished reading sp[]
+      __syncthreads();
       // ---- [B(dt/2) at st = 0] A(...) [O A] in place, wrap, cell of every atom
       int cel[MD_SLOTS];
 #pragma unroll
@@ -235,9 +237,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
           r.x += p.x * im * ta; r.y += p.y * im * ta; r.z += p.z * im * ta;
           if (a.integrator == 1) {
             float f0, f1, f2;
-            tmul(g.Lif, r.x, r.y, r.z, f0, f1, f2);
-            f0 = wrap_half(f0); f1 = wrap_half(f1); f2 = wrap_half(f2);
-            tmul(g.Lf, f0, f1, f2, r.x, r.y, r.z);
+            wrap_shift(g.L, g.Lif, r.x, r.y, r.z, f0, f1, f2);
             float e0, e1, e2;
             normals3(a.seed, (uint32_t)gi, a.step0 + st, e0, e1, e2);
             const float sm = c2 * sqrtf(1.0f / im);
@@ -249,9 +249,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
           pb[slot] = p;
         }
         float f0, f1, f2;
-        tmul(g.Lif, r.x, r.y, r.z, f0, f1, f2);
-        f0 = wrap_half(f0); f1 = wrap_half(f1); f2 = wrap_half(f2);
-        tmul(g.Lf, f0, f1, f2, r.x, r.y, r.z);
+        wrap_shift(g.L, g.Lif, r.x, r.y, r.z, f0, f1, f2);
         cel[k] = (cell_index(f2, ncz) * ncy + cell_index(f1, ncy)) * ncx + cell_index(f0, ncx);
         r.w = __int_as_float((__float_as_int(r.w) & 0xff) | (cel[k] << 8));
         sp[slot] = r;

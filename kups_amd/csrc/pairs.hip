@@ -1,0 +1,421 @@
+// General pair path: LJ + Ewald real-space energies, forces and virials of
+// systems of any size up to 16384 atoms, state in HBM, one launch per
+// evaluation.  Used by the Ewald MD step, by GCMC set-up and wherever a
+// system does not fit the LDS-resident MD kernel.
+//
+//   bin_kernel   one workgroup per system: wrap, counting sort of the atoms
+//                into cells at least rc/2 wide (stable: ties keep atom order),
+//                sorted copies (x, y, z, q | species | exclusion ids) to HBM.
+//   pair_kernel  one thread per sorted atom, 125-cell stencil read through
+//                L1/L2 (neighbouring lanes read the same candidates), pair
+//                vectors in FP64, potential arithmetic in FP32, FP64
+//                re-decision of the cutoff band, per-block FP64 partial
+//                energies and virials.
+//   reduce_kernel  fixed-order FP64 sum of the partials of each system.
+//
+// Reference: potential/classical/lennard_jones.py:149-178 (LJ per directed
+// edge, sum / 2), potential/classical/ewald.py:305-325 (real-space sum
+// K/2 sum q_i q_j erfc(alpha r)/r [r < rc]), neighbor semantics of
+// core/neighborlist.py:692-786 (strict cutoff, image grid for narrow cells,
+// exclusion on the minimum image only, 731-735).
+#include "common.hpp"
+
+namespace kupsmi {
+
+constexpr int PC_BIN_THREADS = 1024;
+constexpr int PC_NMAX = 16384;
+constexpr int PC_MAX_NC_AXIS = 16;
+constexpr int PC_NCMAX = PC_MAX_NC_AXIS * PC_MAX_NC_AXIS * PC_MAX_NC_AXIS;
+constexpr int PC_TILE = 256;
+constexpr int PC_NSPEC = 8;
+constexpr double PC_BAND64 = 1e-12;
+constexpr int PC_NPART = 8;  // e_lj, e_real, w00 w01 w02 w11 w12 w22
+
+struct PairArgs {
+  int n_sys, n_species, has_lj, has_coul, has_lj_excl, has_coul_excl, accumulate;
+  int tiles;
+  const int32_t* sys_off;
+  const float* pos;
+  const float* q;
+  const int32_t* species;
+  const int32_t* lj_excl;
+  const int32_t* cl_excl;
+  const double* sigma;
+  const double* epsilon;
+  const double* lj_rc;
+  const double* alpha;
+  const double* cl_rc;
+  const double* tril;
+  const SysGeom* geom;
+  // sorted copies
+  float4* spos;
+  int* sidx;
+  int* scell;
+  int* sspec;
+  int* slj;
+  int* scl;
+  int* cstart;
+  double* partial;
+  float* forces;
+  double* energy;
+  double* virial;
+};
+
+__global__ void pc_geom_kernel(PairArgs a) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n_sys) return;
+  double rc = 0.0;
+  if (a.has_lj) rc = a.lj_rc[s];
+  if (a.has_coul && a.cl_rc[s] > rc) rc = a.cl_rc[s];
+  make_geom(a.tril + 6 * s, rc, PC_MAX_NC_AXIS, const_cast<SysGeom*>(a.geom)[s], 2);
+}
+
+// Exclusive scan of cnt[0..n) into start[0..n], n <= 4 * blockDim.
+__device__ void block_scan4(const int* cnt, int* start, int n, int* wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x / 64;
+  int v4[4], own = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int c = tid * 4 + k;
+    v4[k] = c < n ? cnt[c] : 0;
+    own += v4[k];
+  }
+  int v = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) wsum[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    int w = lane < nw ? wsum[lane] : 0;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(w, o, 64);
+      if (lane >= o) w += y;
+    }
+    if (lane < nw) wsum[lane] = w;
+  }
+  __syncthreads();
+  int ex = v - own + (wv > 0 ? wsum[wv - 1] : 0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int c = tid * 4 + k;
+    if (c < n) start[c] = ex;
+    ex += v4[k];
+    if (c == n - 1) start[n] = ex;
+  }
+}
+
+__device__ inline int cell_of(const SysGeom& g, float x, float y, float z, float& wx, float& wy,
+                              float& wz) {
+  float f0, f1, f2;
+  wx = x; wy = y; wz = z;
+  wrap_shift(g.L, g.Lif, wx, wy, wz, f0, f1, f2);
+  return (cell_index(f2, g.nc[2]) * g.nc[1] + cell_index(f1, g.nc[1])) * g.nc[0] +
+         cell_index(f0, g.nc[0]);
+}
+
+__global__ __launch_bounds__(PC_BIN_THREADS) void pc_bin_kernel(PairArgs a) {
+  __shared__ int cnt[PC_NCMAX];
+  __shared__ int start[PC_NCMAX + 1];
+  __shared__ short keys[PC_NMAX];
+  __shared__ int wsum[PC_BIN_THREADS / 64];
+  __shared__ SysGeom g;
+  const int tid = threadIdx.x;
+  for (int s = blockIdx.x; s < a.n_sys; s += gridDim.x) {
+    const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
+    __syncthreads();
+    if (tid == 0) g = a.geom[s];
+    __syncthreads();
+    const int ncell = g.ncell;
+    int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
+    if (n <= 0 || n > PC_NMAX) {
+      for (int c = tid; c <= ncell; c += PC_BIN_THREADS) cs[c] = 0;
+      continue;
+    }
+    for (int c = tid; c < ncell; c += PC_BIN_THREADS) cnt[c] = 0;
+    __syncthreads();
+    float wx, wy, wz;
+    for (int i = tid; i < n; i += PC_BIN_THREADS) {
+      const float* r = a.pos + 3 * (size_t)(i0 + i);
+      atomicAdd(&cnt[cell_of(g, r[0], r[1], r[2], wx, wy, wz)], 1);
+    }
+    __syncthreads();
+    block_scan4(cnt, start, ncell, wsum);
+    __syncthreads();
+    for (int c = tid; c < ncell; c += PC_BIN_THREADS) cnt[c] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += PC_BIN_THREADS) {
+      const float* r = a.pos + 3 * (size_t)(i0 + i);
+      const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
+      keys[start[c] + atomicAdd(&cnt[c], 1)] = (short)i;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += PC_BIN_THREADS) {
+      const float* r = a.pos + 3 * (size_t)(i0 + i);
+      const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
+      const int b = start[c], e = start[c + 1];
+      int rank = 0;
+      for (int t = b; t < e; ++t) rank += (int)(unsigned short)keys[t] < i;
+      const int d = i0 + b + rank;
+      a.spos[d] = make_float4(wx, wy, wz, a.q ? a.q[i0 + i] : 0.0f);
+      a.sidx[d] = i;
+      a.scell[d] = c;
+      a.sspec[d] = a.species ? a.species[i0 + i] : 0;
+      if (a.has_lj_excl) a.slj[d] = a.lj_excl[i0 + i];
+      if (a.has_coul_excl) a.scl[d] = a.cl_excl[i0 + i];
+    }
+    for (int c = tid; c <= ncell; c += PC_BIN_THREADS) cs[c] = start[c];
+  }
+}
+
+// FP64 re-decision |(f_i - f_j - s) @ L|^2 < rc2 (core/neighborlist.py:542-564).
+__device__ inline bool keep64(const SysGeom& g, float4 ri, float4 rj, int sx, int sy, int sz,
+                              double rc2) {
+  asm volatile("" ::: "memory");
+  double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
+  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
+  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
+  tmul(g.L, fi0 - fj0 - sx, fi1 - fj1 - sy, fi2 - fj2 - sz, d0, d1, d2);
+  return d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+}
+
+// Minimum-image test of a replicated system's pair: s == round(f_i - f_j).
+__device__ inline bool is_min_image(const SysGeom& g, float4 ri, float4 rj, int sx, int sy,
+                                    int sz) {
+  double fi0, fi1, fi2, fj0, fj1, fj2;
+  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
+  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
+  return rint(fi0 - fj0) == sx && rint(fi1 - fj1) == sy && rint(fi2 - fj2) == sz;
+}
+
+template <bool EXCL>
+__global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
+  __shared__ SysGeom g;
+  __shared__ float s_sig2[PC_NSPEC * PC_NSPEC], s_e4[PC_NSPEC * PC_NSPEC],
+      s_e24[PC_NSPEC * PC_NSPEC];
+  __shared__ double red[PC_TILE / 64][PC_NPART];
+  const int s = blockIdx.y;
+  const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
+  const int start = blockIdx.x * PC_TILE;
+  const int tid = threadIdx.x;
+  double* part = a.partial + ((size_t)s * a.tiles + blockIdx.x) * PC_NPART;
+  if (start >= n || n > PC_NMAX) {
+    if (tid < PC_NPART) part[tid] = 0.0;
+    return;
+  }
+  if (tid == 0) g = a.geom[s];
+  const int ns = a.n_species;
+  for (int k = tid; k < ns * ns; k += PC_TILE) {
+    if (a.has_lj) {
+      const double sg = a.sigma[k];
+      s_sig2[k] = (float)(sg * sg);
+      s_e4[k] = (float)(4.0 * a.epsilon[k]);
+      s_e24[k] = (float)(24.0 * a.epsilon[k]);
+    }
+  }
+  __syncthreads();
+  const double lj_rc2 = a.has_lj ? a.lj_rc[s] * a.lj_rc[s] : 0.0;
+  const double cl_rc = a.has_coul ? a.cl_rc[s] : 0.0;
+  const double cl_rc2 = cl_rc * cl_rc;
+  // FP64 band around each cutoff re-decided with the reference's fractional
+  // formula (the Cartesian FP64 r^2 differs from it by a few ulp)
+  const double lj_hi = lj_rc2 * (1.0 + PC_BAND64), lj_lo = lj_rc2 * (1.0 - PC_BAND64);
+  const double cl_hi = cl_rc2 * (1.0 + PC_BAND64), cl_lo = cl_rc2 * (1.0 - PC_BAND64);
+  const double all_hi = fmax(lj_hi, cl_hi);
+  const float alpha = a.has_coul ? (float)a.alpha[s] : 0.0f;
+  const float K = 14.399645351950548f;  // HARTREE * BOHR, eV A
+  const float two_a_sqrtpi = 2.0f * alpha * 0.5641895835477563f;
+  const float alpha2 = alpha * alpha;
+  const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
+  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+
+  const int slot = start + tid;
+  float ax = 0.f, ay = 0.f, az = 0.f, elj = 0.f, ecl = 0.f;
+  float w00 = 0.f, w01 = 0.f, w02 = 0.f, w11 = 0.f, w12 = 0.f, w22 = 0.f;
+  if (slot < n) {
+    const float4 ri = a.spos[i0 + slot];
+    const int c = a.scell[i0 + slot];
+    const int si = a.sspec[i0 + slot];
+    const int xlj = EXCL && a.has_lj_excl ? a.slj[i0 + slot] : 0;
+    const int xcl = EXCL && a.has_coul_excl ? a.scl[i0 + slot] : 0;
+    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
+      int zz = cz + dz;
+      const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
+      zz -= sz * ncz;
+      for (int dy = -g.h[1]; dy <= g.h[1]; ++dy) {
+        int yy = cy + dy;
+        const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
+        yy -= sy * ncy;
+        for (int dx = -g.h[0]; dx <= g.h[0]; ++dx) {
+          int xx = cx + dx;
+          const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
+          xx -= sx * ncx;
+          // pair vectors are formed in FP64 (same VALU rate as unpacked FP32 on
+          // gfx950): |r| up to half a box would otherwise cost ~1e-5 relative
+          double Sx, Sy, Sz;
+          tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
+          const double qx = (double)ri.x - Sx, qy = (double)ri.y - Sy, qz = (double)ri.z - Sz;
+          const bool center = dx == 0 && dy == 0 && dz == 0;
+          const int nb = (zz * ncy + yy) * ncx + xx;
+          const int tb = cs[nb], te = cs[nb + 1];
+          for (int t = tb; t < te; ++t) {
+            const float4 rj = a.spos[i0 + t];
+            const double exd = (double)rj.x - qx, eyd = (double)rj.y - qy, ezd = (double)rj.z - qz;
+            const double r2d = exd * exd + eyd * eyd + ezd * ezd;
+            if (!(r2d < all_hi)) continue;
+            const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
+            bool min_img = true;
+            if (EXCL && g.rep) min_img = is_min_image(g, ri, rj, sx, sy, sz);
+            // particle exclusion: the zero-shift self pair
+            const bool self = center && t == slot;
+            float gsum = 0.f;  // d(E_directed)/d(r^2) * 2
+            if (a.has_lj && r2d < lj_hi && !self) {
+              bool ok = !(r2d > lj_lo) || keep64(g, ri, rj, sx, sy, sz, lj_rc2);
+              if (EXCL && a.has_lj_excl && min_img && a.slj[i0 + t] == xlj) ok = false;
+              if (ok) {
+                const int p = si * ns + a.sspec[i0 + t];
+                const float ir2 = 1.0f / r2;
+                float c6 = s_sig2[p] * ir2;
+                c6 = c6 * c6 * c6;
+                elj += s_e4[p] * c6 * (c6 - 1.0f);
+                gsum += s_e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;
+              }
+            }
+            if (a.has_coul && r2d < cl_hi && !self) {
+              bool ok = !(r2d > cl_lo) || keep64(g, ri, rj, sx, sy, sz, cl_rc2);
+              if (EXCL && a.has_coul_excl && min_img && a.scl[i0 + t] == xcl) ok = false;
+              if (ok) {
+                const float r = sqrtf(r2);
+                const float ir = 1.0f / r;
+                const float qq = ri.w * rj.w * K;
+                const float ec = erfcf(alpha * r);
+                ecl += qq * ec * ir;
+                // dU/dr = qq (-2a/sqrt(pi) e^{-a^2 r^2} / r - erfc / r^2); 2 u' = dU/dr / r
+                gsum += qq * (-two_a_sqrtpi * __expf(-alpha2 * r2) - ec * ir) * ir * ir;
+              }
+            }
+            ax += gsum * ex; ay += gsum * ey; az += gsum * ez;
+            const float hg = -0.5f * gsum;
+            w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
+            w11 += hg * ey * ey; w12 += hg * ey * ez; w22 += hg * ez * ez;
+          }
+        }
+      }
+    }
+    const int gi = i0 + a.sidx[i0 + slot];
+    if (a.accumulate) {
+      a.forces[3 * gi] += ax; a.forces[3 * gi + 1] += ay; a.forces[3 * gi + 2] += az;
+    } else {
+      a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
+    }
+  }
+  // per-block FP64 partials (fixed order: wave sums, then waves in order)
+  const double v[PC_NPART] = {0.5 * elj, 0.5 * ecl, w00, w01, w02, w11, w12, w22};
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < PC_NPART; ++k) {
+    double x = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = x;
+  }
+  __syncthreads();
+  if (tid < PC_NPART) {
+    double x = 0.0;
+    for (int w = 0; w < PC_TILE / 64; ++w) x += red[w][tid];
+    part[tid] = x;
+  }
+}
+
+__global__ void pc_reduce_kernel(PairArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n_sys) return;
+  double acc[PC_NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = 0; t < a.tiles; ++t)
+    for (int k = 0; k < PC_NPART; ++k) acc[k] += a.partial[((size_t)s * a.tiles + t) * PC_NPART + k];
+  if (a.energy) {
+    double* e = a.energy + 2 * (size_t)s;
+    if (a.accumulate) { e[0] += acc[0]; e[1] += acc[1]; } else { e[0] = acc[0]; e[1] = acc[1]; }
+  }
+  if (a.virial) {
+    const double w[9] = {acc[2], acc[3], acc[4], acc[3], acc[5], acc[6], acc[4], acc[6], acc[7]};
+    double* W = a.virial + 9 * (size_t)s;
+    for (int k = 0; k < 9; ++k) W[k] = a.accumulate ? W[k] + w[k] : w[k];
+  }
+}
+
+}  // namespace kupsmi
+
+using namespace kupsmi;
+
+extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
+                                  int n_atoms, int max_atoms_per_sys, const float* pos,
+                                  const float* charges, const int32_t* species, int n_species,
+                                  const double* sigma, const double* epsilon,
+                                  const double* lj_cutoff, const int32_t* lj_exclusion,
+                                  const double* alpha, const double* coul_cutoff,
+                                  const int32_t* coul_exclusion, const double* tril,
+                                  int accumulate, float* forces, double* energy, double* virial) {
+  KUPSMI_REQUIRE(ctx != nullptr, "null context");
+  KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0, "negative size");
+  KUPSMI_REQUIRE(max_atoms_per_sys <= PC_NMAX, "systems of the pair path hold at most 16384 atoms");
+  KUPSMI_REQUIRE(n_species >= 1 && n_species <= PC_NSPEC, "n_species must be in [1, 8]");
+  const bool has_lj = sigma && epsilon && lj_cutoff;
+  const bool has_cl = alpha && coul_cutoff && charges;
+  KUPSMI_REQUIRE(has_lj || has_cl, "nothing to compute: pass LJ tables or Ewald parameters");
+  KUPSMI_REQUIRE(forces != nullptr, "forces output required");
+  if (n_sys == 0 || n_atoms == 0) return KUPSMI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  PairArgs a{};
+  a.n_sys = n_sys; a.n_species = n_species; a.has_lj = has_lj; a.has_coul = has_cl;
+  a.has_lj_excl = lj_exclusion != nullptr; a.has_coul_excl = coul_exclusion != nullptr;
+  a.accumulate = accumulate;
+  a.tiles = (max_atoms_per_sys + PC_TILE - 1) / PC_TILE;
+  if (a.tiles < 1) a.tiles = 1;
+  a.sys_off = sys_off; a.pos = pos; a.q = charges; a.species = species;
+  a.lj_excl = lj_exclusion; a.cl_excl = coul_exclusion; a.sigma = sigma; a.epsilon = epsilon;
+  a.lj_rc = lj_cutoff; a.alpha = alpha; a.cl_rc = coul_cutoff; a.tril = tril;
+  a.forces = forces; a.energy = energy; a.virial = virial;
+  // scratch layout (slot 3)
+  const size_t na = (size_t)n_atoms;
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+  const size_t o_geom = take(sizeof(SysGeom) * n_sys);
+  const size_t o_spos = take(sizeof(float4) * na);
+  const size_t o_sidx = take(sizeof(int) * na);
+  const size_t o_scell = take(sizeof(int) * na);
+  const size_t o_sspec = take(sizeof(int) * na);
+  const size_t o_slj = take(sizeof(int) * na);
+  const size_t o_scl = take(sizeof(int) * na);
+  const size_t o_cs = take(sizeof(int) * (size_t)n_sys * (PC_NCMAX + 1));
+  const size_t o_part = take(sizeof(double) * (size_t)n_sys * a.tiles * PC_NPART);
+  void* buf = nullptr;
+  int rc = scratch(ctx, 3, off, &buf);
+  if (rc) return rc;
+  char* b = static_cast<char*>(buf);
+  a.geom = reinterpret_cast<SysGeom*>(b + o_geom);
+  a.spos = reinterpret_cast<float4*>(b + o_spos);
+  a.sidx = reinterpret_cast<int*>(b + o_sidx);
+  a.scell = reinterpret_cast<int*>(b + o_scell);
+  a.sspec = reinterpret_cast<int*>(b + o_sspec);
+  a.slj = reinterpret_cast<int*>(b + o_slj);
+  a.scl = reinterpret_cast<int*>(b + o_scl);
+  a.cstart = reinterpret_cast<int*>(b + o_cs);
+  a.partial = reinterpret_cast<double*>(b + o_part);
+  pc_geom_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  int dev = 0, ncu = 256;
+  KUPSMI_HIP(hipGetDevice(&dev));
+  KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  pc_bin_kernel<<<n_sys < ncu ? n_sys : ncu, PC_BIN_THREADS, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  dim3 grid(a.tiles, n_sys);
+  if (a.has_lj_excl || a.has_coul_excl)
+    pc_pair_kernel<true><<<grid, PC_TILE, 0, st>>>(a);
+  else
+    pc_pair_kernel<false><<<grid, PC_TILE, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  pc_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  return KUPSMI_OK;
+}

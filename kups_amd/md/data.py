@@ -34,6 +34,7 @@ class MDState:
     temperature: torch.Tensor  # (n_sys,) f64, K
     friction: torch.Tensor  # (n_sys,) f64, 1 / internal time
     step: int = 0
+    charges: torch.Tensor | None = None  # (n,) f32, for Ewald systems
     extras: dict = field(default_factory=dict)
 
     @property
@@ -65,7 +66,8 @@ def maxwell_boltzmann_momenta(masses: np.ndarray, sizes, temperature, rng: np.ra
 
 
 def make_md_state(positions, masses, species, sizes, tril, cutoff, *, time_step_fs,
-                  temperature, friction_per_fs=0.0, momenta=None, device=None) -> MDState:
+                  temperature, friction_per_fs=0.0, momenta=None, charges=None,
+                  device=None) -> MDState:
     dev = device or require_gpu()
     n_sys = len(sizes)
     n = int(np.sum(sizes))
@@ -89,4 +91,6 @@ def make_md_state(positions, masses, species, sizes, tril, cutoff, *, time_step_
         time_step=per_sys(np.asarray(time_step_fs, np.float64) * FEMTO_SECOND),
         temperature=per_sys(temperature),
         friction=per_sys(np.asarray(friction_per_fs, np.float64) / FEMTO_SECOND),
+        charges=None if charges is None else
+        torch.as_tensor(np.asarray(charges, np.float32), device=dev).contiguous(),
     )

@@ -95,3 +95,97 @@ def normals(seed: int, step: int, n_atoms: int, device=None) -> torch.Tensor:
     _lib.check(_lib.load().kupsmi_normals(_lib.stream_ptr(), int(seed) & (2**64 - 1), int(step),
                                           n_atoms, _lib.ptr(out)))
     return out
+
+
+@dataclass
+class EwaldObservables:
+    lj: torch.Tensor          # (n_sys, n_steps)
+    short_range: torch.Tensor
+    long_range: torch.Tensor
+    self_energy: torch.Tensor  # (n_sys,)
+    kinetic_energy: torch.Tensor
+
+    @property
+    def potential_energy(self) -> torch.Tensor:
+        return self.lj + self.short_range + self.long_range + self.self_energy[:, None]
+
+
+class EwaldLJPropagator:
+    """MD of charged LJ systems (LJ + Ewald) on the HBM-resident path, one
+    force evaluation per step: integrate(phase 0) -> kupsmi_pair_forces (LJ and
+    real-space Ewald, one pair loop) -> kupsmi_ewald_recip (k-space, added) ->
+    integrate(phase 1).  Same step order as the fused LJ kernel."""
+
+    def __init__(self, params: LennardJonesParameters, ewald, integrator: int = BAOAB_LANGEVIN,
+                 seed: int = 0):
+        self.params = params
+        self.ewald = ewald
+        self.integrator = integrator
+        self.seed = seed
+        self._dev = None
+
+    def _prep(self, state: MDState):
+        dev = state.positions.device
+        if self._dev != dev:
+            self._sig = torch.as_tensor(self.params.sigma, dtype=torch.float64, device=dev).contiguous()
+            self._eps = torch.as_tensor(self.params.epsilon, dtype=torch.float64, device=dev).contiguous()
+            self._alpha = torch.as_tensor(self.ewald.alpha, dtype=torch.float64, device=dev).contiguous()
+            self._rc = torch.as_tensor(self.ewald.cutoff, dtype=torch.float64, device=dev).contiguous()
+            self._kmax = torch.as_tensor(self.ewald.kmax, dtype=torch.float64, device=dev).contiguous()
+            self._dev = dev
+        self._maxn = int(state.sizes.max()) if state.sizes.size else 0
+
+    def forces(self, state: MDState, e2, elr, vir=None):
+        lib = _lib.load()
+        _lib.check(lib.kupsmi_pair_forces(
+            _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
+            _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.species),
+            self.params.n_species, _lib.ptr(self._sig), _lib.ptr(self._eps), _lib.ptr(state.cutoff),
+            None, _lib.ptr(self._alpha), _lib.ptr(self._rc), None, _lib.ptr(state.tril), 0,
+            _lib.ptr(state.forces), _lib.ptr(e2), _lib.ptr(vir)))
+        mx, my, mz = self.ewald.grid
+        _lib.check(lib.kupsmi_ewald_recip(
+            _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
+            _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.tril),
+            _lib.ptr(self._alpha), _lib.ptr(self._kmax), mx, my, mz, 1, _lib.ptr(state.forces),
+            _lib.ptr(elr), _lib.ptr(vir), None))
+
+    def evaluate_forces(self, state: MDState):
+        self._prep(state)
+        dev = state.positions.device
+        e2 = torch.zeros((state.n_sys, 2), dtype=torch.float64, device=dev)
+        elr = torch.zeros(state.n_sys, dtype=torch.float64, device=dev)
+        vir = torch.zeros((state.n_sys, 3, 3), dtype=torch.float64, device=dev)
+        self.forces(state, e2, elr, vir)
+        return e2, elr, vir
+
+    def __call__(self, state: MDState, n_steps: int = 1) -> EwaldObservables:
+        from ..potential.ewald import self_energy
+
+        self._prep(state)
+        dev = state.positions.device
+        lib = _lib.load()
+        e2 = torch.zeros((n_steps, state.n_sys, 2), dtype=torch.float64, device=dev)
+        elr = torch.zeros((n_steps, state.n_sys), dtype=torch.float64, device=dev)
+        kin = torch.zeros((n_steps, state.n_sys), dtype=torch.float64, device=dev)
+        kT = state.kT.contiguous()
+        seed = int(self.seed) & (2**64 - 1)
+        for k in range(n_steps):
+            _lib.check(lib.kupsmi_integrate(
+                _lib.stream_ptr(), 0, int(self.integrator), state.n_sys, _lib.ptr(state.sys_off),
+                self._maxn, _lib.ptr(state.positions), _lib.ptr(state.momenta),
+                _lib.ptr(state.forces), _lib.ptr(state.masses), _lib.ptr(state.tril),
+                _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction), seed,
+                int(state.step), None))
+            self.forces(state, e2[k], elr[k])
+            _lib.check(lib.kupsmi_integrate(
+                _lib.stream_ptr(), 1, int(self.integrator), state.n_sys, _lib.ptr(state.sys_off),
+                self._maxn, _lib.ptr(state.positions), _lib.ptr(state.momenta),
+                _lib.ptr(state.forces), _lib.ptr(state.masses), _lib.ptr(state.tril),
+                _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction), seed,
+                int(state.step), _lib.ptr(kin[k])))
+            state.step += 1
+        system = torch.repeat_interleave(torch.arange(state.n_sys, device=dev),
+                                         torch.as_tensor(state.sizes, device=dev))
+        es = self_energy(state.charges, system, self._alpha, state.n_sys)
+        return EwaldObservables(e2[:, :, 0].T, e2[:, :, 1].T, elr.T, es, kin.T)

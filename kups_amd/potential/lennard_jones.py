@@ -106,12 +106,16 @@ class LennardJonesPotential:
         species = particles.species
         if species is None:
             species = torch.zeros(particles.n, dtype=torch.int32, device=dev)
+        # converted inputs are bound to names so they outlive the launch
+        pos = particles.positions.to(torch.float32).contiguous()
+        species = species.to(torch.int32).contiguous()
+        cutoffs = torch.as_tensor(cutoffs, dtype=torch.float64, device=dev).contiguous()
         forces = torch.empty((particles.n, 3), dtype=torch.float32, device=dev)
         energy = torch.empty(n_sys, dtype=torch.float64, device=dev)
         virial = torch.empty((n_sys, 3, 3), dtype=torch.float64, device=dev)
         _lib.check(_lib.load().kupsmi_lj_forces(
             _lib.ctx(), _lib.stream_ptr(), n_sys, _lib.ptr(sys_off), particles.n,
-            _lib.ptr(particles.positions.contiguous()), _lib.ptr(species.to(torch.int32).contiguous()),
+            _lib.ptr(pos), _lib.ptr(species),
             self.params.n_species, _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(systems.tril),
-            _lib.ptr(cutoffs.contiguous()), _lib.ptr(forces), _lib.ptr(energy), _lib.ptr(virial)))
+            _lib.ptr(cutoffs), _lib.ptr(forces), _lib.ptr(energy), _lib.ptr(virial)))
         return PotentialOut(energy, -forces, virial)

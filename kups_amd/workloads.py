@@ -100,3 +100,31 @@ def batched_lj_boxes(n_sys: int = 2048, nmin: int = 64, nmax: int = 4096, seed: 
                  np.array(trils).reshape(-1, 6), np.zeros(n, np.int32), np.full(n, AR_MASS),
                  description=f"{len(sizes)} LJ boxes, N in [{sizes.min() if n else 0},"
                              f"{sizes.max() if n else 0}], rho*={density}, jitter {jitter} sigma")
+
+
+NA_MASS, CL_MASS = 22.98977, 35.453
+
+
+def nacl_melts(n_sys: int = 512, mmin: int = 5, mmax: int = 10, seed: int = 0, a: float = 5.64,
+               jitter: float = 0.1, ms=None) -> Batch:
+    """Charged LJ melts: rock-salt lattices of N = 8 m^3 ions (m ~ U{mmin..mmax}),
+    lattice constant a, Gaussian jitter (A), charges +1 (species 0) / -1
+    (species 1) alternating on the lattice (neutral)."""
+    rng = np.random.default_rng(seed)
+    if ms is None:
+        ms = rng.integers(mmin, mmax + 1, size=n_sys)
+    pos_all, q_all, sp_all, trils = [], [], [], []
+    for m in ms:
+        g = np.stack(np.meshgrid(*[np.arange(2 * m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        sp = (g.sum(-1) % 2).astype(np.int32)
+        L = m * a
+        p = g * (a / 2) + rng.normal(0.0, jitter, size=g.shape)
+        pos_all.append(wrap_cubic(p, L))
+        sp_all.append(sp)
+        q_all.append(np.where(sp == 0, 1.0, -1.0))
+        trils.append(cubic_tril(L)[0])
+    sizes = np.array([8 * m**3 for m in ms], np.int64)
+    sp = np.concatenate(sp_all)
+    return Batch(np.concatenate(pos_all), sizes, np.array(trils), sp,
+                 np.where(sp == 0, NA_MASS, CL_MASS), charges=np.concatenate(q_all),
+                 description=f"{len(ms)} NaCl-like melts, N=8m^3, m in [{min(ms)},{max(ms)}]")

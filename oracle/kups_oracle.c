@@ -710,3 +710,332 @@ int orc_num_threads(void) {
   return 1;
 #endif
 }
+
+/* ================================================================ GCMC == */
+/*
+ * Rigid-molecule GCMC of a batch of chains (application/simulations/
+ * mcmc_rigid.py:281-307 propagator; mcmc/moves.py:122-295, 573-835,
+ * 1078-1230 moves; mcmc/probability.py:60-225 acceptance;
+ * core/propagator.py:244-344 step; core/parameter_scheduler.py:161-208
+ * step-width adaptation).  FP64 restatement of the GPU kernel's algorithm,
+ * drawing the same counter-based random numbers, so accept sequences can be
+ * compared trial by trial.
+ *
+ * Random numbers (shared with the GPU kernel; the reference draws from
+ * jax.random, which cannot be reproduced bit for bit):
+ *   batch draw  philox({0xFFFFFFFF, t, t>>32, TAG_BATCH}, seed): u32[0] picks
+ *               the move type by the cumulative weights, u32[1] >> 31 the
+ *               insert (0) / delete (1) coin (one move type and one coin for
+ *               every chain, core/propagator.py:263-268, moves.py:1217);
+ *   chain draw  philox({chain, t, t>>32, TAG_A}): [group bits, accept u,
+ *               spare, spare]; TAG_B: quaternion uniforms; TAG_C: offsets or
+ *               the translation normals (Box-Muller).
+ * Float conversions of the uniforms are done in float, as on the device.
+ */
+#define GC_TAG_BATCH 0x42415443u
+#define GC_TAG_A 0x47434d41u
+#define GC_TAG_B 0x47434d42u
+#define GC_TAG_C 0x47434d43u
+
+static void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += W0; k1 += W1;
+  }
+}
+static float gc_u01(uint32_t x) { return ((float)x + 0.5f) * 2.3283064365386963e-10f; }
+static void gc_draw(uint64_t seed, uint32_t a, int64_t t, uint32_t tag, uint32_t out[4]) {
+  out[0] = a; out[1] = (uint32_t)t; out[2] = (uint32_t)((uint64_t)t >> 32); out[3] = tag;
+  philox10(out, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+void orc_philox(uint64_t seed, uint32_t a, int64_t t, uint32_t tag, uint32_t* out) {
+  gc_draw(seed, a, t, tag, out);
+}
+
+/* Shoemake random quaternion (core/utils/quaternion.py:70-97), the power q^w
+ * (_pow_quaternion: sign fix w >= 0, angle scaled) and the rotation matrix
+ * (_quat_to_mat), applied as row-vector @ R (moves.py:190). */
+static void gc_quat(const float u[3], double w, double R[3][3]) {
+  double u1 = u[0], u2 = u[1], u3 = u[2];
+  double q[4] = {sqrt(1 - u1) * sin(2 * M_PI * u2), sqrt(1 - u1) * cos(2 * M_PI * u2),
+                 sqrt(u1) * sin(2 * M_PI * u3), sqrt(u1) * cos(2 * M_PI * u3)};
+  if (q[0] < 0) for (int k = 0; k < 4; ++k) q[k] = -q[k];
+  double cw = q[0] > 1 ? 1 : (q[0] < -1 ? -1 : q[0]);
+  double ang = 2 * acos(cw);
+  double sn = sqrt(q[1]*q[1] + q[2]*q[2] + q[3]*q[3]);
+  double ax[3] = {1, 0, 0};
+  if (sn > 1e-10) { ax[0] = q[1]/sn; ax[1] = q[2]/sn; ax[2] = q[3]/sn; }
+  double na = ang * w;
+  double W = cos(na / 2), X = ax[0]*sin(na/2), Y = ax[1]*sin(na/2), Z = ax[2]*sin(na/2);
+  double x2 = X*X, y2 = Y*Y, z2 = Z*Z, w2 = W*W;
+  R[0][0] = x2 - y2 - z2 + w2; R[0][1] = 2*(X*Y - Z*W); R[0][2] = 2*(X*Z + Y*W);
+  R[1][0] = 2*(X*Y + Z*W); R[1][1] = -x2 + y2 - z2 + w2; R[1][2] = 2*(Y*Z - X*W);
+  R[2][0] = 2*(X*Z - Y*W); R[2][1] = 2*(Y*Z + X*W); R[2][2] = -x2 - y2 + z2 + w2;
+}
+
+static void wrap_f(const cell_t* c, double* r) { wrap_pos(c, r); }
+
+/* minimum-image difference vector b - a (round of fractional components) */
+static void min_image(const cell_t* c, const double* a, const double* b, double* d) {
+  double dd[3] = {b[0]-a[0], b[1]-a[1], b[2]-a[2]}, f[3];
+  rmul(c->Li, dd, f);
+  for (int k = 0; k < 3; ++k) f[k] -= rint(f[k]);
+  rmul(c->L, f, d);
+}
+
+typedef struct {
+  /* sizes */
+  int n_chain, n_steps, gmax, m, nspec, mx, my, mz;
+  int64_t step0;
+  uint64_t seed;
+  /* hosts */
+  const int32_t* host_off; const double* host_pos; const double* host_q; const int32_t* host_spec;
+  const double* tril;
+  /* motif (template centred at its geometric centre) */
+  const double* motif_pos; const double* motif_q; const int32_t* motif_spec;
+  /* potentials */
+  const double* sig; const double* eps; double rc_lj; double rc_cl;
+  const double* alpha; const double* kmax; const double* tail; /* n_chain x nspec x nspec */
+  const double* mol_const; /* n_chain: E_self + E_excl of one molecule */
+  const double* kT; const double* log_activity;
+  double w_move[4]; /* translation, rotation, reinsertion, exchange */
+  double upper[4];  /* step-width upper bounds */
+  /* state (in/out) */
+  double* guest_pos; int32_t* n_guest; int32_t* counts; /* n_chain x nspec */
+  double* sk; double* energy;
+  float* width; uint32_t* hist; int32_t* hist_idx; /* n_chain x 4 (x 4 words) */
+  int64_t* stats; /* n_chain x 8: trials[4], accepted[4] */
+  int64_t* ins_acc; int64_t* del_acc;
+  /* optional per-trial trace of chain 0..n_trace-1 */
+  int n_trace; double* trace; /* n_trace x n_steps x 4: move, dE, logp, accepted */
+} gc_args;
+
+static double gc_pair(const gc_args* a, const cell_t* c, int ch, const double* r, double q, int sp,
+                      const double* rj, double qj, int spj) {
+  double d[3]; min_image(c, r, rj, d);
+  double r2 = d[0]*d[0] + d[1]*d[1] + d[2]*d[2];
+  double e = 0;
+  int p = sp * a->nspec + spj;
+  if (r2 < a->rc_lj * a->rc_lj && a->eps[p] != 0) {
+    double c6 = pow(a->sig[p]*a->sig[p]/r2, 3);
+    e += 4*a->eps[p]*(c6*c6 - c6);
+  }
+  double rr = sqrt(r2);
+  if (rr < a->rc_cl) e += coulomb_k() * q * qj * erfc(a->alpha[ch]*rr) / rr;
+  return e;
+}
+
+static void gc_grid_n(const gc_args* a, int k, int* n0, int* n1, int* n2) {
+  int wy = 2*a->my + 1, wz = 2*a->mz + 1;
+  *n0 = k / (wy*wz); *n1 = (k / wz) % wy - a->my; *n2 = k % wz - a->mz;
+}
+
+int orc_gcmc_run(gc_args* a) {
+  const double K = coulomb_k();
+  const int nk = (a->mx + 1) * (2*a->my + 1) * (2*a->mz + 1);
+  const int M = a->m;
+#pragma omp parallel for schedule(dynamic)
+  for (int ch = 0; ch < a->n_chain; ++ch) {
+    cell_t c; cell_from_tril(a->tril + 6*ch, &c);
+    double* P = (double*)malloc(sizeof(double) * nk);
+    double (*kv)[3] = malloc(sizeof(double) * 3 * nk);
+    double* dS = (double*)malloc(sizeof(double) * 2 * nk);
+    for (int k = 0; k < nk; ++k) {
+      int n0, n1, n2; gc_grid_n(a, k, &n0, &n1, &n2);
+      int n[3] = {n0, n1, n2};
+      for (int x = 0; x < 3; ++x)
+        kv[k][x] = 2*M_PI*(c.Li[x][0]*n[0] + c.Li[x][1]*n[1] + c.Li[x][2]*n[2]);
+      double k2 = kv[k][0]*kv[k][0] + kv[k][1]*kv[k][1] + kv[k][2]*kv[k][2];
+      double al = a->alpha[ch];
+      int in = a->kmax[ch] < 0 || sqrt(k2) <= a->kmax[ch];
+      P[k] = (k2 > 0 && in) ? 2*M_PI/c.vol*exp(-k2/(4*al*al))/k2*(n0 == 0 ? 1.0 : 2.0) : 0.0;
+    }
+    const int h0 = a->host_off[ch], h1 = a->host_off[ch+1];
+    double* gp = a->guest_pos + (size_t)ch * a->gmax * M * 3;
+    double* S = a->sk + (size_t)ch * nk * 2;
+    int32_t* cnt = a->counts + (size_t)ch * a->nspec;
+    const double* T = a->tail + (size_t)ch * a->nspec * a->nspec;
+    for (int st = 0; st < a->n_steps; ++st) {
+      int64_t t = a->step0 + st;
+      uint32_t bb[4]; gc_draw(a->seed, 0xFFFFFFFFu, t, GC_TAG_BATCH, bb);
+      double wsum = a->w_move[0] + a->w_move[1] + a->w_move[2] + a->w_move[3];
+      double um = gc_u01(bb[0]) * wsum, acc_w = 0;
+      int move = 3;
+      for (int k = 0; k < 4; ++k) { acc_w += a->w_move[k]; if (um < acc_w) { move = k; break; } }
+      int deleting = (bb[1] >> 31) & 1;
+      uint32_t ra[4], rb[4], rcn[4];
+      gc_draw(a->seed, (uint32_t)ch, t, GC_TAG_A, ra);
+      gc_draw(a->seed, (uint32_t)ch, t, GC_TAG_B, rb);
+      gc_draw(a->seed, (uint32_t)ch, t, GC_TAG_C, rcn);
+      int ng = a->n_guest[ch];
+      /* old / new sites */
+      double oldp[8][3], newp[8][3];
+      int n_old = 0, n_new = 0, slot = -1;
+      float* wid = a->width + (size_t)ch * 4;
+      if (move < 3 || (move == 3 && deleting)) {
+        if (ng > 0) {
+          slot = (int)(ra[0] % (uint32_t)ng);
+          n_old = M;
+          for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) oldp[s][x] = gp[(slot*M + s)*3 + x];
+        }
+      }
+      if (move == 0 && n_old) {
+        /* translation N(0,1) * width per axis (moves.py:222-244) */
+        float u0 = gc_u01(rcn[0]), u1 = gc_u01(rcn[1]), u2 = gc_u01(rcn[2]), u3 = gc_u01(rcn[3]);
+        double r0 = sqrt(-2.0*log((double)u0)), r1 = sqrt(-2.0*log((double)u2));
+        double g3[3] = {r0*cos(2*M_PI*u1), r0*sin(2*M_PI*u1), r1*cos(2*M_PI*u3)};
+        for (int s = 0; s < M; ++s) {
+          for (int x = 0; x < 3; ++x) newp[s][x] = oldp[s][x] + g3[x] * wid[0];
+          wrap_f(&c, newp[s]);
+        }
+        n_new = M;
+      } else if ((move == 1 || move == 2) && n_old) {
+        /* rotation about the geometric centre (moves.py:158-197), q^w; reinsertion
+         * = full rotation + uniform fractional offset (268-295) */
+        float uq[3] = {gc_u01(rb[0]), gc_u01(rb[1]), gc_u01(rb[2])};
+        double R[3][3]; gc_quat(uq, move == 1 ? wid[1] : 1.0, R);
+        double rel[8][3], cen[3] = {0, 0, 0};
+        for (int s = 0; s < M; ++s) {
+          min_image(&c, oldp[M-1], oldp[s], rel[s]);
+          for (int x = 0; x < 3; ++x) cen[x] += rel[s][x] / M;
+        }
+        double com[3] = {oldp[M-1][0] + cen[0], oldp[M-1][1] + cen[1], oldp[M-1][2] + cen[2]};
+        wrap_f(&c, com);
+        double off[3] = {0, 0, 0};
+        if (move == 2) {
+          double fu[3] = {gc_u01(rcn[0]), gc_u01(rcn[1]), gc_u01(rcn[2])};
+          rmul(c.L, fu, off);
+        }
+        for (int s = 0; s < M; ++s) {
+          double rr[3]; min_image(&c, com, oldp[s], rr);
+          for (int x = 0; x < 3; ++x)
+            newp[s][x] = com[x] + rr[0]*R[0][x] + rr[1]*R[1][x] + rr[2]*R[2][x] + off[x];
+          wrap_f(&c, newp[s]);
+        }
+        n_new = M;
+      } else if (move == 3 && !deleting) {
+        /* insertion: template @ R + U L (moves.py:573-675) */
+        float uq[3] = {gc_u01(rb[0]), gc_u01(rb[1]), gc_u01(rb[2])};
+        double R[3][3]; gc_quat(uq, 1.0, R);
+        double fu[3] = {gc_u01(rcn[0]), gc_u01(rcn[1]), gc_u01(rcn[2])}, off[3];
+        rmul(c.L, fu, off);
+        if (ng < a->gmax) {
+          for (int s = 0; s < M; ++s) {
+            const double* tp = a->motif_pos + 3*s;
+            for (int x = 0; x < 3; ++x)
+              newp[s][x] = tp[0]*R[0][x] + tp[1]*R[1][x] + tp[2]*R[2][x] + off[x];
+            wrap_f(&c, newp[s]);
+          }
+          n_new = M;
+        }
+      }
+      /* energy difference */
+      double dE = 0;
+      for (int pass = 0; pass < 2; ++pass) {
+        int ns = pass ? n_new : n_old;
+        double sgn = pass ? 1.0 : -1.0;
+        for (int s = 0; s < ns; ++s) {
+          double* r = pass ? newp[s] : oldp[s];
+          double q = a->motif_q[s]; int sp = a->motif_spec[s];
+          double e = 0;
+          for (int j = h0; j < h1; ++j)
+            e += gc_pair(a, &c, ch, r, q, sp, a->host_pos + 3*j, a->host_q[j], a->host_spec[j]);
+          for (int g = 0; g < ng; ++g) {
+            if (g == slot) continue;
+            for (int s2 = 0; s2 < M; ++s2)
+              e += gc_pair(a, &c, ch, r, q, sp, gp + (g*M + s2)*3, a->motif_q[s2], a->motif_spec[s2]);
+          }
+          dE += sgn * e;
+        }
+      }
+      double dElr = 0;
+      for (int k = 0; k < nk; ++k) {
+        double dr = 0, di = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+          int ns = pass ? n_new : n_old;
+          double sgn = pass ? 1.0 : -1.0;
+          for (int s = 0; s < ns; ++s) {
+            double* r = pass ? newp[s] : oldp[s];
+            double th = kv[k][0]*r[0] + kv[k][1]*r[1] + kv[k][2]*r[2];
+            dr += sgn * a->motif_q[s] * cos(th);
+            di += sgn * a->motif_q[s] * sin(th);
+          }
+        }
+        dS[2*k] = dr; dS[2*k+1] = di;
+        double sr = S[2*k], si = S[2*k+1];
+        dElr += K * P[k] * (2*(sr*dr + si*di) + dr*dr + di*di);
+      }
+      dE += dElr;
+      double logp = 0;
+      int dn = 0;
+      if (move == 3) dn = n_new ? 1 : (n_old ? -1 : 0);
+      if (dn) {
+        dE += dn * a->mol_const[ch];
+        /* tail: sum_ab T_ab (N'_a N'_b - N_a N_b) */
+        int np_[16];
+        for (int s = 0; s < a->nspec; ++s) np_[s] = cnt[s];
+        for (int s = 0; s < M; ++s) np_[a->motif_spec[s]] += dn;
+        double et = 0;
+        for (int x = 0; x < a->nspec; ++x)
+          for (int y = 0; y < a->nspec; ++y)
+            et += T[x*a->nspec + y] * ((double)np_[x]*np_[y] - (double)cnt[x]*cnt[y]);
+        dE += et;
+        double lnfV = a->log_activity[ch] + log(c.vol);
+        if (dn > 0) logp += lnfV - log((double)(ng + 1));
+        else logp += -lnfV + log((double)ng);
+      }
+      logp += -dE / a->kT[ch];
+      double lu = log((double)gc_u01(ra[1]));
+      int accept = logp > lu;
+      if (ch < a->n_trace) {
+        double* tr = a->trace + ((size_t)ch * a->n_steps + st) * 4;
+        tr[0] = move * 2 + (move == 3 ? deleting : 0); tr[1] = dE; tr[2] = logp; tr[3] = accept;
+      }
+      int64_t* stt = a->stats + (size_t)ch * 8;
+      stt[move] += 1;
+      if (accept) {
+        stt[4 + move] += 1;
+        if (n_new && n_old) {
+          for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) gp[(slot*M + s)*3 + x] = newp[s][x];
+        } else if (n_new) {
+          for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) gp[(ng*M + s)*3 + x] = newp[s][x];
+          a->n_guest[ch] = ng + 1;
+          a->ins_acc[ch] += 1;
+        } else if (n_old) {
+          for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) gp[(slot*M + s)*3 + x] = gp[((ng-1)*M + s)*3 + x];
+          a->n_guest[ch] = ng - 1;
+          a->del_acc[ch] += 1;
+        }
+        if (dn) for (int s = 0; s < M; ++s) cnt[a->motif_spec[s]] += dn;
+        for (int k = 0; k < nk; ++k) { S[2*k] += dS[2*k]; S[2*k+1] += dS[2*k+1]; }
+        a->energy[ch] += dE;
+      }
+      /* acceptance-target schedule of the chosen move (parameter_scheduler.py:161-208) */
+      uint32_t* hb = a->hist + ((size_t)ch * 4 + move) * 4;
+      int32_t* hi = a->hist_idx + (size_t)ch * 4 + move;
+      int idx = *hi;
+      if (accept) hb[idx >> 5] |= (1u << (idx & 31)); else hb[idx >> 5] &= ~(1u << (idx & 31));
+      idx = (idx + 1) % 100;
+      *hi = idx;
+      if (idx == 0) {
+        int ones = 0;
+        for (int w = 0; w < 4; ++w) ones += __builtin_popcount(hb[w] & (w == 3 ? 0xFu : 0xFFFFFFFFu));
+        double avg = ones / 100.0;
+        if (fabs(avg - 0.5) > 0.05) {
+          double v = wid[move];
+          v = avg < 0.5 ? v / 1.1 : v * 1.1;
+          if (v < 0) v = 0;
+          if (v > a->upper[move]) v = a->upper[move];
+          wid[move] = (float)v;
+        }
+      }
+    }
+    free(P); free(kv); free(dS);
+  }
+  return 0;
+}

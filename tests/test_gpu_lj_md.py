@@ -102,6 +102,15 @@ def _lj_batch(sizes, seed):
     return workloads.batched_lj_boxes(sizes=sizes, seed=seed)
 
 
+def _check_energy(eg, er, sizes, rtol):
+    """Per-system energies within rtol of max(|E|, N eps): the relative error of a
+    system whose pair energies cancel to a small total is measured against the
+    energy scale N eps, not against the near-zero total."""
+    scale = np.maximum(np.abs(er), np.asarray(sizes) * workloads.AR_EPS)
+    err = np.abs(np.asarray(eg) - er) / scale
+    assert err.max() < rtol, (err.max(), eg, er)
+
+
 def _check_forces(fg, fr, tol=1e-5):
     err = np.linalg.norm(fg - fr) / np.linalg.norm(fr)
     assert err < tol, err
@@ -117,7 +126,7 @@ def test_lj_forces_vs_oracle(gpu, sizes):
     out = LennardJonesPotential(AR)(pb, sb)
     cut = np.full(b.n_sys, workloads.AR_CUTOFF)
     e, f, w = oracle.lj_cells(_f32(pos32), b.species, b.sys_off, b.tril, cut, AR.sigma, AR.epsilon)
-    np.testing.assert_allclose(out.total_energies.cpu().numpy(), e, rtol=1e-5)
+    _check_energy(out.total_energies.cpu().numpy(), e, b.sizes, 1e-5)
     _check_forces(-out.position_gradients.cpu().numpy(), f)
     np.testing.assert_allclose(out.virial.cpu().numpy(), w, rtol=1e-5,
                                atol=1e-5 * np.abs(w).max())
@@ -146,9 +155,11 @@ def test_md_velocity_verlet_vs_oracle(gpu):
     for k in range(n_steps):
         e, _ = oracle.md_step(0, pos, mom, b.masses.astype(np.float32), b.species, b.sys_off,
                               b.tril, cut, AR.sigma, AR.epsilon, dt, dt * 0, dt * 0, eta, frc)
-        np.testing.assert_allclose(obs.potential_energy[:, k].cpu().numpy(), e, rtol=2e-5)
-    d = workloads.wrap_cubic(st.positions.cpu().numpy() - pos, 1.0)  # box-agnostic small diff
-    assert np.abs(st.positions.cpu().numpy() - pos).max() < 1e-3 or np.abs(d).max() < 1e-3
+        _check_energy(obs.potential_energy[:, k].cpu().numpy(), e, b.sizes, 1e-5)
+    box = np.repeat(b.tril[:, 0], b.sizes)[:, None]
+    d = st.positions.cpu().numpy() - pos
+    d -= np.round(d / box) * box  # compare modulo the periodic box
+    assert np.abs(d).max() < 1e-4
     _check_forces(st.forces.cpu().numpy(), frc, tol=1e-4)
     _check_forces(st.momenta.cpu().numpy(), mom, tol=1e-4)
 
@@ -169,7 +180,7 @@ def test_md_baoab_shared_noise_vs_oracle(gpu):
         eta = normals(1234, step0 + k, b.n).cpu().numpy().astype(np.float64)
         e, _ = oracle.md_step(1, pos, mom, b.masses.astype(np.float32), b.species, b.sys_off,
                               b.tril, cut, AR.sigma, AR.epsilon, dt, kT, gam, eta, frc)
-        np.testing.assert_allclose(obs.potential_energy[:, k].cpu().numpy(), e, rtol=2e-5)
+        _check_energy(obs.potential_energy[:, k].cpu().numpy(), e, b.sizes, 1e-5)
     _check_forces(st.momenta.cpu().numpy(), mom, tol=1e-4)
 
 
@@ -191,12 +202,16 @@ def test_md_full_batch_properties(gpu):
     st2 = _md_state(b)
     st2.forces.copy_(st.forces)
     obs = prop(st, 20)
-    prop(st2, 20, record=False)
+    prop(st2, 20, record=False)  # no observables: no reductions between the steps
     assert torch.equal(st.positions, st2.positions) and torch.equal(st.momenta, st2.momenta)
     etot = (obs.potential_energy + obs.kinetic_energy).cpu().numpy()
     n = b.sizes
     drift = np.abs(etot[:, -1] - etot[:, 0]) / n
-    assert drift.max() < 2e-4, drift.max()   # eV per atom over 20 steps
+    # hard-cutoff LJ (no shift, as the reference) jumps by u(rc) = -0.0163 eps
+    # whenever a pair crosses rc, so small boxes drift in steps; the bulk must
+    # conserve energy to FP32 accuracy
+    assert np.median(drift) < 2e-5, np.median(drift)   # eV per atom over 20 steps
+    assert drift.max() < 2e-3, drift.max()
     p = st.momenta.cpu().numpy().astype(np.float64)
     ptot = np.add.reduceat(p, b.sys_off[:-1], axis=0)
     assert np.abs(ptot).max() < 1e-3
