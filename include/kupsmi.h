@@ -183,6 +183,46 @@ int kupsmi_integrate(void* stream, int phase, int integrator, int n_sys,
                      const double* dt, const double* kT, const double* gamma, uint64_t seed,
                      int64_t step, double* kin_energy);
 
+/*
+ * Rigid-molecule GCMC: n_steps trial moves of every chain, one workgroup per
+ * chain with its host framework and guests resident in LDS.  Replaces the
+ * MCMC cycle of kups_mcmc_rigid (application/simulations/mcmc_rigid.py:
+ * 281-307): per step ONE move type for the whole batch (translation,
+ * rotation, reinsertion, or exchange with one insert/delete coin;
+ * core/propagator.py:244-273, mcmc/moves.py:1078-1230), proposals of
+ * moves.py:122-295 and 573-835, Delta E from LJ + Ewald real-space pairs of
+ * the changed sites, the structure-factor update S' = S + dS
+ * (ewald.py:442-486), the self/exclusion energy of an inserted or deleted
+ * molecule and the LJ tail change, acceptance log p = -dE/kT + fugacity term
+ * (mcmc/probability.py:60-225) against log U in FP64, and the
+ * acceptance-target step-width schedule (core/parameter_scheduler.py:161-208).
+ *   host_xyzq      [n_host_total x 4] f32 (x, y, z, charge), chains by host_off
+ *   motif_*        the guest template (m <= 4 sites, centred)
+ *   kn / pk        per chain k list: nk packed (n0 | (n1+64) << 8 | (n2+64) << 16)
+ *                  and FP64 prefactors P(k) (0 = padding); nm = max |n|
+ *   guest          [n_chain x gmax x m x 3] f32, n_guest, counts[n_chain x n_species],
+ *                  structure_factor [n_chain x nk x 2] f64, energy, width[n_chain x 4],
+ *                  hist[n_chain x 4 x 4], hist_idx[n_chain x 4]: state, in/out
+ *   move_weights   HOST array[4]: unnormalised weights of translation, rotation,
+ *                  reinsertion, exchange (default 1, 1, 1, 3 in kUPS's mcmc app)
+ *   width_upper    HOST array[4]: upper bounds of the adapted step widths
+ *   stats          [n_chain x 10] int64: trials[4], accepted[4], insertions, deletions
+ *   trace          optional [n_trace x n_steps x 4] f64 (move, dE, log p, accepted)
+ * Random numbers: Philox4x32-10 keyed by seed, counters (chain, step, tag).
+ */
+int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t step0, uint64_t seed,
+                    int max_host, const int32_t* host_off, const float* host_xyzq,
+                    const int32_t* host_species, const double* tril, int gmax, int m,
+                    const float* motif_pos, const float* motif_q, const int32_t* motif_species,
+                    int n_species, const double* sigma, const double* epsilon, double rc_lj,
+                    double rc_coul, const double* alpha, const double* tail,
+                    const double* mol_const, const double* kT, const double* log_activity,
+                    int nk, int nm, const int32_t* kn, const double* pk,
+                    const double* move_weights, const double* width_upper, float* guest,
+                    int32_t* n_guest, int32_t* counts, double* structure_factor,
+                    double* energy, float* width, uint32_t* hist, int32_t* hist_idx,
+                    long long* stats, int n_trace, double* trace);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,12 @@ _SIGNATURES = {
         [vp, vp, i32, vp, i32, i32, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp],
         C.c_int,
     ),
+    "kupsmi_gcmc_run": (
+        [vp, i32, i32, i64, u64, i32, vp, vp, vp, vp, i32, i32, vp, vp, vp, i32, vp, vp,
+         C.c_double, C.c_double, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp,
+         vp, vp, vp, vp, vp, vp, i32, vp],
+        C.c_int,
+    ),
     "kupsmi_integrate": (
         [vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64, vp],
         C.c_int,

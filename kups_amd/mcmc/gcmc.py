@@ -1,0 +1,321 @@
+"""Rigid-molecule GCMC driver (application/simulations/mcmc_rigid.py).
+
+``RigidGCMC`` builds the state of kups_mcmc_rigid for a batch of chains (hosts
+with the reference's exclusion semantics, one guest species, TraPPE LJ with
+Lorentz-Berthelot mixing and the global tail correction, Ewald with the
+reference's estimator, Peng-Robinson fugacity) and advances all chains with
+the LDS-resident HIP kernel ``kupsmi_gcmc_run``.  Set-up that the reference
+does on the host (estimator, k vectors, fugacity, tail matrix) is done on the
+host here too; the initial structure factors and energies come from the GPU
+Ewald and pair kernels.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..core.batch import ParticleBatch, SystemBatch, offsets_from_sizes
+from ..core.constants import BOLTZMANN_CONSTANT, PASCAL, TO_STANDARD_UNITS
+from ..potential.ewald import EwaldParameters, EwaldPotential, estimate_ewald_parameters, kvec_grid
+from ..potential.lennard_jones import LennardJonesParameters
+from .fugacity import peng_robinson_log_fugacity
+
+MOVES = ("translation", "rotation", "reinsertion", "exchange")
+
+
+@dataclass
+class Motif:
+    """Rigid guest template (positions centred on the geometric centre)."""
+
+    species: tuple[str, ...]
+    positions: np.ndarray
+    charges: np.ndarray
+    critical_temperature: float
+    critical_pressure: float
+    acentric_factor: float
+
+    @classmethod
+    def from_dict(cls, d) -> "Motif":
+        p = np.asarray(d["positions"], float)
+        return cls(tuple(d["symbols"]), p - p.mean(axis=0), np.asarray(d["charges"], float),
+                   float(d["critical_temperature"]), float(d["critical_pressure"]),
+                   float(d["acentric_factor"]))
+
+
+def molecule_constant(motif: Motif, alpha: float) -> float:
+    """E_self + E_excl of one rigid molecule: -K alpha/sqrt(pi) sum q^2
+    (ewald.py:277-302) + sum_{i<j} K q_i q_j (erfc(alpha r) - 1)/r (328-346)."""
+    from scipy.special import erfc
+
+    q = motif.charges
+    e = -TO_STANDARD_UNITS * alpha / math.sqrt(math.pi) * float(q @ q)
+    for i in range(len(q)):
+        for j in range(i + 1, len(q)):
+            r = float(np.linalg.norm(motif.positions[i] - motif.positions[j]))
+            e += TO_STANDARD_UNITS * q[i] * q[j] * (erfc(alpha * r) - 1.0) / r
+    return e
+
+
+def tail_matrix(lj: LennardJonesParameters, volume: float) -> np.ndarray:
+    """T_ab with E_tail = sum_ab T_ab N_a N_b (lennard_jones.py:276-313, pairs with
+    eps > 0)."""
+    t1 = (lj.sigma / lj.cutoff) ** 3
+    T = (8 / 3) * math.pi / volume * lj.epsilon * lj.sigma**3 * (t1**3 / 3 - t1)
+    return T * (lj.epsilon > 0)
+
+
+class RigidGCMC:
+    """GCMC of one rigid guest species in a batch of rigid hosts."""
+
+    def __init__(self, host_positions, host_charges, host_species, host_sizes, lattices,
+                 pressure, temperature, motif: Motif, lj: LennardJonesParameters,
+                 guest_species: tuple[int, ...], *, ewald_precision: float = 1e-6,
+                 ewald_cutoff: float = 12.0, weights=(1.0, 1.0, 1.0, 3.0), gmax: int = 64,
+                 seed: int = 0, device=None):
+        dev = device or _lib.require_gpu()
+        self.dev = dev
+        self.motif = motif
+        self.lj = lj
+        self.seed = int(seed)
+        self.step = 0
+        self.gmax = int(gmax)
+        self.weights = np.asarray(weights, float)
+        sizes = np.asarray(host_sizes, np.int64)
+        self.sizes = sizes
+        n = len(sizes)
+        self.n_chain = n
+        lat = np.asarray(lattices, float).reshape(n, 3, 3)
+        tril = lat[:, [0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]]
+        vol = np.abs(np.linalg.det(lat))
+        self.volume = vol
+        self.temperature = np.broadcast_to(np.asarray(temperature, float), (n,)).copy()
+        kT = self.temperature * BOLTZMANN_CONSTANT
+        # fugacity (mcmc/data.py:286-290): ln a = ln f - ln kT, f in eV / A^3
+        lnf = np.array([peng_robinson_log_fugacity(p, t, motif.critical_pressure,
+                                                   motif.critical_temperature,
+                                                   motif.acentric_factor)[0][0]
+                        for p, t in zip(np.broadcast_to(pressure, (n,)), self.temperature)])
+        self.log_activity = lnf + math.log(PASCAL) - np.log(kT)
+        # Ewald: fixed real-space cutoff -> alpha and k_max (ewald.py:1254-1277)
+        hq = np.asarray(host_charges, float)
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        est = [estimate_ewald_parameters(hq[off[s]:off[s + 1]] if sizes[s] else motif.charges,
+                                         tril[s], real_cutoff=ewald_cutoff,
+                                         epsilon_total=ewald_precision) for s in range(n)]
+        self.alpha = np.array([e["alpha"] for e in est])
+        self.kmax = np.array([e["k_max"] for e in est])
+        self.ewald_cutoff = float(ewald_cutoff)
+        kl = [kvec_grid(tril[s], self.kmax[s]) for s in range(n)]
+        nk = max(len(k) for k in kl)
+        self.nm = int(max(np.abs(k).max() for k in kl))
+        if self.nm > 12:
+            raise ValueError("k vectors beyond |n| = 12 are not supported")
+        kn = np.zeros((n, nk), np.int32)
+        pk = np.zeros((n, nk))
+        gidx = np.zeros((n, nk), np.int64)  # index into the (nm, nm, nm) box grid
+        w = 2 * self.nm + 1
+        for s in range(n):
+            k = kl[s]
+            kn[s, :len(k)] = k[:, 0] | ((k[:, 1] + 64) << 8) | ((k[:, 2] + 64) << 16)
+            kv = 2 * np.pi * k @ np.linalg.inv(lat[s]).T
+            k2 = (kv**2).sum(-1)
+            P = np.where(k2 > 0, 2 * np.pi / vol[s] * np.exp(-k2 / (4 * self.alpha[s]**2))
+                         / np.where(k2 > 0, k2, 1) * np.where(k[:, 0] == 0, 1.0, 2.0), 0.0)
+            pk[s, :len(k)] = P
+            gidx[s, :len(k)] = (k[:, 0] * w + (k[:, 1] + self.nm)) * w + (k[:, 2] + self.nm)
+        self.nk = nk
+        self._kgrid_index = gidx
+        # species: host types then guest types
+        self.n_species = lj.n_species
+        self.guest_species = np.asarray(guest_species, np.int32)
+        self.mol_const = np.array([molecule_constant(motif, a) for a in self.alpha])
+        self.tail = np.stack([tail_matrix(lj, v) for v in vol])
+        counts = np.zeros((n, self.n_species), np.int32)
+        hs = np.asarray(host_species, np.int32)
+        for s in range(n):
+            counts[s] = np.bincount(hs[off[s]:off[s + 1]], minlength=self.n_species)
+        # translation width <= half the smallest perpendicular width of the batch
+        # (application/simulations/mcmc_rigid.py:357-371); others unbounded
+        self.upper = np.array([self._min_half_box(lat), 1e30, 1e30, 1e30])
+        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+        f32, f64, i32 = torch.float32, torch.float64, torch.int32
+        self.d = dict(
+            host_off=t(off.astype(np.int32), i32),
+            host=t(np.concatenate([np.asarray(host_positions, float), hq[:, None]], 1), f32),
+            host_species=t(hs, i32), tril=t(tril, f64),
+            motif_pos=t(motif.positions, f32), motif_q=t(motif.charges, f32),
+            motif_species=t(self.guest_species, i32), sigma=t(lj.sigma, f64),
+            epsilon=t(lj.epsilon, f64), alpha=t(self.alpha, f64), tail=t(self.tail, f64),
+            mol_const=t(self.mol_const, f64), kT=t(kT, f64), log_activity=t(self.log_activity, f64),
+            kn=t(kn, i32), pk=t(pk, f64),
+            guest=torch.zeros((n, self.gmax, len(motif.charges), 3), dtype=f32, device=dev),
+            n_guest=torch.zeros(n, dtype=i32, device=dev), counts=t(counts, i32),
+            sk=torch.zeros((n, nk, 2), dtype=f64, device=dev),
+            energy=torch.zeros(n, dtype=f64, device=dev),
+            width=torch.full((n, 4), 0.1, dtype=f32, device=dev),
+            hist=torch.zeros((n, 4, 4), dtype=torch.int32, device=dev),
+            hist_idx=torch.zeros((n, 4), dtype=i32, device=dev),
+            stats=torch.zeros((n, 10), dtype=torch.int64, device=dev),
+        )
+        # move weights and width bounds are host arrays of the entry point
+        self._w_host = np.ascontiguousarray(self.weights, dtype=np.float64)
+        self._u_host = np.ascontiguousarray(self.upper, dtype=np.float64)
+        self.max_host = int(sizes.max()) if n else 0
+        self.recompute_state()
+
+    @staticmethod
+    def _min_half_box(lat) -> float:
+        lat = np.asarray(lat).reshape(-1, 3, 3)
+        return min(float((abs(np.linalg.det(L)) /
+                          np.linalg.norm(np.cross(L[[1, 0, 0]], L[[2, 2, 1]]), axis=-1)).min())
+                   for L in lat) / 2.0
+
+    def sr_intramolecular(self) -> np.ndarray:
+        """E_sr over the intramolecular pairs of one guest, per chain.  The
+        reference's full evaluation contains it for every molecule present at
+        that time, its incremental Delta E does not."""
+        from scipy.special import erfc
+
+        q, p = self.motif.charges, self.motif.positions
+        out = np.zeros(self.n_chain)
+        for i in range(len(q)):
+            for j in range(i + 1, len(q)):
+                r = float(np.linalg.norm(p[i] - p[j]))
+                out += TO_STANDARD_UNITS * q[i] * q[j] * erfc(self.alpha * r) / r
+        return out
+
+    # ---- full evaluation (initial energies, structure factors, checks)
+    def configuration(self):
+        """Host + guest atoms of every chain as a ParticleBatch (chain-sorted)."""
+        d = self.d
+        n = self.n_chain
+        M = len(self.motif.charges)
+        ng = d["n_guest"].cpu().numpy()
+        host = d["host"].cpu().numpy()
+        hoff = d["host_off"].cpu().numpy()
+        gpos = d["guest"].cpu().numpy()
+        pos, q, sp, grp, sizes = [], [], [], [], []
+        for s in range(n):
+            h = host[hoff[s]:hoff[s + 1]]
+            g = gpos[s, :ng[s]].reshape(-1, 3)
+            pos += [h[:, :3], g]
+            q += [h[:, 3], np.tile(self.motif.charges, ng[s])]
+            sp += [d["host_species"].cpu().numpy()[hoff[s]:hoff[s + 1]], np.tile(self.guest_species, ng[s])]
+            grp += [np.full(len(h), -1), np.repeat(np.arange(ng[s]), M)]
+            sizes.append(len(h) + len(g))
+        return (np.concatenate(pos), np.concatenate(q), np.concatenate(sp).astype(np.int32),
+                np.concatenate(grp).astype(np.int32), np.asarray(sizes))
+
+    def full_energy(self) -> np.ndarray:
+        """Absolute energy of every chain with the reference's composition
+        (application/simulations/mcmc_rigid.py:281-307): Ewald over the atomic
+        view (all pairs, intramolecular included, ewald.py:946-967) + self +
+        exclusion correction of every guest + LJ with host-host and
+        intramolecular pairs excluded + global LJ tail."""
+        pos, q, sp, grp, sizes = self.configuration()
+        dev = self.dev
+        n = self.n_chain
+        system = np.repeat(np.arange(n), sizes).astype(np.int32)
+        off = offsets_from_sizes(sizes, dev)
+        pb = ParticleBatch(torch.as_tensor(pos, dtype=torch.float32, device=dev),
+                           torch.as_tensor(system, device=dev),
+                           species=torch.as_tensor(sp, device=dev),
+                           charges=torch.as_tensor(q, dtype=torch.float32, device=dev))
+        sb = SystemBatch(self.d["tril"])
+        ep = EwaldParameters(self.alpha, np.full(n, self.ewald_cutoff), self.kmax,
+                             (self.nm, self.nm, self.nm))
+        out = EwaldPotential(ep)(pb, sb)
+        # LJ: exclusion id = group (hosts share one id per chain)
+        big = np.int32(2**30)
+        excl = np.where(grp < 0, big, grp + np.repeat(np.arange(n), sizes) * self.gmax).astype(np.int32)
+        excl_t = torch.as_tensor(excl, device=dev)
+        sig = torch.as_tensor(self.lj.sigma, device=dev)
+        eps = torch.as_tensor(self.lj.epsilon, device=dev)
+        rc = torch.full((n,), self.lj.cutoff, dtype=torch.float64, device=dev)
+        f = torch.zeros((len(pos), 3), dtype=torch.float32, device=dev)
+        e2 = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+        _lib.check(_lib.load().kupsmi_pair_forces(
+            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), len(pos), int(sizes.max()),
+            _lib.ptr(pb.positions), None, _lib.ptr(pb.species), self.n_species, _lib.ptr(sig),
+            _lib.ptr(eps), _lib.ptr(rc), _lib.ptr(excl_t), None, None, None, _lib.ptr(sb.tril), 0,
+            _lib.ptr(f), _lib.ptr(e2), None))
+        ng = self.d["n_guest"].cpu().numpy()
+        counts = self.d["counts"].cpu().numpy().astype(float)
+        e_tail = np.einsum("sab,sa,sb->s", self.tail, counts, counts)
+        # intramolecular exclusion correction of every guest molecule; the
+        # atomic view above already holds E_sr(intra) of each guest (the
+        # reference's composition, see the full-evaluation note in DESIGN)
+        e_excl = ng * (self.mol_const - self._self_mol())
+        return (out.short_range + out.long_range + out.self_energy).cpu().numpy() + e_excl + \
+            e2[:, 0].cpu().numpy() + e_tail
+
+    def _self_mol(self):
+        q = self.motif.charges
+        return -TO_STANDARD_UNITS * self.alpha / math.sqrt(math.pi) * float(q @ q)
+
+    def recompute_state(self):
+        """Structure factors of the current configuration (GPU) and the tracked
+        energy set to the full evaluation."""
+        pos, q, sp, grp, sizes = self.configuration()
+        dev = self.dev
+        n = self.n_chain
+        off = offsets_from_sizes(sizes, dev)
+        p = torch.as_tensor(pos, dtype=torch.float32, device=dev)
+        qq = torch.as_tensor(q, dtype=torch.float32, device=dev)
+        alpha = torch.as_tensor(self.alpha, dtype=torch.float64, device=dev)
+        nm = self.nm
+        sk = torch.empty((n, (nm + 1) * (2 * nm + 1) ** 2, 2), dtype=torch.float64, device=dev)
+        _lib.check(_lib.load().kupsmi_ewald_recip(
+            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), len(pos), int(sizes.max()),
+            _lib.ptr(p), _lib.ptr(qq), _lib.ptr(self.d["tril"]), _lib.ptr(alpha), None, nm, nm, nm,
+            0, None, None, None, _lib.ptr(sk)))
+        gi = torch.as_tensor(self._kgrid_index, device=dev)
+        self.d["sk"].copy_(torch.gather(sk, 1, gi[:, :, None].expand(-1, -1, 2)))
+        self.d["energy"].copy_(torch.as_tensor(self.full_energy(), device=dev))
+
+    # ---- the MC cycle
+    def run(self, n_steps: int, n_trace: int = 0):
+        d = self.d
+        trace = torch.zeros((max(n_trace, 1), max(n_steps, 1), 4), dtype=torch.float64, device=self.dev)
+        M = len(self.motif.charges)
+        _lib.check(_lib.load().kupsmi_gcmc_run(
+            _lib.stream_ptr(), self.n_chain, int(n_steps), int(self.step), self.seed & (2**64 - 1),
+            self.max_host, _lib.ptr(d["host_off"]), _lib.ptr(d["host"]), _lib.ptr(d["host_species"]),
+            _lib.ptr(d["tril"]), self.gmax, M, _lib.ptr(d["motif_pos"]), _lib.ptr(d["motif_q"]),
+            _lib.ptr(d["motif_species"]), self.n_species, _lib.ptr(d["sigma"]), _lib.ptr(d["epsilon"]),
+            float(self.lj.cutoff), float(self.ewald_cutoff), _lib.ptr(d["alpha"]), _lib.ptr(d["tail"]),
+            _lib.ptr(d["mol_const"]), _lib.ptr(d["kT"]), _lib.ptr(d["log_activity"]), self.nk,
+            self.nm, _lib.ptr(d["kn"]), _lib.ptr(d["pk"]), _lib.ptr(d["weights"]),
+            _lib.ptr(d["upper"]), _lib.ptr(d["guest"]), _lib.ptr(d["n_guest"]), _lib.ptr(d["counts"]),
+            _lib.ptr(d["sk"]), _lib.ptr(d["energy"]), _lib.ptr(d["width"]), _lib.ptr(d["hist"]),
+            _lib.ptr(d["hist_idx"]), _lib.ptr(d["stats"]), int(n_trace), _lib.ptr(trace)))
+        self.step += int(n_steps)
+        return trace[:n_trace] if n_trace else None
+
+    @property
+    def loading(self) -> torch.Tensor:
+        return self.d["n_guest"]
+
+    @property
+    def stats(self) -> dict:
+        s = self.d["stats"].cpu().numpy()
+        return {"trials": s[:, :4], "accepted": s[:, 4:8], "insertions": s[:, 8], "deletions": s[:, 9]}
+
+
+def make_co2_gcmc(hosts, *, gmax: int = 64, seed: int = 0, weights=(1.0, 1.0, 1.0, 3.0),
+                  device=None) -> RigidGCMC:
+    """GCMC of CO2 in the synthetic rigid hosts of kups_amd.workloads.gcmc_hosts."""
+    from .. import workloads as w
+
+    labels = list(w.HOST_TYPES) + ["C_co2", "O_co2"]
+    lj = LennardJonesParameters.from_dict(12.0, {k: w.GCMC_LJ[k] for k in labels})
+    motif = Motif.from_dict(w.CO2)
+    guest_species = tuple(labels.index(sym) for sym in motif.species)
+    return RigidGCMC(hosts.positions, hosts.charges, hosts.species, hosts.sizes, hosts.lattices,
+                     hosts.pressure, hosts.temperature, motif, lj, guest_species, gmax=gmax,
+                     seed=seed, weights=weights, device=device)

@@ -128,3 +128,103 @@ def nacl_melts(n_sys: int = 512, mmin: int = 5, mmax: int = 10, seed: int = 0, a
     return Batch(np.concatenate(pos_all), sizes, np.array(trils), sp,
                  np.where(sp == 0, NA_MASS, CL_MASS), charges=np.concatenate(q_all),
                  description=f"{len(ms)} NaCl-like melts, N=8m^3, m in [{min(ms)},{max(ms)}]")
+
+
+# TraPPE/UFF-style LJ parameters (sigma [A], epsilon [eV]) of the host types
+# and of the CO2 sites, as listed in the reference's statistical inputs
+# (ci/statistical/inputs/lennard_jones/trappe.yaml:47,113,167,191,617,623-628).
+GCMC_LJ = {
+    "C_": (3.4309, 0.004553698956747378),
+    "H_": (2.5711, 0.0019082130125430423),
+    "O_": (3.1181, 0.002602114920617121),
+    "N_": (3.2607, 0.002992428280911038),
+    "Zn": (2.4616, 0.005377696700922425),
+    "C_co2": (2.8, 0.0023266791910486473),
+    "O_co2": (3.05, 0.006807690966401598),
+}
+HOST_TYPES = ("C_", "H_", "O_", "N_", "Zn")
+HOST_TYPE_P = (0.4, 0.3, 0.15, 0.1, 0.05)
+# rigid linear CO2 (ci/statistical/inputs/ci/gcmc_co2_rubtak.yaml:2-12)
+CO2 = {
+    "symbols": ("C_co2", "O_co2", "O_co2"),
+    "positions": ((0.0, 0.0, 0.0), (-1.16, 0.0, 0.0), (1.16, 0.0, 0.0)),
+    "charges": (0.7, -0.35, -0.35),
+    "critical_temperature": 303.75,
+    "critical_pressure": 7_840_000.0,
+    "acentric_factor": 0.22394,
+}
+
+
+def lattice_from_lengths_angles(lengths, angles_deg) -> np.ndarray:
+    """Lower-triangular lattice (rows a, b, c) of core/unitcell.py:100-136."""
+    a, b, c = lengths
+    al, be, ga = np.radians(angles_deg)
+    cz = np.sqrt(1 - np.cos(be) ** 2 - ((np.cos(al) - np.cos(be) * np.cos(ga)) / np.sin(ga)) ** 2)
+    return np.array([[a, 0, 0], [b * np.cos(ga), b * np.sin(ga), 0],
+                     [c * np.cos(be), c * (np.cos(al) - np.cos(be) * np.cos(ga)) / np.sin(ga), c * cz]])
+
+
+def perpendicular_widths(lat) -> np.ndarray:
+    v = abs(np.linalg.det(lat))
+    return v / np.linalg.norm(np.cross(lat[[1, 0, 0]], lat[[2, 2, 1]]), axis=-1)
+
+
+@dataclass
+class GcmcHosts:
+    positions: np.ndarray  # (n_host_total, 3)
+    charges: np.ndarray
+    species: np.ndarray  # index into HOST_TYPES
+    sizes: np.ndarray
+    lattices: np.ndarray  # (n_chain, 3, 3)
+    pressure: np.ndarray  # Pa
+    temperature: np.ndarray  # K
+    description: str = ""
+
+    @property
+    def n_chain(self) -> int:
+        return len(self.sizes)
+
+    @property
+    def tril(self) -> np.ndarray:
+        return self.lattices[:, [0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]]
+
+
+def gcmc_hosts(n_chain: int = 1024, nmin: int = 1500, nmax: int = 3000, seed: int = 0,
+               lmin: float = 25.0, lmax: float = 30.0, angle_dev: float = 5.0,
+               vacancy: float = 0.3, jitter: float = 0.1, qstd: float = 0.3,
+               pmin: float = 1e4, pmax: float = 1e6, temperature: float = 298.0,
+               min_width: float = 24.5, sizes=None) -> GcmcHosts:
+    """Rigid hosts: N_h ~ U{nmin..nmax} atoms on a jittered cubic fractional
+    lattice with a `vacancy` fraction of empty sites, in a triclinic box with
+    lengths U(lmin, lmax) and angles 90 +- U(angle_dev) degrees (resampled
+    until every perpendicular width exceeds min_width, so a 12 A cutoff needs no
+    periodic images), charges N(0, qstd) neutralised, host types drawn with
+    HOST_TYPE_P, pressure log-uniform in [pmin, pmax] Pa."""
+    rng = np.random.default_rng(seed)
+    if sizes is None:
+        sizes = rng.integers(nmin, nmax + 1, size=n_chain)
+    sizes = np.asarray(sizes, np.int64)
+    pos, q, sp, lats = [], [], [], []
+    for n in sizes:
+        while True:
+            lat = lattice_from_lengths_angles(rng.uniform(lmin, lmax, 3),
+                                              90.0 + rng.uniform(-angle_dev, angle_dev, 3))
+            if perpendicular_widths(lat).min() > min_width:
+                break
+        side = int(np.ceil((n / (1 - vacancy)) ** (1 / 3)))
+        g = np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        pick = np.sort(rng.choice(len(g), size=int(n), replace=False))
+        f = (g[pick] + 0.5) / side - 0.5
+        r = f @ lat + rng.normal(0, jitter, (int(n), 3))
+        fr = r @ np.linalg.inv(lat)
+        fr = (fr + 0.5) % 1.0 - 0.5
+        pos.append(fr @ lat)
+        qq = rng.normal(0, qstd, int(n))
+        q.append(qq - qq.mean())
+        sp.append(rng.choice(len(HOST_TYPES), size=int(n), p=HOST_TYPE_P))
+        lats.append(lat)
+    press = np.exp(rng.uniform(np.log(pmin), np.log(pmax), len(sizes)))
+    return GcmcHosts(np.concatenate(pos), np.concatenate(q), np.concatenate(sp).astype(np.int32),
+                     sizes, np.array(lats), press, np.full(len(sizes), temperature),
+                     description=f"{len(sizes)} rigid hosts, N_h in [{sizes.min()},{sizes.max()}], "
+                                 f"{int(vacancy * 100)}% vacancies, angles 90+-{angle_dev}")

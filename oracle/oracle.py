@@ -200,3 +200,76 @@ def md_vv_f32(n_steps, pos, mom, mass, species, sys_off, tril, cutoff, sigma, ep
     if rc != 0:
         raise ValueError("orc_md_vv_f32 supports only systems that need no periodic images")
     return e
+
+
+class _GcArgs(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("n_chain", "n_steps", "gmax", "m", "nspec", "mx", "my", "mz")] + [
+        ("step0", C.c_int64), ("seed", C.c_uint64),
+        ("host_off", C.c_void_p), ("host_pos", C.c_void_p), ("host_q", C.c_void_p),
+        ("host_spec", C.c_void_p), ("tril", C.c_void_p), ("motif_pos", C.c_void_p),
+        ("motif_q", C.c_void_p), ("motif_spec", C.c_void_p), ("sig", C.c_void_p),
+        ("eps", C.c_void_p), ("rc_lj", C.c_double), ("rc_cl", C.c_double),
+        ("alpha", C.c_void_p), ("kmax", C.c_void_p), ("tail", C.c_void_p),
+        ("mol_const", C.c_void_p), ("kT", C.c_void_p), ("log_activity", C.c_void_p),
+        ("w_move", C.c_double * 4), ("upper", C.c_double * 4),
+        ("guest_pos", C.c_void_p), ("n_guest", C.c_void_p), ("counts", C.c_void_p),
+        ("sk", C.c_void_p), ("energy", C.c_void_p), ("width", C.c_void_p), ("hist", C.c_void_p),
+        ("hist_idx", C.c_void_p), ("stats", C.c_void_p), ("ins_acc", C.c_void_p),
+        ("del_acc", C.c_void_p), ("n_trace", C.c_int), ("trace", C.c_void_p)]
+
+
+def gcmc_run(state: dict, n_steps: int, step0: int, seed: int, n_trace: int = 0):
+    """FP64 GCMC of every chain in place (see orc_gcmc_run).  ``state`` holds
+    numpy arrays: host_off, host_pos, host_q, host_spec, tril, motif_pos, motif_q,
+    motif_spec, sigma, epsilon, alpha, kmax, tail, mol_const, kT, log_activity,
+    guest_pos (n, gmax, m, 3), n_guest, counts, sk (n, nk_grid, 2), energy, width
+    (n, 4) f32, hist (n, 4, 4) u32, hist_idx (n, 4), stats (n, 8) i64, ins_acc,
+    del_acc; scalars rc_lj, rc_cl, grid (mx, my, mz), w_move, upper."""
+    a = _GcArgs()
+    n = len(state["n_guest"])
+    gmax, m = state["guest_pos"].shape[1:3]
+    a.n_chain, a.n_steps, a.gmax, a.m = n, int(n_steps), gmax, m
+    a.nspec = state["sigma"].shape[0]
+    a.mx, a.my, a.mz = state["grid"]
+    a.step0, a.seed = int(step0), int(seed) & (2**64 - 1)
+    keep = {}
+    for key, dt in (("host_off", np.int32), ("host_pos", np.float64), ("host_q", np.float64),
+                    ("host_spec", np.int32), ("tril", np.float64), ("motif_pos", np.float64),
+                    ("motif_q", np.float64), ("motif_spec", np.int32), ("sigma", np.float64),
+                    ("epsilon", np.float64), ("alpha", np.float64), ("kmax", np.float64),
+                    ("tail", np.float64), ("mol_const", np.float64), ("kT", np.float64),
+                    ("log_activity", np.float64)):
+        keep[key] = np.ascontiguousarray(state[key], dtype=dt)
+    for key, dt in (("guest_pos", np.float64), ("n_guest", np.int32), ("counts", np.int32),
+                    ("sk", np.float64), ("energy", np.float64), ("width", np.float32),
+                    ("hist", np.uint32), ("hist_idx", np.int32), ("stats", np.int64),
+                    ("ins_acc", np.int64), ("del_acc", np.int64)):
+        arr = state[key]
+        assert arr.dtype == dt and arr.flags.c_contiguous, key
+        keep[key] = arr
+    trace = np.zeros((max(n_trace, 1), max(n_steps, 1), 4))
+    a.host_off, a.host_pos, a.host_q = _p(keep["host_off"]), _p(keep["host_pos"]), _p(keep["host_q"])
+    a.host_spec, a.tril = _p(keep["host_spec"]), _p(keep["tril"])
+    a.motif_pos, a.motif_q, a.motif_spec = _p(keep["motif_pos"]), _p(keep["motif_q"]), _p(keep["motif_spec"])
+    a.sig, a.eps = _p(keep["sigma"]), _p(keep["epsilon"])
+    a.rc_lj, a.rc_cl = float(state["rc_lj"]), float(state["rc_cl"])
+    a.alpha, a.kmax, a.tail = _p(keep["alpha"]), _p(keep["kmax"]), _p(keep["tail"])
+    a.mol_const, a.kT, a.log_activity = _p(keep["mol_const"]), _p(keep["kT"]), _p(keep["log_activity"])
+    for i in range(4):
+        a.w_move[i] = float(state["w_move"][i])
+        a.upper[i] = float(state["upper"][i])
+    a.guest_pos, a.n_guest, a.counts = _p(keep["guest_pos"]), _p(keep["n_guest"]), _p(keep["counts"])
+    a.sk, a.energy, a.width = _p(keep["sk"]), _p(keep["energy"]), _p(keep["width"])
+    a.hist, a.hist_idx, a.stats = _p(keep["hist"]), _p(keep["hist_idx"]), _p(keep["stats"])
+    a.ins_acc, a.del_acc = _p(keep["ins_acc"]), _p(keep["del_acc"])
+    a.n_trace, a.trace = int(n_trace), _p(trace)
+    lib().orc_gcmc_run(C.byref(a))
+    return trace[:n_trace]
+
+
+def philox(seed: int, a: int, t: int, tag: int) -> np.ndarray:
+    """The four Philox4x32-10 words of counter (a, t, t >> 32, tag)."""
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox(C.c_uint64(int(seed) & (2**64 - 1)), C.c_uint32(a), C.c_int64(t),
+                     C.c_uint32(tag), _p(out))
+    return out
