@@ -256,7 +256,8 @@ extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0 && max_atoms_per_sys >= 0, "negative size");
   KUPSMI_REQUIRE(mx >= 0 && my >= 0 && mz >= 0 && mx <= EK_NM && my <= EK_NM && mz <= EK_NM,
                  "k grid half widths must be in [0, 12]");
-  KUPSMI_REQUIRE(pos && charges && tril && alpha, "positions, charges, cell and alpha required");
+  KUPSMI_REQUIRE(tril && alpha, "cell and alpha required");
+  KUPSMI_REQUIRE(n_atoms == 0 || (pos && charges), "positions and charges required");
   if (n_sys == 0) return KUPSMI_OK;
   hipStream_t st = (hipStream_t)stream;
   EkArgs a{};

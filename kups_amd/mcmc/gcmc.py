@@ -220,6 +220,8 @@ class RigidGCMC:
         pos, q, sp, grp, sizes = self.configuration()
         dev = self.dev
         n = self.n_chain
+        if len(pos) == 0:
+            return np.zeros(n)
         system = np.repeat(np.arange(n), sizes).astype(np.int32)
         off = offsets_from_sizes(sizes, dev)
         pb = ParticleBatch(torch.as_tensor(pos, dtype=torch.float32, device=dev),
@@ -264,6 +266,10 @@ class RigidGCMC:
         pos, q, sp, grp, sizes = self.configuration()
         dev = self.dev
         n = self.n_chain
+        if len(pos) == 0:
+            self.d["sk"].zero_()
+            self.d["energy"].zero_()
+            return
         off = offsets_from_sizes(sizes, dev)
         p = torch.as_tensor(pos, dtype=torch.float32, device=dev)
         qq = torch.as_tensor(q, dtype=torch.float32, device=dev)
@@ -290,8 +296,8 @@ class RigidGCMC:
             _lib.ptr(d["motif_species"]), self.n_species, _lib.ptr(d["sigma"]), _lib.ptr(d["epsilon"]),
             float(self.lj.cutoff), float(self.ewald_cutoff), _lib.ptr(d["alpha"]), _lib.ptr(d["tail"]),
             _lib.ptr(d["mol_const"]), _lib.ptr(d["kT"]), _lib.ptr(d["log_activity"]), self.nk,
-            self.nm, _lib.ptr(d["kn"]), _lib.ptr(d["pk"]), _lib.ptr(d["weights"]),
-            _lib.ptr(d["upper"]), _lib.ptr(d["guest"]), _lib.ptr(d["n_guest"]), _lib.ptr(d["counts"]),
+            self.nm, _lib.ptr(d["kn"]), _lib.ptr(d["pk"]), self._w_host.ctypes.data,
+            self._u_host.ctypes.data, _lib.ptr(d["guest"]), _lib.ptr(d["n_guest"]), _lib.ptr(d["counts"]),
             _lib.ptr(d["sk"]), _lib.ptr(d["energy"]), _lib.ptr(d["width"]), _lib.ptr(d["hist"]),
             _lib.ptr(d["hist_idx"]), _lib.ptr(d["stats"]), int(n_trace), _lib.ptr(trace)))
         self.step += int(n_steps)
