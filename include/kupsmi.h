@@ -221,7 +221,55 @@ int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t step0, uint6
                     const double* move_weights, const double* width_upper, float* guest,
                     int32_t* n_guest, int32_t* counts, double* structure_factor,
                     double* energy, float* width, uint32_t* hist, int32_t* hist_idx,
-                    long long* stats, int n_trace, double* trace);
+                    long long* stats, int n_trace, double* trace, const float* prop_cdf,
+                    int prop_nv, int* sample_count, int sample_cap, int32_t* sample_row,
+                    float* sample_y);
+
+/* ---- learned insertion proposal (no counterpart in the reference; it enters
+ * kUPS's MCMC through the ChangesFn proposal log ratio,
+ * core/propagator.py:230-235, added to the acceptance at 332-334) ----
+ * kupsmi_gcmc_run above takes the proposal table prop_cdf [n_chain x nv^3]
+ * (cumulative q over the fractional voxel grid, last entry 1): an insertion
+ * draws voxel v ~ q and a uniform offset inside it and adds
+ * -ln(q(v) nv^3) to log p; a deletion adds +ln(q(v) nv^3) for the voxel of the
+ * molecule's centre (detailed balance).  NULL: uniform insertion.  With
+ * sample_count != NULL every insertion trial appends (chain * nv^3 + voxel,
+ * -dE/kT) to sample_row/sample_y (up to sample_cap): the training set. */
+
+/* number of MLP parameters (32 -> 128 -> 128 -> 128 -> 1, SiLU): 37377 */
+int kupsmi_mlp_param_count(void);
+
+/* 32-bin radial host-density histogram (r < rmax, normalised by shell volume
+ * x rho0) around every voxel centre of every chain, BF16 [n_chain x nv^3 x 32]. */
+int kupsmi_voxel_descriptors(void* stream, int n_chain, int nv, int max_host,
+                             const int32_t* host_off, const float* host_xyzq,
+                             const double* tril, float rmax, float rho0, uint16_t* desc);
+
+/* MLP logits of n_rows descriptors on BF16 MFMA (FP32 accumulate).
+ * weights_bf16 / weights_f32: the flat parameter vector, BF16 and the same
+ * values as FP32. */
+int kupsmi_mlp_infer(void* stream, int n_rows, const uint16_t* desc,
+                     const uint16_t* weights_bf16, const float* weights_f32, float* logits);
+
+/* One training step on a minibatch of descriptor rows and targets:
+ * loss = 1/2 mean (f(x) - y)^2, forward/backward with the BF16-rounded
+ * weights, FP32 gradients (written to grad_out if non-NULL), Adam on the FP32
+ * master weights (apply != 0) and refresh of both rounded copies.
+ * loss_out (optional): per-workgroup loss sums. */
+int kupsmi_mlp_train_step(kupsmi_ctx* ctx, void* stream, int n_samples, const int32_t* rows,
+                          const float* targets, const uint16_t* desc, float* master,
+                          float* m1, float* m2, float* weights_f32, uint16_t* weights_bf16,
+                          float lr, int adam_step, int apply, float* grad_out,
+                          float* loss_out);
+
+/* Rounded copies (FP32 and BF16) of the FP32 master weights. */
+int kupsmi_mlp_round_weights(void* stream, const float* master, float* weights_f32,
+                             uint16_t* weights_bf16);
+
+/* Proposal table: cdf of q = (1 - mix) softmax(logits / temperature) + mix / nvox
+ * per chain (FP64 sums, fixed order). */
+int kupsmi_proposal_table(void* stream, int n_chain, int nvox, const float* logits,
+                          float temperature, float mix, float* cdf);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,13 @@ struct GcArgs {
   int32_t* hist_idx;         // n_chain x 4
   long long* stats;          // n_chain x 10: trials[4], accepted[4], ins, del
   double* trace;             // n_trace x n_steps x 4
+  // learned insertion proposal (NULL: uniform) and insertion-sample recording
+  const float* prop_cdf;     // n_chain x nv^3
+  int prop_nv;
+  int* sample_count;
+  int sample_cap;
+  int32_t* sample_row;       // chain * nv^3 + voxel of the insertion offset
+  float* sample_y;           // -dE / kT of the insertion trial
 };
 
 __device__ inline float2 gc_cmul(float2 a, float2 b) {
@@ -133,9 +140,25 @@ __device__ inline double gc_block_sum(double v, double* red) {
 // Out of line: it runs on one lane and its registers must not weigh on the
 // pair and k loops of the whole workgroup.
 struct GcProposal {
-  int move, deleting, n_old, n_new, slot;
+  int move, deleting, n_old, n_new, slot, voxel;
   uint32_t accept_bits;
+  double log_ratio;  // proposal log ratio ln(q(x|x') / q(x'|x)) of the move
 };
+
+// Voxel of a fractional insertion offset u in [0, 1)^3.
+__device__ inline int gc_voxel(float u0, float u1, float u2, int nv) {
+  int a = (int)(u0 * nv), b = (int)(u1 * nv), c = (int)(u2 * nv);
+  a = a < 0 ? 0 : (a >= nv ? nv - 1 : a);
+  b = b < 0 ? 0 : (b >= nv ? nv - 1 : b);
+  c = c < 0 ? 0 : (c >= nv ? nv - 1 : c);
+  return (a * nv + b) * nv + c;
+}
+
+// q(v) nvox of the learned table (the exact probability the sampler uses).
+__device__ inline double gc_qn(const float* cdf, int v, int nvox) {
+  const double lo = v > 0 ? (double)cdf[v - 1] : 0.0;
+  return ((double)cdf[v] - lo) * nvox;
+}
 
 __device__ __noinline__ GcProposal gc_propose(const GcArgs& a, const GcCell& cell, int ch, int64_t t,
                                               int ng, const float4* gst, const float* wid,
@@ -155,7 +178,8 @@ __device__ __noinline__ GcProposal gc_propose(const GcArgs& a, const GcCell& cel
   gc_draw(a.seed, (uint32_t)ch, t, GC_TAG_A, ra);
   gc_draw(a.seed, (uint32_t)ch, t, GC_TAG_B, rb);
   gc_draw(a.seed, (uint32_t)ch, t, GC_TAG_C, rcn);
-  int n_old = 0, n_new = 0, slot = -1;
+  int n_old = 0, n_new = 0, slot = -1, voxel = -1;
+  double log_ratio = 0.0;
   float oldp[GC_MSITE][3], newp[GC_MSITE][3];
   if (move < 3 || deleting) {
     if (ng > 0) {
@@ -166,6 +190,21 @@ __device__ __noinline__ GcProposal gc_propose(const GcArgs& a, const GcCell& cel
         oldp[s][0] = g.x; oldp[s][1] = g.y; oldp[s][2] = g.z;
       }
     }
+  }
+  if (move == 3 && deleting && n_old && a.prop_cdf) {
+    // reverse move = learned insertion at the molecule's centre
+    float rel[3], cen[3] = {0.f, 0.f, 0.f};
+    for (int s = 0; s < M; ++s) {
+      gc_minimg(cell, oldp[M - 1], oldp[s], rel);
+      for (int x = 0; x < 3; ++x) cen[x] += rel[x] / M;
+    }
+    float f0, f1, f2;
+    tmul(cell.Lif, oldp[M - 1][0] + cen[0], oldp[M - 1][1] + cen[1], oldp[M - 1][2] + cen[2],
+         f0, f1, f2);
+    f0 -= floorf(f0); f1 -= floorf(f1); f2 -= floorf(f2);
+    const int nv = a.prop_nv, nvox = nv * nv * nv;
+    const int v = gc_voxel(f0, f1, f2, nv);
+    log_ratio = log(gc_qn(a.prop_cdf + (size_t)ch * nvox, v, nvox));
   }
   if (move == 0 && n_old) {
     const float u0 = u01(rcn[0]), u1 = u01(rcn[1]), u2 = u01(rcn[2]), u3 = u01(rcn[3]);
@@ -204,8 +243,28 @@ __device__ __noinline__ GcProposal gc_propose(const GcArgs& a, const GcCell& cel
     const float uq[3] = {u01(rb[0]), u01(rb[1]), u01(rb[2])};
     float R[3][3];
     gc_rotation(uq, 1.0f, R);
+    float fu[3] = {u01(rcn[0]), u01(rcn[1]), u01(rcn[2])};
+    if (a.prop_cdf) {
+      // voxel ~ q (first v with u < cdf[v]), uniform offset inside it
+      const int nv = a.prop_nv, nvox = nv * nv * nv;
+      const float* cdf = a.prop_cdf + (size_t)ch * nvox;
+      const float uv = u01(rb[3]);
+      int lo = 0, hi = nvox - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (uv < cdf[mid]) hi = mid; else lo = mid + 1;
+      }
+      const int i0 = lo / (nv * nv), i1 = (lo / nv) % nv, i2 = lo % nv;
+      fu[0] = (i0 + fu[0]) / nv;
+      fu[1] = (i1 + fu[1]) / nv;
+      fu[2] = (i2 + fu[2]) / nv;
+      voxel = lo;
+      log_ratio = -log(gc_qn(cdf, lo, nvox));
+    } else if (a.prop_nv > 0) {
+      voxel = gc_voxel(fu[0], fu[1], fu[2], a.prop_nv);
+    }
     float off[3];
-    tmul(cell.Lf, u01(rcn[0]), u01(rcn[1]), u01(rcn[2]), off[0], off[1], off[2]);
+    tmul(cell.Lf, fu[0], fu[1], fu[2], off[0], off[1], off[2]);
     if (ng < a.gmax) {
       for (int s = 0; s < M; ++s) {
         const float* tp = a.motif_pos + 3 * s;
@@ -224,6 +283,8 @@ __device__ __noinline__ GcProposal gc_propose(const GcArgs& a, const GcCell& cel
   GcProposal pr;
   pr.move = move; pr.deleting = deleting; pr.n_old = n_old; pr.n_new = n_new; pr.slot = slot;
   pr.accept_bits = ra[1];
+  pr.voxel = voxel;
+  pr.log_ratio = log_ratio;
   return pr;
 }
 
@@ -234,8 +295,8 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
   __shared__ float4 site_old[GC_MSITE], site_new[GC_MSITE];
   __shared__ float2 tab[2 * GC_MSITE][3][GC_W];
   __shared__ double red[GC_THREADS / 64];
-  __shared__ int s_move, s_del, s_nold, s_nnew, s_slot, s_accept;
-  __shared__ double s_dE;
+  __shared__ int s_move, s_del, s_nold, s_nnew, s_slot, s_accept, s_voxel;
+  __shared__ double s_dE, s_lratio;
   const int tid = threadIdx.x;
   const int M = a.m, ns = a.nspec;
   float4* host = lds;
@@ -287,6 +348,8 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         const GcProposal pr = gc_propose(a, cell, ch, t, ng, gst, wid, site_old, site_new);
         s_move = pr.move; s_del = pr.deleting; s_nold = pr.n_old; s_nnew = pr.n_new;
         s_slot = pr.slot;
+        s_voxel = pr.voxel;
+        s_lratio = pr.log_ratio;
         s_accept = (int)pr.accept_bits;  // accept uniform bits, decided after the reduction
       }
       __syncthreads();
@@ -395,7 +458,14 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
           const double lnfV = a.log_activity[ch] + log(vol);
           logp += dn > 0 ? lnfV - log((double)(ng + 1)) : -lnfV + log((double)ng);
         }
-        logp += -dE / kT;
+        logp += -dE / kT + s_lratio;
+        if (a.sample_count && move == 3 && n_new && s_voxel >= 0) {
+          const int at = atomicAdd(a.sample_count, 1);
+          if (at < a.sample_cap) {
+            a.sample_row[at] = ch * (a.prop_nv * a.prop_nv * a.prop_nv) + s_voxel;
+            a.sample_y[at] = (float)(-dE / kT);
+          }
+        }
         const double lu = log((double)u01((uint32_t)s_accept));
         const int accept = logp > lu;
         if (ch < a.n_trace) {
@@ -485,7 +555,9 @@ extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t s
                                const double* width_upper, float* guest, int32_t* n_guest,
                                int32_t* counts, double* structure_factor, double* energy,
                                float* width, uint32_t* hist, int32_t* hist_idx,
-                               long long* stats, int n_trace, double* trace) {
+                               long long* stats, int n_trace, double* trace,
+                               const float* prop_cdf, int prop_nv, int* sample_count,
+                               int sample_cap, int32_t* sample_row, float* sample_y) {
   KUPSMI_REQUIRE(n_chain >= 0 && n_steps >= 0, "negative size");
   KUPSMI_REQUIRE(m >= 1 && m <= GC_MSITE, "molecules must have 1..4 sites");
   KUPSMI_REQUIRE(n_species >= 1 && n_species <= GC_NSPEC, "n_species must be in [1, 16]");
@@ -511,6 +583,11 @@ extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t s
   a.kn = kn; a.pk = pk; a.guest = guest; a.n_guest = n_guest; a.counts = counts;
   a.sk = structure_factor; a.energy = energy; a.width = width; a.hist = hist;
   a.hist_idx = hist_idx; a.stats = stats; a.trace = trace;
+  KUPSMI_REQUIRE(!prop_cdf || prop_nv >= 1, "a proposal table needs its voxel count");
+  KUPSMI_REQUIRE(!sample_count || (prop_nv >= 1 && sample_row && sample_y),
+                 "insertion samples need the voxel grid and output buffers");
+  a.prop_cdf = prop_cdf; a.prop_nv = prop_nv; a.sample_count = sample_count;
+  a.sample_cap = sample_cap; a.sample_row = sample_row; a.sample_y = sample_y;
   hipStream_t st = (hipStream_t)stream;
   int dev = 0, ncu = 256;
   KUPSMI_HIP(hipGetDevice(&dev));

@@ -285,9 +285,17 @@ class RigidGCMC:
         self.d["energy"].copy_(torch.as_tensor(self.full_energy(), device=dev))
 
     # ---- the MC cycle
-    def run(self, n_steps: int, n_trace: int = 0):
+    def run(self, n_steps: int, n_trace: int = 0, proposal=None, samples=None):
+        """Advance every chain by n_steps trial moves.  ``proposal``: a
+        LearnedProposal (its table biases insertions, with the exact density in
+        the acceptance ratio) or None (uniform).  ``samples``: an InsertionSamples
+        buffer that records (voxel row, -dE/kT) of every insertion trial."""
         d = self.d
         trace = torch.zeros((max(n_trace, 1), max(n_steps, 1), 4), dtype=torch.float64, device=self.dev)
+        cdf = proposal.cdf if proposal is not None else None
+        nv = proposal.nv if proposal is not None else (samples.nv if samples is not None else 0)
+        if samples is not None and proposal is not None and samples.nv != proposal.nv:
+            raise ValueError("samples and proposal use different voxel grids")
         M = len(self.motif.charges)
         _lib.check(_lib.load().kupsmi_gcmc_run(
             _lib.stream_ptr(), self.n_chain, int(n_steps), int(self.step), self.seed & (2**64 - 1),
@@ -299,7 +307,11 @@ class RigidGCMC:
             self.nm, _lib.ptr(d["kn"]), _lib.ptr(d["pk"]), self._w_host.ctypes.data,
             self._u_host.ctypes.data, _lib.ptr(d["guest"]), _lib.ptr(d["n_guest"]), _lib.ptr(d["counts"]),
             _lib.ptr(d["sk"]), _lib.ptr(d["energy"]), _lib.ptr(d["width"]), _lib.ptr(d["hist"]),
-            _lib.ptr(d["hist_idx"]), _lib.ptr(d["stats"]), int(n_trace), _lib.ptr(trace)))
+            _lib.ptr(d["hist_idx"]), _lib.ptr(d["stats"]), int(n_trace), _lib.ptr(trace),
+            _lib.ptr(cdf), int(nv), _lib.ptr(samples.count) if samples is not None else None,
+            int(samples.capacity) if samples is not None else 0,
+            _lib.ptr(samples.rows) if samples is not None else None,
+            _lib.ptr(samples.y) if samples is not None else None))
         self.step += int(n_steps)
         return trace[:n_trace] if n_trace else None
 

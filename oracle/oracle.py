@@ -273,3 +273,62 @@ def philox(seed: int, a: int, t: int, tag: int) -> np.ndarray:
     lib().orc_philox(C.c_uint64(int(seed) & (2**64 - 1)), C.c_uint32(a), C.c_int64(t),
                      C.c_uint32(tag), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- MLP --
+# FP64 restatement of the proposal network's forward pass and gradient
+# (textbook backpropagation; the reference has no such network).  Layout of the
+# flat parameter vector: W1[32][128] b1 W2[128][128] b2 W3[128][128] b3 w4[128] b4.
+
+def _unpack(p):
+    p = np.asarray(p, np.float64)
+    o = 0
+    out = []
+    for shape in ((32, 128), (128,), (128, 128), (128,), (128, 128), (128,), (128,), (1,)):
+        n = int(np.prod(shape))
+        out.append(p[o:o + n].reshape(shape))
+        o += n
+    assert o == p.size
+    return out
+
+
+def _silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def _dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def mlp_forward(params, X):
+    W1, b1, W2, b2, W3, b3, w4, b4 = _unpack(params)
+    h1 = _silu(X @ W1 + b1)
+    h2 = _silu(h1 @ W2 + b2)
+    h3 = _silu(h2 @ W3 + b3)
+    return h3 @ w4 + b4[0]
+
+
+def mlp_loss_and_grad(params, X, y):
+    """loss = 1/2 mean (f(X) - y)^2 and its gradient w.r.t. the flat parameters."""
+    W1, b1, W2, b2, W3, b3, w4, b4 = _unpack(params)
+    X = np.asarray(X, np.float64)
+    y = np.asarray(y, np.float64)
+    n = len(y)
+    z1 = X @ W1 + b1; h1 = _silu(z1)
+    z2 = h1 @ W2 + b2; h2 = _silu(z2)
+    z3 = h2 @ W3 + b3; h3 = _silu(z3)
+    out = h3 @ w4 + b4[0]
+    diff = out - y
+    loss = 0.5 * np.mean(diff**2)
+    dout = diff / n
+    gw4 = h3.T @ dout
+    gb4 = np.array([dout.sum()])
+    d3 = np.outer(dout, w4) * _dsilu(z3)
+    gW3 = h2.T @ d3; gb3 = d3.sum(0)
+    d2 = (d3 @ W3.T) * _dsilu(z2)
+    gW2 = h1.T @ d2; gb2 = d2.sum(0)
+    d1 = (d2 @ W2.T) * _dsilu(z1)
+    gW1 = X.T @ d1; gb1 = d1.sum(0)
+    g = np.concatenate([a.ravel() for a in (gW1, gb1, gW2, gb2, gW3, gb3, gw4, gb4)])
+    return loss, g

@@ -116,3 +116,22 @@ def test_batch_shared_move_type():
     assert len({tuple(w) for w in words}) == 3
     # counter-based: the same (seed, counter) always gives the same words
     np.testing.assert_array_equal(oracle.philox(5, 7, 9, 1), oracle.philox(5, 7, 9, 1))
+
+
+def test_oracle_mlp_gradient_finite_differences():
+    """The oracle's MLP gradient (used to check the GPU backward pass) against
+    central finite differences of its loss."""
+    from kups_amd.mcmc.proposal import init_params
+
+    p = init_params(1).astype(np.float64)
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(7, 32))
+    y = rng.normal(size=7)
+    _, g = oracle.mlp_loss_and_grad(p, X, y)
+    h = 1e-6
+    for idx in [0, 4100, 9224, 20000, 30000, 37300, 37376]:
+        q = p.copy(); q[idx] += h
+        lp = oracle.mlp_loss_and_grad(q, X, y)[0]
+        q[idx] -= 2 * h
+        lm = oracle.mlp_loss_and_grad(q, X, y)[0]
+        assert (lp - lm) / (2 * h) == pytest.approx(g[idx], rel=1e-5, abs=1e-9)
