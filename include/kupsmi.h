@@ -200,13 +200,16 @@ int kupsmi_integrate(void* stream, int phase, int integrator, int n_sys,
  *   motif_*        the guest template (m <= 4 sites, centred)
  *   kn / pk        per chain k list: nk packed (n0 | (n1+64) << 8 | (n2+64) << 16)
  *                  and FP64 prefactors P(k) (0 = padding); nm = max |n|
- *   guest          [n_chain x gmax x m x 3] f32, n_guest, counts[n_chain x n_species],
+ *   guest          [n_chain x gmax x m x 3] f64 (rigid-body moves in FP64, so molecules
+ *                  stay rigid), n_guest, counts[n_chain x n_species],
  *                  structure_factor [n_chain x nk x 2] f64, energy, width[n_chain x 4],
  *                  hist[n_chain x 4 x 4], hist_idx[n_chain x 4]: state, in/out
  *   move_weights   HOST array[4]: unnormalised weights of translation, rotation,
  *                  reinsertion, exchange (default 1, 1, 1, 3 in kUPS's mcmc app)
  *   width_upper    HOST array[4]: upper bounds of the adapted step widths
- *   stats          [n_chain x 10] int64: trials[4], accepted[4], insertions, deletions
+ *   stats          [n_chain x 10] int64: trials[4], accepted[4] (moves that changed the
+ *                  configuration; a move of an empty chain is accepted but not counted),
+ *                  insertions, deletions
  *   trace          optional [n_trace x n_steps x 4] f64 (move, dE, log p, accepted)
  * Random numbers: Philox4x32-10 keyed by seed, counters (chain, step, tag).
  */
@@ -218,18 +221,19 @@ int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t step0, uint6
                     double rc_coul, const double* alpha, const double* tail,
                     const double* mol_const, const double* kT, const double* log_activity,
                     int nk, int nm, const int32_t* kn, const double* pk,
-                    const double* move_weights, const double* width_upper, float* guest,
+                    const double* move_weights, const double* width_upper, double* guest,
                     int32_t* n_guest, int32_t* counts, double* structure_factor,
                     double* energy, float* width, uint32_t* hist, int32_t* hist_idx,
                     long long* stats, int n_trace, double* trace, const float* prop_cdf,
-                    int prop_nv, int* sample_count, int sample_cap, int32_t* sample_row,
-                    float* sample_y);
+                    const int32_t* prop_guide, int prop_nv, int* sample_count, int sample_cap,
+                    int32_t* sample_row, float* sample_y);
 
 /* ---- learned insertion proposal (no counterpart in the reference; it enters
  * kUPS's MCMC through the ChangesFn proposal log ratio,
  * core/propagator.py:230-235, added to the acceptance at 332-334) ----
  * kupsmi_gcmc_run above takes the proposal table prop_cdf [n_chain x nv^3]
- * (cumulative q over the fractional voxel grid, last entry 1): an insertion
+ * (cumulative q over the fractional voxel grid, last entry 1) and its guide
+ * table prop_guide [n_chain x nv^3] (both from kupsmi_proposal_table): an insertion
  * draws voxel v ~ q and a uniform offset inside it and adds
  * -ln(q(v) nv^3) to log p; a deletion adds +ln(q(v) nv^3) for the voxel of the
  * molecule's centre (detailed balance).  NULL: uniform insertion.  With
@@ -267,9 +271,11 @@ int kupsmi_mlp_round_weights(void* stream, const float* master, float* weights_f
                              uint16_t* weights_bf16);
 
 /* Proposal table: cdf of q = (1 - mix) softmax(logits / temperature) + mix / nvox
- * per chain (FP64 sums, fixed order). */
+ * per chain (FP64 sums, fixed order), and the guide table of the sampler:
+ * guide[j] = first voxel v with j / nvox < cdf[v] (an insertion starts its
+ * search for u there, so it reads ~2 entries instead of log2(nvox)). */
 int kupsmi_proposal_table(void* stream, int n_chain, int nvox, const float* logits,
-                          float temperature, float mix, float* cdf);
+                          float temperature, float mix, float* cdf, int32_t* guide);
 
 #ifdef __cplusplus
 }

@@ -46,7 +46,7 @@ _SIGNATURES = {
     "kupsmi_gcmc_run": (
         [vp, i32, i32, i64, u64, i32, vp, vp, vp, vp, i32, i32, vp, vp, vp, i32, vp, vp,
          C.c_double, C.c_double, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp,
-         vp, vp, vp, vp, vp, vp, i32, vp, vp, i32, vp, i32, vp, vp],
+         vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp, i32, vp, vp],
         C.c_int,
     ),
     "kupsmi_mlp_param_count": ([], C.c_int),
@@ -55,7 +55,7 @@ _SIGNATURES = {
     "kupsmi_mlp_train_step": (
         [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_float, i32, i32, vp, vp], C.c_int),
     "kupsmi_mlp_round_weights": ([vp, vp, vp, vp], C.c_int),
-    "kupsmi_proposal_table": ([vp, i32, i32, vp, C.c_float, C.c_float, vp], C.c_int),
+    "kupsmi_proposal_table": ([vp, i32, i32, vp, C.c_float, C.c_float, vp, vp], C.c_int),
     "kupsmi_integrate": (
         [vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64, vp],
         C.c_int,

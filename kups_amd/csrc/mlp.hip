@@ -342,6 +342,20 @@ __global__ __launch_bounds__(256) void ml_table_kernel(int nvox, const float* lo
   }
 }
 
+// guide[j] = first v with j / nvox < cdf[v] (binary search per entry).
+__global__ __launch_bounds__(256) void ml_guide_kernel(int nvox, const float* cdf, int32_t* guide) {
+  const int j = blockIdx.x * 256 + threadIdx.x, ch = blockIdx.y;
+  if (j >= nvox) return;
+  const float* c = cdf + (size_t)ch * nvox;
+  const float u = (float)j / (float)nvox;
+  int lo = 0, hi = nvox - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (u < c[mid]) hi = mid; else lo = mid + 1;
+  }
+  guide[(size_t)ch * nvox + j] = lo;
+}
+
 }  // namespace kupsmi
 
 using namespace kupsmi;
@@ -415,12 +429,16 @@ extern "C" int kupsmi_mlp_round_weights(void* stream, const float* master, float
 }
 
 extern "C" int kupsmi_proposal_table(void* stream, int n_chain, int nvox, const float* logits,
-                                     float temperature, float mix, float* cdf) {
+                                     float temperature, float mix, float* cdf, int32_t* guide) {
   KUPSMI_REQUIRE(n_chain >= 0 && nvox >= 1, "bad sizes");
   KUPSMI_REQUIRE(mix > 0.f && mix <= 1.f, "uniform mixing weight must be in (0, 1]");
   KUPSMI_REQUIRE(temperature > 0.f, "temperature must be positive");
+  KUPSMI_REQUIRE(guide != nullptr, "the guide table is required");
   if (n_chain == 0) return KUPSMI_OK;
   ml_table_kernel<<<n_chain, 256, 0, (hipStream_t)stream>>>(nvox, logits, temperature, mix, cdf);
+  KUPSMI_CHECK_LAUNCH();
+  ml_guide_kernel<<<dim3((nvox + 255) / 256, n_chain), 256, 0, (hipStream_t)stream>>>(nvox, cdf,
+                                                                                     guide);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }

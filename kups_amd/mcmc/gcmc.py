@@ -153,7 +153,7 @@ class RigidGCMC:
             epsilon=t(lj.epsilon, f64), alpha=t(self.alpha, f64), tail=t(self.tail, f64),
             mol_const=t(self.mol_const, f64), kT=t(kT, f64), log_activity=t(self.log_activity, f64),
             kn=t(kn, i32), pk=t(pk, f64),
-            guest=torch.zeros((n, self.gmax, len(motif.charges), 3), dtype=f32, device=dev),
+            guest=torch.zeros((n, self.gmax, len(motif.charges), 3), dtype=f64, device=dev),
             n_guest=torch.zeros(n, dtype=i32, device=dev), counts=t(counts, i32),
             sk=torch.zeros((n, nk, 2), dtype=f64, device=dev),
             energy=torch.zeros(n, dtype=f64, device=dev),
@@ -198,6 +198,7 @@ class RigidGCMC:
         ng = d["n_guest"].cpu().numpy()
         host = d["host"].cpu().numpy()
         hoff = d["host_off"].cpu().numpy()
+        hspec = d["host_species"].cpu().numpy()
         gpos = d["guest"].cpu().numpy()
         pos, q, sp, grp, sizes = [], [], [], [], []
         for s in range(n):
@@ -205,7 +206,7 @@ class RigidGCMC:
             g = gpos[s, :ng[s]].reshape(-1, 3)
             pos += [h[:, :3], g]
             q += [h[:, 3], np.tile(self.motif.charges, ng[s])]
-            sp += [d["host_species"].cpu().numpy()[hoff[s]:hoff[s + 1]], np.tile(self.guest_species, ng[s])]
+            sp += [hspec[hoff[s]:hoff[s + 1]], np.tile(self.guest_species, ng[s])]
             grp += [np.full(len(h), -1), np.repeat(np.arange(ng[s]), M)]
             sizes.append(len(h) + len(g))
         return (np.concatenate(pos), np.concatenate(q), np.concatenate(sp).astype(np.int32),
@@ -266,6 +267,7 @@ class RigidGCMC:
         pos, q, sp, grp, sizes = self.configuration()
         dev = self.dev
         n = self.n_chain
+        self._ng_ref = self.d["n_guest"].cpu().numpy().copy()
         if len(pos) == 0:
             self.d["sk"].zero_()
             self.d["energy"].zero_()
@@ -284,6 +286,29 @@ class RigidGCMC:
         self.d["sk"].copy_(torch.gather(sk, 1, gi[:, :, None].expand(-1, -1, 2)))
         self.d["energy"].copy_(torch.as_tensor(self.full_energy(), device=dev))
 
+    def energy_error(self) -> np.ndarray:
+        """Tracked energy minus a full re-evaluation, per chain (eV).  The full
+        evaluation holds E_sr(intra) of every guest, the incremental Delta E of
+        the kernel does not, so guests added since the last full evaluation are
+        accounted for."""
+        ng = self.d["n_guest"].cpu().numpy()
+        full = self.full_energy() - (ng - self._ng_ref) * self.sr_intramolecular()
+        return self.d["energy"].cpu().numpy() - full
+
+    _STATE = ("guest", "n_guest", "counts", "sk", "energy", "width", "hist", "hist_idx", "stats")
+
+    def snapshot(self) -> dict:
+        """Device copy of the mutable chain state (guests, S(k), energies,
+        schedulers, counters) and the step counter."""
+        snap = {k: self.d[k].clone() for k in self._STATE}
+        snap["_step"], snap["_ng_ref"] = self.step, self._ng_ref.copy()
+        return snap
+
+    def restore(self, snap: dict):
+        for k in self._STATE:
+            self.d[k].copy_(snap[k])
+        self.step, self._ng_ref = snap["_step"], snap["_ng_ref"].copy()
+
     # ---- the MC cycle
     def run(self, n_steps: int, n_trace: int = 0, proposal=None, samples=None):
         """Advance every chain by n_steps trial moves.  ``proposal``: a
@@ -293,6 +318,7 @@ class RigidGCMC:
         d = self.d
         trace = torch.zeros((max(n_trace, 1), max(n_steps, 1), 4), dtype=torch.float64, device=self.dev)
         cdf = proposal.cdf if proposal is not None else None
+        guide = proposal.guide if proposal is not None else None
         nv = proposal.nv if proposal is not None else (samples.nv if samples is not None else 0)
         if samples is not None and proposal is not None and samples.nv != proposal.nv:
             raise ValueError("samples and proposal use different voxel grids")
@@ -308,7 +334,7 @@ class RigidGCMC:
             self._u_host.ctypes.data, _lib.ptr(d["guest"]), _lib.ptr(d["n_guest"]), _lib.ptr(d["counts"]),
             _lib.ptr(d["sk"]), _lib.ptr(d["energy"]), _lib.ptr(d["width"]), _lib.ptr(d["hist"]),
             _lib.ptr(d["hist_idx"]), _lib.ptr(d["stats"]), int(n_trace), _lib.ptr(trace),
-            _lib.ptr(cdf), int(nv), _lib.ptr(samples.count) if samples is not None else None,
+            _lib.ptr(cdf), _lib.ptr(guide), int(nv), _lib.ptr(samples.count) if samples is not None else None,
             int(samples.capacity) if samples is not None else 0,
             _lib.ptr(samples.rows) if samples is not None else None,
             _lib.ptr(samples.y) if samples is not None else None))

@@ -132,7 +132,7 @@ def test_detailed_balance_ideal_gas_with_biased_proposal(gpu):
                                 device=gpu) * 3.0)
     from kups_amd import _lib
     _lib.check(_lib.load().kupsmi_proposal_table(_lib.stream_ptr(), n, lp.nvox, _lib.ptr(lp.logits),
-                                                 1.0, 0.05, _lib.ptr(lp.cdf)))
+                                                 1.0, 0.05, _lib.ptr(lp.cdf), _lib.ptr(lp.guide)))
     samples = []
     for _ in range(60):
         g.run(100, proposal=lp)
