@@ -1,4 +1,4 @@
-"""Benchmark: batched LJ molecular dynamics on one MI355X.
+"""Benchmark: batched LJ molecular dynamics and learned-proposal GCMC on one MI355X.
 
 Workload (the batched-MD configuration of the reference path): 2048
 independent cubic PBC boxes, atom counts N ~ U{64..4096} (seed 0, ~4.26 M
@@ -8,6 +8,16 @@ Maxwell-Boltzmann momenta at 94.4 K, NVE velocity Verlet, dt = 2 fs, FP32
 positions/forces with FP64 per-system energies, neighbor set rebuilt every
 step.  A "step" is one MD step of every system; all timed steps run in one
 persistent launch (``--steps-per-launch`` splits them).
+
+GCMC leg (``gcmc`` object of the line; ``--no-gcmc`` skips it): the
+learned-proposal workload, 4096 rigid hosts of 1500-3000 atoms (jittered cubic
+lattice, 30 % vacancies, 25-30 A triclinic cells, charges N(0, 0.3)
+neutralised), CO2 at 298 K and 0.1-10 bar, LJ + Ewald.  10^6 insertion samples
+from the plain kernel train the MLP proposal on the GPU; after equilibration
+with it, the uniform and the learned chains run the same number of trials from
+the same state (same random stream), and the line reports trials/s, accepted
+moves/s and accepted insertions/s of both, their time-averaged loadings with a
+paired z score, and the tracked-energy error against a full re-evaluation.
 
 Prints one JSON line (rank 0, N = 1 only).
 """
@@ -47,6 +57,14 @@ def parse():
     ap.add_argument("--cpu-sample-every", type=int, default=8,
                     help="CPU baseline runs every k-th system of the batch")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-gcmc", action="store_true")
+    ap.add_argument("--gcmc-chains", type=int, default=4096)
+    ap.add_argument("--gcmc-samples", type=int, default=1_000_000)
+    ap.add_argument("--gcmc-equil", type=int, default=20000, help="learned equilibration trials")
+    ap.add_argument("--gcmc-steps", type=int, default=20000, help="timed trials per chain and proposal")
+    ap.add_argument("--gcmc-chunk", type=int, default=500, help="trials per launch (loading samples)")
+    ap.add_argument("--gcmc-cpu-chains", type=int, default=64)
+    ap.add_argument("--gcmc-cpu-steps", type=int, default=1000)
     return ap.parse_args()
 
 
@@ -87,6 +105,141 @@ def cpu_baseline(batch, params, args):
                    f"{atoms} atoms x {args.cpu_steps} FP32 VV steps, cell list rebuilt every step, "
                    f"OpenMP over systems, {el:.1f} s; CPU restatement of kUPS, not kUPS"),
     }
+
+
+# MLP 32 -> 128 -> 128 -> 128 -> 1: multiply-adds per descriptor row
+MLP_FLOP_PER_ROW = 2.0 * (32 * 128 + 128 * 128 + 128 * 128 + 128)
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def gcmc_cpu_baseline(g, snap, args):
+    """The oracle's FP64 GCMC (OpenMP over chains) on a sample of the
+    equilibrated chains, uniform insertion, trials/s."""
+    import oracle
+
+    g.restore(snap)
+    idx = np.linspace(0, g.n_chain - 1, min(args.gcmc_cpu_chains, g.n_chain)).astype(int)
+    st = oracle.gcmc_state(g, idx)
+    t0 = time.perf_counter()
+    oracle.gcmc_run(st, args.gcmc_cpu_steps, g.step, g.seed)
+    el = time.perf_counter() - t0
+    trials = len(idx) * args.gcmc_cpu_steps
+    return {
+        "value": trials / el, "unit": "trials/s", "cores": oracle.num_threads(), "kind": "port",
+        "sample": f"{len(idx)} of {g.n_chain} equilibrated chains x {args.gcmc_cpu_steps} trials, "
+                  f"uniform insertion, FP64, OpenMP over chains, {el:.1f} s; CPU restatement of "
+                  "kUPS, not kUPS",
+    }
+
+
+def gcmc_leg(args):
+    import torch
+
+    from kups_amd import workloads
+    from kups_amd.mcmc.gcmc import make_co2_gcmc
+    from kups_amd.mcmc.proposal import InsertionSamples, LearnedProposal
+
+    sync = torch.cuda.synchronize
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    hosts = workloads.gcmc_hosts(args.gcmc_chains, seed=0)
+    g = make_co2_gcmc(hosts, gmax=64, seed=1)
+    lp = LearnedProposal(g)
+    sync()
+    setup_s = time.perf_counter() - t0
+    # 1. training set from the plain kernel
+    samples = InsertionSamples(args.gcmc_samples, lp.nv, g.dev)
+    t0 = time.perf_counter()
+    while samples.n < args.gcmc_samples:
+        g.run(200, samples=samples)
+    sync()
+    collect_s = time.perf_counter() - t0
+    # 2. training (GPU forward/backward + Adam) and the proposal table
+    t0 = time.perf_counter()
+    hist = lp.train(samples)
+    sync()
+    train_s = time.perf_counter() - t0
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    logits = lp.infer()
+    ev1.record(stream)
+    sync()
+    infer_ms = ev0.elapsed_time(ev1)
+    del logits
+    t0 = time.perf_counter()
+    lp.build_table()
+    sync()
+    table_s = time.perf_counter() - t0
+    # 3. equilibrate with the learned proposal, then run both from that state
+    for _ in range(0, args.gcmc_equil, 1000):
+        g.run(1000, proposal=lp)
+    sync()
+    snap = g.snapshot()
+    out = {}
+    loads = {}
+    for name, prop in (("uniform", None), ("learned", lp)):
+        g.restore(snap)
+        s0 = g.stats
+        acc = torch.zeros(g.n_chain, dtype=torch.float64, device=g.dev)
+        nchunk = max(1, args.gcmc_steps // args.gcmc_chunk)
+        sync()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(nchunk):
+            g.run(args.gcmc_chunk, proposal=prop)
+            acc += g.loading
+        e1.record(stream)
+        sync()
+        el = e0.elapsed_time(e1) * 1e-3
+        s1 = g.stats
+        trials = int((s1["trials"] - s0["trials"]).sum())
+        moved = int((s1["accepted"] - s0["accepted"]).sum())
+        ins = int((s1["insertions"] - s0["insertions"]).sum())
+        dele = int((s1["deletions"] - s0["deletions"]).sum())
+        loads[name] = (acc / nchunk).cpu().numpy()
+        err = np.abs(g.energy_error())
+        energy = np.abs(g.d["energy"].cpu().numpy())
+        out[name] = {
+            "trials_per_s": trials / el,
+            "accepted_moves_per_s": moved / el,
+            "accepted_insertions_per_s": ins / el,
+            "accepted_insertions": ins, "accepted_deletions": dele,
+            "seconds": el,
+            "loading_mean": float(loads[name].mean()),
+            "loading_sem": float(loads[name].std(ddof=1) / np.sqrt(g.n_chain)),
+            "energy_error_max_eV": float(err.max()),
+            "energy_error_max_rel": float((err / np.maximum(energy, 1.0)).max()),
+        }
+    d = loads["learned"] - loads["uniform"]
+    z = float(d.mean() / (d.std(ddof=1) / np.sqrt(len(d)))) if d.std() > 0 else 0.0
+    u, l = out["uniform"], out["learned"]
+    res = {
+        "workload": f"learned-proposal GCMC: {hosts.description}; CO2 (3 sites, q C +0.70 / "
+                    "O -0.35), 298 K, 0.1-10 bar log-uniform; LJ (TraPPE, LB mixing, rc 12 A, "
+                    "tail) + Ewald (rc 12 A, precision 1e-6); moves translation/rotation/"
+                    "reinsertion/exchange weights 1/1/1/3",
+        "n_chains": g.n_chain, "nk_max": g.nk,
+        "trials_per_chain": nchunk * args.gcmc_chunk,
+        "equilibration_trials": args.gcmc_equil,
+        "uniform": u, "learned": l,
+        "insertion_rate_ratio": l["accepted_insertions_per_s"] / max(u["accepted_insertions_per_s"], 1e-30),
+        "loading_paired_z": z,
+        "training": {"samples": samples.n, "epochs": len(hist), "loss": hist, "seconds": train_s,
+                     "collect_seconds": collect_s, "table_seconds": table_s,
+                     "voxels_per_chain": lp.nvox, "temperature": lp.temperature, "mix": lp.mix},
+        "setup_seconds": setup_s,
+        "mlp_roofline": {
+            "kernel": "ml_infer_kernel", "bound": "mfma", "rows": int(lp.desc.shape[0]),
+            "ms": infer_ms,
+            "achieved": MLP_FLOP_PER_ROW * lp.desc.shape[0] / (infer_ms * 1e-3) / 1e12,
+            "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+        },
+    }
+    res["mlp_roofline"]["frac"] = res["mlp_roofline"]["achieved"] / PEAK_BF16_TFLOPS
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = gcmc_cpu_baseline(g, snap, args)
+    return res
 
 
 def load_traffic():
@@ -188,6 +341,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(batch, params, args)
+    if not args.no_gcmc:
+        del st, prop
+        torch.cuda.empty_cache()
+        line["gcmc"] = gcmc_leg(args)
     print(json.dumps(line), flush=True)
 
 

@@ -267,6 +267,43 @@ def gcmc_run(state: dict, n_steps: int, step0: int, seed: int, n_trace: int = 0)
     return trace[:n_trace]
 
 
+def gcmc_state(g, chains=None) -> dict:
+    """The oracle's state dict for chains of a kups_amd RigidGCMC (all chains by
+    default), copied from its device arrays: S(k) scattered back onto the
+    oracle's full k grid, host arrays re-based to the selected chains."""
+    d = g.d
+    idx = np.arange(g.n_chain) if chains is None else np.asarray(chains)
+    cpu = lambda k: d[k].cpu().numpy()
+    nm = g.nm
+    w = 2 * nm + 1
+    skc = cpu("sk")[idx]
+    sk = np.zeros((len(idx), (nm + 1) * w * w, 2))
+    for r, s in enumerate(idx):
+        sk[r, g._kgrid_index[s]] = skc[r]
+    off = cpu("host_off")
+    host = cpu("host").astype(np.float64)
+    spec = cpu("host_species")
+    rows = np.concatenate([np.arange(off[s], off[s + 1]) for s in idx]).astype(np.int64)
+    sizes = off[idx + 1] - off[idx]
+    return dict(
+        host_off=np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32),
+        host_pos=np.ascontiguousarray(host[rows, :3]), host_q=np.ascontiguousarray(host[rows, 3]),
+        host_spec=spec[rows], tril=cpu("tril")[idx],
+        motif_pos=cpu("motif_pos").astype(np.float64), motif_q=cpu("motif_q").astype(np.float64),
+        motif_spec=cpu("motif_species"), sigma=g.lj.sigma, epsilon=g.lj.epsilon,
+        alpha=g.alpha[idx], kmax=g.kmax[idx], tail=g.tail[idx], mol_const=g.mol_const[idx],
+        kT=cpu("kT")[idx], log_activity=g.log_activity[idx],
+        guest_pos=np.ascontiguousarray(cpu("guest")[idx].astype(np.float64)),
+        n_guest=np.ascontiguousarray(cpu("n_guest")[idx]),
+        counts=np.ascontiguousarray(cpu("counts")[idx]), sk=sk,
+        energy=np.ascontiguousarray(cpu("energy")[idx]), width=np.ascontiguousarray(cpu("width")[idx]),
+        hist=np.ascontiguousarray(cpu("hist")[idx].view(np.uint32)),
+        hist_idx=np.ascontiguousarray(cpu("hist_idx")[idx]),
+        stats=np.zeros((len(idx), 8), np.int64), ins_acc=np.zeros(len(idx), np.int64),
+        del_acc=np.zeros(len(idx), np.int64), rc_lj=g.lj.cutoff, rc_cl=g.ewald_cutoff,
+        grid=(nm, nm, nm), w_move=g.weights, upper=g.upper)
+
+
 def philox(seed: int, a: int, t: int, tag: int) -> np.ndarray:
     """The four Philox4x32-10 words of counter (a, t, t >> 32, tag)."""
     out = np.zeros(4, np.uint32)

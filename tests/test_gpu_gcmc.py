@@ -23,33 +23,7 @@ pytestmark = pytest.mark.gpu
 
 
 def oracle_state(g):
-    d = g.d
-    f32 = lambda t: t.cpu().numpy().astype(np.float64)
-    n = g.n_chain
-    nm = g.nm
-    w = 2 * nm + 1
-    nk_grid = (nm + 1) * w * w
-    sk = np.zeros((n, nk_grid, 2))
-    gi = g._kgrid_index
-    skc = d["sk"].cpu().numpy()
-    for s in range(n):
-        sk[s, gi[s]] = skc[s]
-    host = f32(d["host"])
-    return dict(
-        host_off=d["host_off"].cpu().numpy(), host_pos=host[:, :3].copy(), host_q=host[:, 3].copy(),
-        host_spec=d["host_species"].cpu().numpy(), tril=d["tril"].cpu().numpy(),
-        motif_pos=f32(d["motif_pos"]), motif_q=f32(d["motif_q"]),
-        motif_spec=d["motif_species"].cpu().numpy(), sigma=g.lj.sigma, epsilon=g.lj.epsilon,
-        alpha=g.alpha, kmax=g.kmax, tail=g.tail, mol_const=g.mol_const, kT=d["kT"].cpu().numpy(),
-        log_activity=g.log_activity, guest_pos=f32(d["guest"]).copy(),
-        n_guest=d["n_guest"].cpu().numpy().copy(), counts=d["counts"].cpu().numpy().copy(),
-        sk=sk, energy=d["energy"].cpu().numpy().copy(),
-        width=d["width"].cpu().numpy().copy(),
-        hist=d["hist"].cpu().numpy().view(np.uint32).copy(),
-        hist_idx=d["hist_idx"].cpu().numpy().copy(),
-        stats=np.zeros((n, 8), np.int64), ins_acc=np.zeros(n, np.int64),
-        del_acc=np.zeros(n, np.int64), rc_lj=g.lj.cutoff, rc_cl=g.ewald_cutoff,
-        grid=(nm, nm, nm), w_move=g.weights, upper=g.upper)
+    return oracle.gcmc_state(g)
 
 
 def _small_gcmc(seed=0, n_chain=4, pressure=1e6, nmin=300, nmax=400, gmax=64):
