@@ -72,6 +72,25 @@ int kupsmi_neighborlist(kupsmi_ctx* ctx, void* stream, int n_sys,
                         int32_t* edges_ij, int32_t* edges_shift, int64_t* count);
 
 /*
+ * Morse bond energy, forces and virial over an explicit bond list.  Replaces
+ * morse_bond_energy (potential/classical/morse.py:142-168) and its gradient:
+ * for bond b = (i, j) with image shift s_b, d = r_j - r_i + s_b @ L and
+ * E = D [1 - exp(-alpha (r - r0))]^2 with D/alpha/r0 taken from the
+ * (n_species x n_species) tables at (species[i], species[j]); every bond is
+ * counted once (no 1/2).  bonds: n_bonds x 2 int32 global particle indices
+ * (particles sorted by system, sys_off as everywhere); shifts: n_bonds x 3
+ * int32 or NULL.  Bond vector and energy in FP64; forces (f32) and per-system
+ * energy/virial (f64) are added with atomics (summation order not fixed).
+ * accumulate = 0 clears forces, energy and virial first.
+ */
+int kupsmi_morse_bonds(void* stream, int n_sys, const int32_t* sys_off, int n_atoms,
+                       const float* pos, const int32_t* species, int n_species,
+                       const double* D, const double* alpha, const double* r0,
+                       long long n_bonds, const int32_t* bonds, const int32_t* shifts,
+                       const double* tril, int accumulate, float* forces, double* energy,
+                       double* virial);
+
+/*
  * Lennard-Jones energy, forces and virial of every system at the given
  * positions.  Replaces the LJ Potential (potential/classical/lennard_jones.py:
  * 149-178 energy, differentiated by potential/common/energy.py:263-355):

@@ -49,6 +49,9 @@ _SIGNATURES = {
          vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp, i32, vp, vp],
         C.c_int,
     ),
+    "kupsmi_morse_bonds": (
+        [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, C.c_longlong, vp, vp, vp, i32, vp, vp, vp],
+        C.c_int),
     "kupsmi_mlp_param_count": ([], C.c_int),
     "kupsmi_voxel_descriptors": ([vp, i32, i32, i32, vp, vp, vp, C.c_float, C.c_float, vp], C.c_int),
     "kupsmi_mlp_infer": ([vp, i32, vp, vp, vp, vp], C.c_int),

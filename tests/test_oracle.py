@@ -361,3 +361,60 @@ def test_baoab_o_step_formula():
     np.testing.assert_allclose(p, pe, rtol=1e-12)
     x = pos0 + np.array([[1.0, 2, 3], [-1, 0.5, 0.2]]) / m[:, None] * dt / 2 + pe / m[:, None] * dt / 2
     np.testing.assert_allclose(pos, x, rtol=1e-12, atol=1e-12)
+
+
+def test_morse_parameter_constructors():
+    # test_morse.py:126-133 (from_harmonic) and UFF equations 2 and 4
+    from kups_amd.potential.morse import MorseBondParameters, uff_bond_length
+
+    r0 = np.array([[1.0, 1.2], [1.2, 1.5]])
+    k = np.array([[100.0, 80.0], [80.0, 120.0]])
+    D = np.array([[2.0, 1.5], [1.5, 2.5]])
+    p = MorseBondParameters.from_harmonic(("A", "B"), r0, k, D)
+    np.testing.assert_allclose(p.alpha, np.sqrt(k / D))
+    # equal electronegativity: no correction, r_ij = r_i + r_j
+    r = uff_bond_length([0.7, 0.9], [2.5, 2.5])
+    np.testing.assert_allclose(r, [[1.4, 1.6], [1.6, 1.8]])
+    # C-O (UFF radii 0.757, 0.658; GMP chi 5.343, 8.741): r_EN by hand
+    r = uff_bond_length([0.757, 0.658], [5.343, 8.741])
+    ren = 0.757 * 0.658 * (np.sqrt(5.343) - np.sqrt(8.741)) ** 2 / (5.343 * 0.757 + 8.741 * 0.658)
+    assert r[0, 1] == pytest.approx(0.757 + 0.658 - ren, rel=1e-14)
+    assert r[0, 1] == r[1, 0]
+    # near equilibrium Morse ~ harmonic with k = 2 D alpha^2 (U = k/2 x^2)
+    pu = MorseBondParameters.from_uff(("C", "O"), [0.757, 0.658], [5.343, 8.741], [1.912, 2.3],
+                                      np.full((2, 2), 3.0))
+    kk = 664.12 * 1.912 * 2.3 / pu.r0[0, 1] ** 3
+    assert 2 * pu.D[0, 1] * pu.alpha[0, 1] ** 2 == pytest.approx(kk, rel=1e-12)
+
+
+def test_pressure_observables():
+    """Ideal gas: P = N kT / V from K = 3/2 N kT with zero stress; LJ: the
+    configurational pressure tr(W / V) / 3 equals -dU/dV under isotropic
+    scaling of cell and positions (central differences of the oracle energy)."""
+    import torch
+
+    from kups_amd.core.constants import BOLTZMANN_CONSTANT
+    from kups_amd.md.observables import instantaneous_pressure, stress_via_virial, volume
+
+    kT = 300.0 * BOLTZMANN_CONSTANT
+    tril = torch.tensor([[20.0, 0.0, 21.0, 0.0, 0.0, 22.0]], dtype=torch.float64)
+    V = volume(tril)
+    P = instantaneous_pressure(torch.tensor([1.5 * 100 * kT], dtype=torch.float64),
+                               torch.zeros((1, 3, 3), dtype=torch.float64), V)
+    assert P.item() == pytest.approx(100 * kT / (20 * 21 * 22), rel=1e-14)
+
+    b = workloads.argon_fcc(4)
+    pos = b.positions.astype(np.float64)
+    rc = [workloads.AR_CUTOFF]
+    sig, eps = np.array([[workloads.AR_SIGMA]]), np.array([[workloads.AR_EPS]])
+    e, f, w = oracle.lj_cells(pos, np.zeros(len(pos), np.int32), b.sys_off, b.tril, rc, sig, eps)
+    V0 = b.tril[0, 0] * b.tril[0, 2] * b.tril[0, 5]
+    p_conf = np.trace(stress_via_virial(torch.as_tensor(w.reshape(1, 9)),
+                                        torch.as_tensor(b.tril)).numpy()[0]) / 3
+    h = 1e-5
+    es = []
+    for s in (1 + h, 1 - h):
+        es.append(oracle.lj_cells(pos * s, np.zeros(len(pos), np.int32), b.sys_off, b.tril * s, rc,
+                                  sig, eps)[0][0])
+    dUdV = (es[0] - es[1]) / (V0 * ((1 + h) ** 3 - (1 - h) ** 3))
+    assert p_conf == pytest.approx(-dUdV, rel=1e-6)
