@@ -174,6 +174,40 @@ int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
                        double* virial);
 
 /*
+ * The same evaluation over a Verlet list of radius rc + skin kept in a
+ * kupsmi_verlet handle and reused across calls.  Replaces the per-step
+ * rebuild of CellListNeighborList (core/neighborlist.py:985-1077) with the
+ * neighbor-list reuse of a skin (no counterpart in the reference): each call
+ * moves the stored positions by the lattice vectors the caller's wrap added,
+ * and rebuilds the list of a system on the device once one of its atoms has
+ * moved more than skin / 2 since the last build, so every pair within the
+ * cutoff is in the list and the results equal kupsmi_pair_forces'.
+ * max_neighbors: list entries per atom; an atom with more candidates within
+ * rc + skin is evaluated over the cell stencil instead (exact, slower).
+ * Between calls only positions may change (cells, charges, species,
+ * exclusion ids and sizes fixed); kupsmi_verlet_invalidate forces a rebuild.
+ * The handle allocates its buffers on first use (and when n_sys, n_atoms,
+ * max_neighbors or the largest system grow) and is bound to one device and
+ * to one stream's sequence of calls.
+ */
+typedef struct kupsmi_verlet kupsmi_verlet;
+int kupsmi_verlet_create(kupsmi_verlet** out);
+int kupsmi_verlet_destroy(kupsmi_verlet* v);
+int kupsmi_verlet_invalidate(kupsmi_verlet* v, void* stream);
+/* out[0]: system rebuilds, out[1]: atom lists that overflowed (host array,
+ * synchronises the stream). */
+int kupsmi_verlet_counters(kupsmi_verlet* v, void* stream, unsigned long long* out);
+int kupsmi_pair_forces_verlet(kupsmi_verlet* v, void* stream, int n_sys, const int32_t* sys_off,
+                              int n_atoms, int max_atoms_per_sys, const float* pos,
+                              const float* charges, const int32_t* species, int n_species,
+                              const double* sigma, const double* epsilon,
+                              const double* lj_cutoff, const int32_t* lj_exclusion,
+                              const double* alpha, const double* coul_cutoff,
+                              const int32_t* coul_exclusion, const double* tril, double skin,
+                              int max_neighbors, int accumulate, float* forces, double* energy,
+                              double* virial);
+
+/*
  * Reciprocal-space Ewald energy, forces, virial and structure factors.
  * Replaces ewald_long_range_energy (potential/classical/ewald.py:585-601,
  * structure factor 394-439, prefactor 363-391) and its gradient.

@@ -49,6 +49,14 @@ _SIGNATURES = {
          vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp, i32, vp, vp],
         C.c_int,
     ),
+    "kupsmi_verlet_create": ([vp], C.c_int),
+    "kupsmi_verlet_destroy": ([vp], C.c_int),
+    "kupsmi_verlet_invalidate": ([vp, vp], C.c_int),
+    "kupsmi_verlet_counters": ([vp, vp, vp], C.c_int),
+    "kupsmi_pair_forces_verlet": (
+        [vp, vp, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
+         i32, i32, vp, vp, vp],
+        C.c_int),
     "kupsmi_morse_bonds": (
         [vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, C.c_longlong, vp, vp, vp, i32, vp, vp, vp],
         C.c_int),

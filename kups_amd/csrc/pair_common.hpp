@@ -1,0 +1,257 @@
+// Pair-path helpers shared by the cell-list kernel (pairs.hip) and the
+// Verlet-list kernel (verlet.hip): constants, block scan, cell binning, the
+// FP64 re-decision of the cutoff band, the minimum-image test and the LJ +
+// real-space Ewald terms of one directed pair.
+#pragma once
+#include "common.hpp"
+
+namespace kupsmi {
+
+constexpr int PC_BIN_THREADS = 1024;
+constexpr int PC_NMAX = 16384;
+constexpr int PC_MAX_NC_AXIS = 16;
+constexpr int PC_NCMAX = PC_MAX_NC_AXIS * PC_MAX_NC_AXIS * PC_MAX_NC_AXIS;
+constexpr int PC_TILE = 256;
+constexpr int PC_NSPEC = 8;
+constexpr double PC_BAND64 = 1e-12;
+constexpr int PC_NPART = 8;  // e_lj, e_real, w00 w01 w02 w11 w12 w22
+
+// Exclusive scan of cnt[0..n) into start[0..n], n <= 4 * blockDim.
+__device__ void block_scan4(const int* cnt, int* start, int n, int* wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x / 64;
+  int v4[4], own = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int c = tid * 4 + k;
+    v4[k] = c < n ? cnt[c] : 0;
+    own += v4[k];
+  }
+  int v = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) wsum[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    int w = lane < nw ? wsum[lane] : 0;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(w, o, 64);
+      if (lane >= o) w += y;
+    }
+    if (lane < nw) wsum[lane] = w;
+  }
+  __syncthreads();
+  int ex = v - own + (wv > 0 ? wsum[wv - 1] : 0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int c = tid * 4 + k;
+    if (c < n) start[c] = ex;
+    ex += v4[k];
+    if (c == n - 1) start[n] = ex;
+  }
+}
+
+__device__ inline int cell_of(const SysGeom& g, float x, float y, float z, float& wx, float& wy,
+                              float& wz) {
+  float f0, f1, f2;
+  wx = x; wy = y; wz = z;
+  wrap_shift(g.L, g.Lif, wx, wy, wz, f0, f1, f2);
+  return (cell_index(f2, g.nc[2]) * g.nc[1] + cell_index(f1, g.nc[1])) * g.nc[0] +
+         cell_index(f0, g.nc[0]);
+}
+
+// LDS of the per-system counting sort.
+struct PcBinLds {
+  int cnt[PC_NCMAX];
+  int start[PC_NCMAX + 1];
+  short keys[PC_NMAX];
+  int wsum[PC_BIN_THREADS / 64];
+};
+
+// Sorted copies of a system's atoms (global arrays indexed i0 + d) and its
+// cell starts cs[0..ncell].
+struct PcSorted {
+  float4* spos;  // wrapped (x, y, z, q)
+  int* sidx;     // local index of the atom in the caller's order
+  int* scell;
+  int* sspec;
+  int* slj;      // NULL when unused
+  int* scl;
+  int* cs;
+};
+
+// One workgroup (PC_BIN_THREADS) bins system i0..i0+n into cells of g:
+// wrap, stable counting sort (ties keep atom order), sorted copies.
+__device__ inline void pc_bin_system(const SysGeom& g, int i0, int n, const float* pos,
+                                     const float* q, const int32_t* species,
+                                     const int32_t* lj_excl, const int32_t* cl_excl, PcSorted o,
+                                     PcBinLds& L) {
+  const int tid = threadIdx.x;
+  const int ncell = g.ncell;
+  if (n <= 0 || n > PC_NMAX) {
+    for (int c = tid; c <= ncell; c += PC_BIN_THREADS) o.cs[c] = 0;
+    return;
+  }
+  for (int c = tid; c < ncell; c += PC_BIN_THREADS) L.cnt[c] = 0;
+  __syncthreads();
+  float wx, wy, wz;
+  for (int i = tid; i < n; i += PC_BIN_THREADS) {
+    const float* r = pos + 3 * (size_t)(i0 + i);
+    atomicAdd(&L.cnt[cell_of(g, r[0], r[1], r[2], wx, wy, wz)], 1);
+  }
+  __syncthreads();
+  block_scan4(L.cnt, L.start, ncell, L.wsum);
+  __syncthreads();
+  for (int c = tid; c < ncell; c += PC_BIN_THREADS) L.cnt[c] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += PC_BIN_THREADS) {
+    const float* r = pos + 3 * (size_t)(i0 + i);
+    const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
+    L.keys[L.start[c] + atomicAdd(&L.cnt[c], 1)] = (short)i;
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += PC_BIN_THREADS) {
+    const float* r = pos + 3 * (size_t)(i0 + i);
+    const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
+    const int b = L.start[c], e = L.start[c + 1];
+    int rank = 0;
+    for (int t = b; t < e; ++t) rank += (int)(unsigned short)L.keys[t] < i;
+    const int d = i0 + b + rank;
+    o.spos[d] = make_float4(wx, wy, wz, q ? q[i0 + i] : 0.0f);
+    o.sidx[d] = i;
+    o.scell[d] = c;
+    o.sspec[d] = species ? species[i0 + i] : 0;
+    if (lj_excl) o.slj[d] = lj_excl[i0 + i];
+    if (cl_excl) o.scl[d] = cl_excl[i0 + i];
+  }
+  for (int c = tid; c <= ncell; c += PC_BIN_THREADS) o.cs[c] = L.start[c];
+}
+
+// FP64 re-decision |(f_i - f_j - s) @ L|^2 < rc2 (core/neighborlist.py:542-564).
+__device__ inline bool keep64(const SysGeom& g, float4 ri, float4 rj, int sx, int sy, int sz,
+                              double rc2) {
+  asm volatile("" ::: "memory");
+  double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
+  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
+  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
+  tmul(g.L, fi0 - fj0 - sx, fi1 - fj1 - sy, fi2 - fj2 - sz, d0, d1, d2);
+  return d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+}
+
+// Minimum-image test of a replicated system's pair: s == round(f_i - f_j).
+__device__ inline bool is_min_image(const SysGeom& g, float4 ri, float4 rj, int sx, int sy,
+                                    int sz) {
+  double fi0, fi1, fi2, fj0, fj1, fj2;
+  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
+  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
+  return rint(fi0 - fj0) == sx && rint(fi1 - fj1) == sy && rint(fi2 - fj2) == sz;
+}
+
+// Per-system constants of the pair terms (cutoffs, FP64 bands, Ewald alpha).
+struct PairParams {
+  int has_lj, has_coul, has_lj_excl, has_coul_excl, ns;
+  double lj_rc2, lj_lo, lj_hi, cl_rc2, cl_lo, cl_hi, all_hi;
+  float alpha, two_a_sqrtpi, alpha2;
+  const float *sig2, *e4, *e24;  // LDS tables
+};
+
+__device__ inline void pc_pair_params(PairParams& pp, int has_lj, int has_coul, int has_lj_excl,
+                                      int has_coul_excl, const double* lj_rc, const double* cl_rc,
+                                      const double* alpha, int s, int ns, const float* sig2,
+                                      const float* e4, const float* e24) {
+  pp.has_lj = has_lj; pp.has_coul = has_coul;
+  pp.has_lj_excl = has_lj_excl; pp.has_coul_excl = has_coul_excl; pp.ns = ns;
+  pp.lj_rc2 = has_lj ? lj_rc[s] * lj_rc[s] : 0.0;
+  const double crc = has_coul ? cl_rc[s] : 0.0;
+  pp.cl_rc2 = crc * crc;
+  // FP64 band around each cutoff re-decided with the reference's fractional
+  // formula (the Cartesian FP64 r^2 differs from it by a few ulp)
+  pp.lj_hi = pp.lj_rc2 * (1.0 + PC_BAND64); pp.lj_lo = pp.lj_rc2 * (1.0 - PC_BAND64);
+  pp.cl_hi = pp.cl_rc2 * (1.0 + PC_BAND64); pp.cl_lo = pp.cl_rc2 * (1.0 - PC_BAND64);
+  pp.all_hi = fmax(pp.lj_hi, pp.cl_hi);
+  pp.alpha = has_coul ? (float)alpha[s] : 0.0f;
+  pp.two_a_sqrtpi = 2.0f * pp.alpha * 0.5641895835477563f;
+  pp.alpha2 = pp.alpha * pp.alpha;
+  pp.sig2 = sig2; pp.e4 = e4; pp.e24 = e24;
+}
+
+// LJ and real-space Ewald terms of the directed pair (i -> j + s): adds the
+// pair energies to elj / ecl and returns 2 d(E_directed)/d(r^2).  r2d is the
+// FP64 squared distance, r2 its FP32 rounding; exclusion ids are compared on
+// the minimum image only (core/neighborlist.py:731-735).
+template <bool EXCL>
+__device__ inline float pc_pair_terms(const PairParams& pp, const SysGeom& g, float4 ri, float4 rj,
+                                      int sx, int sy, int sz, double r2d, float r2, bool self,
+                                      int si, int sj, int xlj_i, int xlj_j, int xcl_i, int xcl_j,
+                                      float& elj, float& ecl) {
+  const float K = 14.399645351950548f;  // HARTREE * BOHR, eV A
+  bool min_img = true;
+  if (EXCL && g.rep) min_img = is_min_image(g, ri, rj, sx, sy, sz);
+  float gsum = 0.f;
+  if (pp.has_lj && r2d < pp.lj_hi && !self) {
+    bool ok = !(r2d > pp.lj_lo) || keep64(g, ri, rj, sx, sy, sz, pp.lj_rc2);
+    if (EXCL && pp.has_lj_excl && min_img && xlj_j == xlj_i) ok = false;
+    if (ok) {
+      const int p = si * pp.ns + sj;
+      const float ir2 = 1.0f / r2;
+      float c6 = pp.sig2[p] * ir2;
+      c6 = c6 * c6 * c6;
+      elj += pp.e4[p] * c6 * (c6 - 1.0f);
+      gsum += pp.e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;
+    }
+  }
+  if (pp.has_coul && r2d < pp.cl_hi && !self) {
+    bool ok = !(r2d > pp.cl_lo) || keep64(g, ri, rj, sx, sy, sz, pp.cl_rc2);
+    if (EXCL && pp.has_coul_excl && min_img && xcl_j == xcl_i) ok = false;
+    if (ok) {
+      const float r = sqrtf(r2);
+      const float ir = 1.0f / r;
+      const float qq = ri.w * rj.w * K;
+      const float ec = erfcf(pp.alpha * r);
+      ecl += qq * ec * ir;
+      // dU/dr = qq (-2a/sqrt(pi) e^{-a^2 r^2} / r - erfc / r^2); 2 u' = dU/dr / r
+      gsum += qq * (-pp.two_a_sqrtpi * __expf(-pp.alpha2 * r2) - ec * ir) * ir * ir;
+    }
+  }
+  return gsum;
+}
+
+// Per-block FP64 partials of a tile (fixed order: wave sums, then the waves
+// in order) written to part[0..PC_NPART).
+template <int TILE>
+__device__ inline void pc_block_partials(const double (&v)[PC_NPART], double (*red)[PC_NPART],
+                                         double* part) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < PC_NPART; ++k) {
+    const double x = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = x;
+  }
+  __syncthreads();
+  if (tid < PC_NPART) {
+    double x = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) x += red[w][tid];
+    part[tid] = x;
+  }
+}
+
+// Fixed-order sum of the tile partials of system s into energy[2 s..] (LJ,
+// real space) and the symmetric virial[9 s..].
+__device__ inline void pc_reduce_system(int s, int tiles, const double* partial, int accumulate,
+                                        double* energy, double* virial) {
+  double acc[PC_NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = 0; t < tiles; ++t)
+    for (int k = 0; k < PC_NPART; ++k) acc[k] += partial[((size_t)s * tiles + t) * PC_NPART + k];
+  if (energy) {
+    double* e = energy + 2 * (size_t)s;
+    if (accumulate) { e[0] += acc[0]; e[1] += acc[1]; } else { e[0] = acc[0]; e[1] = acc[1]; }
+  }
+  if (virial) {
+    const double w[9] = {acc[2], acc[3], acc[4], acc[3], acc[5], acc[6], acc[4], acc[6], acc[7]};
+    double* W = virial + 9 * (size_t)s;
+    for (int k = 0; k < 9; ++k) W[k] = accumulate ? W[k] + w[k] : w[k];
+  }
+}
+
+}  // namespace kupsmi

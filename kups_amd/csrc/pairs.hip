@@ -18,18 +18,9 @@
 // K/2 sum q_i q_j erfc(alpha r)/r [r < rc]), neighbor semantics of
 // core/neighborlist.py:692-786 (strict cutoff, image grid for narrow cells,
 // exclusion on the minimum image only, 731-735).
-#include "common.hpp"
+#include "pair_common.hpp"
 
 namespace kupsmi {
-
-constexpr int PC_BIN_THREADS = 1024;
-constexpr int PC_NMAX = 16384;
-constexpr int PC_MAX_NC_AXIS = 16;
-constexpr int PC_NCMAX = PC_MAX_NC_AXIS * PC_MAX_NC_AXIS * PC_MAX_NC_AXIS;
-constexpr int PC_TILE = 256;
-constexpr int PC_NSPEC = 8;
-constexpr double PC_BAND64 = 1e-12;
-constexpr int PC_NPART = 8;  // e_lj, e_real, w00 w01 w02 w11 w12 w22
 
 struct PairArgs {
   int n_sys, n_species, has_lj, has_coul, has_lj_excl, has_coul_excl, accumulate;
@@ -70,56 +61,8 @@ __global__ void pc_geom_kernel(PairArgs a) {
   make_geom(a.tril + 6 * s, rc, PC_MAX_NC_AXIS, const_cast<SysGeom*>(a.geom)[s], 2);
 }
 
-// Exclusive scan of cnt[0..n) into start[0..n], n <= 4 * blockDim.
-__device__ void block_scan4(const int* cnt, int* start, int n, int* wsum) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x / 64;
-  int v4[4], own = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int c = tid * 4 + k;
-    v4[k] = c < n ? cnt[c] : 0;
-    own += v4[k];
-  }
-  int v = own;
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(v, o, 64);
-    if (lane >= o) v += y;
-  }
-  if (lane == 63) wsum[wv] = v;
-  __syncthreads();
-  if (wv == 0) {
-    int w = lane < nw ? wsum[lane] : 0;
-    for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(w, o, 64);
-      if (lane >= o) w += y;
-    }
-    if (lane < nw) wsum[lane] = w;
-  }
-  __syncthreads();
-  int ex = v - own + (wv > 0 ? wsum[wv - 1] : 0);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int c = tid * 4 + k;
-    if (c < n) start[c] = ex;
-    ex += v4[k];
-    if (c == n - 1) start[n] = ex;
-  }
-}
-
-__device__ inline int cell_of(const SysGeom& g, float x, float y, float z, float& wx, float& wy,
-                              float& wz) {
-  float f0, f1, f2;
-  wx = x; wy = y; wz = z;
-  wrap_shift(g.L, g.Lif, wx, wy, wz, f0, f1, f2);
-  return (cell_index(f2, g.nc[2]) * g.nc[1] + cell_index(f1, g.nc[1])) * g.nc[0] +
-         cell_index(f0, g.nc[0]);
-}
-
 __global__ __launch_bounds__(PC_BIN_THREADS) void pc_bin_kernel(PairArgs a) {
-  __shared__ int cnt[PC_NCMAX];
-  __shared__ int start[PC_NCMAX + 1];
-  __shared__ short keys[PC_NMAX];
-  __shared__ int wsum[PC_BIN_THREADS / 64];
+  __shared__ PcBinLds L;
   __shared__ SysGeom g;
   const int tid = threadIdx.x;
   for (int s = blockIdx.x; s < a.n_sys; s += gridDim.x) {
@@ -127,66 +70,10 @@ __global__ __launch_bounds__(PC_BIN_THREADS) void pc_bin_kernel(PairArgs a) {
     __syncthreads();
     if (tid == 0) g = a.geom[s];
     __syncthreads();
-    const int ncell = g.ncell;
-    int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
-    if (n <= 0 || n > PC_NMAX) {
-      for (int c = tid; c <= ncell; c += PC_BIN_THREADS) cs[c] = 0;
-      continue;
-    }
-    for (int c = tid; c < ncell; c += PC_BIN_THREADS) cnt[c] = 0;
-    __syncthreads();
-    float wx, wy, wz;
-    for (int i = tid; i < n; i += PC_BIN_THREADS) {
-      const float* r = a.pos + 3 * (size_t)(i0 + i);
-      atomicAdd(&cnt[cell_of(g, r[0], r[1], r[2], wx, wy, wz)], 1);
-    }
-    __syncthreads();
-    block_scan4(cnt, start, ncell, wsum);
-    __syncthreads();
-    for (int c = tid; c < ncell; c += PC_BIN_THREADS) cnt[c] = 0;
-    __syncthreads();
-    for (int i = tid; i < n; i += PC_BIN_THREADS) {
-      const float* r = a.pos + 3 * (size_t)(i0 + i);
-      const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
-      keys[start[c] + atomicAdd(&cnt[c], 1)] = (short)i;
-    }
-    __syncthreads();
-    for (int i = tid; i < n; i += PC_BIN_THREADS) {
-      const float* r = a.pos + 3 * (size_t)(i0 + i);
-      const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
-      const int b = start[c], e = start[c + 1];
-      int rank = 0;
-      for (int t = b; t < e; ++t) rank += (int)(unsigned short)keys[t] < i;
-      const int d = i0 + b + rank;
-      a.spos[d] = make_float4(wx, wy, wz, a.q ? a.q[i0 + i] : 0.0f);
-      a.sidx[d] = i;
-      a.scell[d] = c;
-      a.sspec[d] = a.species ? a.species[i0 + i] : 0;
-      if (a.has_lj_excl) a.slj[d] = a.lj_excl[i0 + i];
-      if (a.has_coul_excl) a.scl[d] = a.cl_excl[i0 + i];
-    }
-    for (int c = tid; c <= ncell; c += PC_BIN_THREADS) cs[c] = start[c];
+    PcSorted o{a.spos, a.sidx, a.scell, a.sspec, a.slj, a.scl, a.cstart + (size_t)s * (PC_NCMAX + 1)};
+    pc_bin_system(g, i0, n, a.pos, a.q, a.species, a.has_lj_excl ? a.lj_excl : nullptr,
+                  a.has_coul_excl ? a.cl_excl : nullptr, o, L);
   }
-}
-
-// FP64 re-decision |(f_i - f_j - s) @ L|^2 < rc2 (core/neighborlist.py:542-564).
-__device__ inline bool keep64(const SysGeom& g, float4 ri, float4 rj, int sx, int sy, int sz,
-                              double rc2) {
-  asm volatile("" ::: "memory");
-  double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
-  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
-  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
-  tmul(g.L, fi0 - fj0 - sx, fi1 - fj1 - sy, fi2 - fj2 - sz, d0, d1, d2);
-  return d0 * d0 + d1 * d1 + d2 * d2 < rc2;
-}
-
-// Minimum-image test of a replicated system's pair: s == round(f_i - f_j).
-__device__ inline bool is_min_image(const SysGeom& g, float4 ri, float4 rj, int sx, int sy,
-                                    int sz) {
-  double fi0, fi1, fi2, fj0, fj1, fj2;
-  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
-  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
-  return rint(fi0 - fj0) == sx && rint(fi1 - fj1) == sy && rint(fi2 - fj2) == sz;
 }
 
 template <bool EXCL>
@@ -215,18 +102,10 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
     }
   }
   __syncthreads();
-  const double lj_rc2 = a.has_lj ? a.lj_rc[s] * a.lj_rc[s] : 0.0;
-  const double cl_rc = a.has_coul ? a.cl_rc[s] : 0.0;
-  const double cl_rc2 = cl_rc * cl_rc;
-  // FP64 band around each cutoff re-decided with the reference's fractional
-  // formula (the Cartesian FP64 r^2 differs from it by a few ulp)
-  const double lj_hi = lj_rc2 * (1.0 + PC_BAND64), lj_lo = lj_rc2 * (1.0 - PC_BAND64);
-  const double cl_hi = cl_rc2 * (1.0 + PC_BAND64), cl_lo = cl_rc2 * (1.0 - PC_BAND64);
-  const double all_hi = fmax(lj_hi, cl_hi);
-  const float alpha = a.has_coul ? (float)a.alpha[s] : 0.0f;
-  const float K = 14.399645351950548f;  // HARTREE * BOHR, eV A
-  const float two_a_sqrtpi = 2.0f * alpha * 0.5641895835477563f;
-  const float alpha2 = alpha * alpha;
+  PairParams pp;
+  pc_pair_params(pp, a.has_lj, a.has_coul, a.has_lj_excl, a.has_coul_excl, a.lj_rc, a.cl_rc,
+                 a.alpha, s, ns, s_sig2, s_e4, s_e24);
+  const double all_hi = pp.all_hi;
   const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
   const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
 
@@ -266,36 +145,12 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
             const double r2d = exd * exd + eyd * eyd + ezd * ezd;
             if (!(r2d < all_hi)) continue;
             const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
-            bool min_img = true;
-            if (EXCL && g.rep) min_img = is_min_image(g, ri, rj, sx, sy, sz);
             // particle exclusion: the zero-shift self pair
             const bool self = center && t == slot;
-            float gsum = 0.f;  // d(E_directed)/d(r^2) * 2
-            if (a.has_lj && r2d < lj_hi && !self) {
-              bool ok = !(r2d > lj_lo) || keep64(g, ri, rj, sx, sy, sz, lj_rc2);
-              if (EXCL && a.has_lj_excl && min_img && a.slj[i0 + t] == xlj) ok = false;
-              if (ok) {
-                const int p = si * ns + a.sspec[i0 + t];
-                const float ir2 = 1.0f / r2;
-                float c6 = s_sig2[p] * ir2;
-                c6 = c6 * c6 * c6;
-                elj += s_e4[p] * c6 * (c6 - 1.0f);
-                gsum += s_e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;
-              }
-            }
-            if (a.has_coul && r2d < cl_hi && !self) {
-              bool ok = !(r2d > cl_lo) || keep64(g, ri, rj, sx, sy, sz, cl_rc2);
-              if (EXCL && a.has_coul_excl && min_img && a.scl[i0 + t] == xcl) ok = false;
-              if (ok) {
-                const float r = sqrtf(r2);
-                const float ir = 1.0f / r;
-                const float qq = ri.w * rj.w * K;
-                const float ec = erfcf(alpha * r);
-                ecl += qq * ec * ir;
-                // dU/dr = qq (-2a/sqrt(pi) e^{-a^2 r^2} / r - erfc / r^2); 2 u' = dU/dr / r
-                gsum += qq * (-two_a_sqrtpi * __expf(-alpha2 * r2) - ec * ir) * ir * ir;
-              }
-            }
+            const float gsum = pc_pair_terms<EXCL>(
+                pp, g, ri, rj, sx, sy, sz, r2d, r2, self, si, a.sspec[i0 + t], xlj,
+                EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
+                EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
             ax += gsum * ex; ay += gsum * ey; az += gsum * ez;
             const float hg = -0.5f * gsum;
             w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
@@ -311,37 +166,14 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
       a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
     }
   }
-  // per-block FP64 partials (fixed order: wave sums, then waves in order)
   const double v[PC_NPART] = {0.5 * elj, 0.5 * ecl, w00, w01, w02, w11, w12, w22};
-  const int lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-  for (int k = 0; k < PC_NPART; ++k) {
-    double x = wave_sum(v[k]);
-    if (lane == 0) red[wv][k] = x;
-  }
-  __syncthreads();
-  if (tid < PC_NPART) {
-    double x = 0.0;
-    for (int w = 0; w < PC_TILE / 64; ++w) x += red[w][tid];
-    part[tid] = x;
-  }
+  pc_block_partials<PC_TILE>(v, red, part);
 }
 
 __global__ void pc_reduce_kernel(PairArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n_sys) return;
-  double acc[PC_NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int t = 0; t < a.tiles; ++t)
-    for (int k = 0; k < PC_NPART; ++k) acc[k] += a.partial[((size_t)s * a.tiles + t) * PC_NPART + k];
-  if (a.energy) {
-    double* e = a.energy + 2 * (size_t)s;
-    if (a.accumulate) { e[0] += acc[0]; e[1] += acc[1]; } else { e[0] = acc[0]; e[1] = acc[1]; }
-  }
-  if (a.virial) {
-    const double w[9] = {acc[2], acc[3], acc[4], acc[3], acc[5], acc[6], acc[4], acc[6], acc[7]};
-    double* W = a.virial + 9 * (size_t)s;
-    for (int k = 0; k < 9; ++k) W[k] = a.accumulate ? W[k] + w[k] : w[k];
-  }
+  pc_reduce_system(s, a.tiles, a.partial, a.accumulate, a.energy, a.virial);
 }
 
 }  // namespace kupsmi

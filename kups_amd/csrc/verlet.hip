@@ -1,0 +1,511 @@
+// LJ + real-space Ewald over a Verlet list with a skin, rebuilt per system on
+// the device when an atom has moved more than skin/2 since the last build
+// (the neighbor-list reuse the reference's per-step rebuild lacks,
+// core/neighborlist.py:985-1077 rebuilds every call).  Same energies, forces,
+// virials and pair semantics as kupsmi_pair_forces; the state lives in a
+// kupsmi_verlet handle.
+//
+// Per call, on the caller's stream:
+//   clear   max displacement per system := 0
+//   gather  per tile: sorted slot d <- the caller's atom sidx[d], moved by
+//           the whole lattice vectors it was wrapped by since the build, so
+//           the image shifts stored in the list stay valid; the squared
+//           displacement since the build feeds the system's maximum;
+//   bin     one workgroup per system that needs a rebuild (first call, or
+//           maximum displacement > skin/2): geometry for rc + skin and the
+//           counting sort of the cell-list path (cells >= (rc + skin)/2);
+//   list    one lane per atom of those systems appends (j, shift, species) of
+//           each 125-cell-stencil candidate within rc + skin (FP64 test) to
+//           its list, stored entry-major (list[k * n_atoms + d]) so a wave's
+//           loads of entry k are contiguous;
+//   pair    one lane per sorted atom walks its list with the next entry and
+//           position in flight: FP64 pair vector, the pair terms of the
+//           cell-list path; an atom whose list overflowed the capacity walks
+//           the 125-cell stencil of the build instead;
+//   reduce  fixed-order FP64 sums per system.
+// Positions, cells (tril), charges, species, exclusion ids and sizes must stay
+// fixed between rebuilds except for the positions; kupsmi_verlet_invalidate
+// forces a rebuild of every system.
+#include "pair_common.hpp"
+
+struct kupsmi_verlet {
+  int device = -1;
+  int n_sys = 0, n_atoms = 0, cap = 0, tiles = 0;
+  void* mem = nullptr;
+  size_t bytes = 0;
+};
+
+namespace kupsmi {
+
+constexpr int VL_TILE = 256;
+constexpr int VL_SHIFT_MAX = 7;  // |image shift| per axis in a packed entry
+
+struct VlArgs {
+  int n_sys, n_atoms, n_species, has_lj, has_coul, has_lj_excl, has_coul_excl, accumulate;
+  int tiles, cap;
+  double skin;
+  const int32_t* sys_off;
+  const float* pos;
+  const float* q;
+  const int32_t* species;
+  const int32_t* lj_excl;
+  const int32_t* cl_excl;
+  const double* sigma;
+  const double* epsilon;
+  const double* lj_rc;
+  const double* alpha;
+  const double* cl_rc;
+  const double* tril;
+  // persistent state
+  SysGeom* geom;        // geometry of the last build (rc + skin)
+  float4* spos;         // current positions in the build frame, sorted; q in w
+  float4* sref;         // positions at the last build, sorted
+  int* sidx;
+  int* scell;
+  int* sspec;
+  int* slj;
+  int* scl;
+  int* cstart;          // n_sys x (PC_NCMAX + 1)
+  int* nlist;           // entries per sorted atom (> cap: overflow)
+  uint32_t* list;       // cap x n_atoms
+  int* valid;           // n_sys
+  int* need;            // n_sys: rebuild in this call
+  unsigned* maxd2;      // n_sys: float bits of the max squared displacement
+  unsigned long long* counters;  // [0] system rebuilds, [1] overflowed atom lists
+  double* partial;
+  float* forces;
+  double* energy;
+  double* virial;
+};
+
+// List entry: partner slot (15 bits), image shift + 8 per axis (4 bits each),
+// partner species (3 bits).
+__device__ inline uint32_t vl_pack(int t, int sx, int sy, int sz, int sp) {
+  return (uint32_t)t | ((uint32_t)(sx + 8) << 15) | ((uint32_t)(sy + 8) << 19) |
+         ((uint32_t)(sz + 8) << 23) | ((uint32_t)sp << 27);
+}
+
+__global__ void vl_clear_kernel(int n_sys, unsigned* maxd2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n_sys) maxd2[s] = 0u;
+}
+
+__global__ void vl_invalidate_kernel(int n_sys, int* valid) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n_sys) valid[s] = 0;
+}
+
+__global__ __launch_bounds__(VL_TILE) void vl_gather_kernel(VlArgs a) {
+  __shared__ float red[VL_TILE / 64];
+  const int s = blockIdx.y;
+  if (!a.valid[s]) return;
+  const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
+  if ((int)blockIdx.x * VL_TILE >= n) return;
+  const int slot = blockIdx.x * VL_TILE + threadIdx.x;
+  const SysGeom& g = a.geom[s];
+  float d2 = 0.f;
+  if (slot < n) {
+    const int d = i0 + slot;
+    const float* r = a.pos + 3 * (size_t)(i0 + a.sidx[d]);
+    const float4 rb = a.sref[d];
+    // whole lattice vectors the atom was wrapped by since the build (its true
+    // displacement is below skin/2, far below half a cell)
+    double f0, f1, f2;
+    tmul(g.Li, (double)r[0] - rb.x, (double)r[1] - rb.y, (double)r[2] - rb.z, f0, f1, f2);
+    double s0, s1, s2;
+    tmul(g.L, rint(f0), rint(f1), rint(f2), s0, s1, s2);
+    const double ux = (double)r[0] - s0, uy = (double)r[1] - s1, uz = (double)r[2] - s2;
+    const double dx = ux - rb.x, dy = uy - rb.y, dz = uz - rb.z;
+    d2 = (float)(dx * dx + dy * dy + dz * dz);
+    a.spos[d] = make_float4((float)ux, (float)uy, (float)uz, rb.w);
+  }
+  for (int o = 32; o > 0; o >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    for (int w = 1; w < VL_TILE / 64; ++w) m = fmaxf(m, red[w]);
+    atomicMax(a.maxd2 + s, __float_as_uint(m));  // non-negative floats order as their bits
+  }
+}
+
+struct VlBinLds {
+  PcBinLds bin;
+  SysGeom g;
+  int need;
+};
+
+// Systems that need a rebuild (first call, or max displacement > skin/2):
+// geometry for rc + skin and the counting sort, one workgroup per system.
+__global__ __launch_bounds__(PC_BIN_THREADS) void vl_bin_kernel(VlArgs a) {
+  __shared__ VlBinLds L;
+  const int tid = threadIdx.x;
+  const float half2 = (float)(0.25 * a.skin * a.skin * (1.0 - 1e-6));
+  for (int s = blockIdx.x; s < a.n_sys; s += gridDim.x) {
+    const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
+    __syncthreads();
+    if (tid == 0) {
+      L.need = !a.valid[s] || __uint_as_float(a.maxd2[s]) > half2;
+      a.need[s] = L.need;
+      if (L.need) {
+        double rc = 0.0;
+        if (a.has_lj) rc = a.lj_rc[s];
+        if (a.has_coul && a.cl_rc[s] > rc) rc = a.cl_rc[s];
+        make_geom(a.tril + 6 * s, rc + a.skin, PC_MAX_NC_AXIS, L.g, 2);
+        a.geom[s] = L.g;
+        a.valid[s] = 1;
+        atomicAdd(a.counters, 1ull);
+      }
+    }
+    __syncthreads();
+    if (!L.need) continue;
+    PcSorted o{a.spos, a.sidx, a.scell, a.sspec, a.slj, a.scl,
+               a.cstart + (size_t)s * (PC_NCMAX + 1)};
+    pc_bin_system(L.g, i0, n, a.pos, a.q, a.species, a.has_lj_excl ? a.lj_excl : nullptr,
+                  a.has_coul_excl ? a.cl_excl : nullptr, o, L.bin);
+  }
+}
+
+// Lists of the rebuilt systems: one lane per sorted atom appends every
+// candidate of the 125-cell stencil within rc + skin (FP64 test).
+__global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
+  __shared__ SysGeom g;
+  const int s = blockIdx.y;
+  if (!a.need[s]) return;
+  const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
+  if ((int)blockIdx.x * VL_TILE >= n || n > PC_NMAX) return;
+  if (threadIdx.x == 0) g = a.geom[s];
+  __syncthreads();
+  const int slot = blockIdx.x * VL_TILE + threadIdx.x;
+  if (slot >= n) return;
+  const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
+  const double rl = g.rc;  // rc + skin
+  const double rl2 = rl * rl * (1.0 + 1e-9);
+  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+  const int d = i0 + slot;
+  const float4 ri = a.spos[d];
+  a.sref[d] = ri;
+  const int c = a.scell[d];
+  const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+  int cnt = 0;
+  for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
+    int zz = cz + dz;
+    const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
+    zz -= sz * ncz;
+    for (int dy = -g.h[1]; dy <= g.h[1]; ++dy) {
+      int yy = cy + dy;
+      const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
+      yy -= sy * ncy;
+      for (int dx = -g.h[0]; dx <= g.h[0]; ++dx) {
+        int xx = cx + dx;
+        const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
+        xx -= sx * ncx;
+        double Sx, Sy, Sz;
+        tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
+        const double qx = (double)ri.x - Sx, qy = (double)ri.y - Sy, qz = (double)ri.z - Sz;
+        const bool center = dx == 0 && dy == 0 && dz == 0;
+        const bool packable = sx >= -VL_SHIFT_MAX && sx <= VL_SHIFT_MAX && sy >= -VL_SHIFT_MAX &&
+                              sy <= VL_SHIFT_MAX && sz >= -VL_SHIFT_MAX && sz <= VL_SHIFT_MAX;
+        const int nb = (zz * ncy + yy) * ncx + xx;
+        const int tb = cs[nb], te = cs[nb + 1];
+        for (int t = tb; t < te; ++t) {
+          if (center && t == slot) continue;
+          const float4 rj = a.spos[i0 + t];
+          const double ex = (double)rj.x - qx, ey = (double)rj.y - qy, ez = (double)rj.z - qz;
+          if (!(ex * ex + ey * ey + ez * ez < rl2)) continue;
+          if (!packable) cnt = a.cap + 1;  // cannot be stored: use the stencil
+          if (cnt < a.cap)
+            a.list[(size_t)cnt * a.n_atoms + d] = vl_pack(t, sx, sy, sz, a.sspec[i0 + t]);
+          ++cnt;
+        }
+      }
+    }
+  }
+  a.nlist[d] = cnt;
+  if (cnt > a.cap) atomicAdd(a.counters + 1, 1ull);
+}
+
+template <bool EXCL>
+__global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
+  __shared__ SysGeom g;
+  __shared__ float s_sig2[PC_NSPEC * PC_NSPEC], s_e4[PC_NSPEC * PC_NSPEC],
+      s_e24[PC_NSPEC * PC_NSPEC];
+  __shared__ double red[VL_TILE / 64][PC_NPART];
+  const int s = blockIdx.y;
+  const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
+  const int start = blockIdx.x * VL_TILE;
+  const int tid = threadIdx.x;
+  double* part = a.partial + ((size_t)s * a.tiles + blockIdx.x) * PC_NPART;
+  if (start >= n || n > PC_NMAX) {
+    if (tid < PC_NPART) part[tid] = 0.0;
+    return;
+  }
+  if (tid == 0) g = a.geom[s];
+  const int ns = a.n_species;
+  for (int k = tid; k < ns * ns; k += VL_TILE) {
+    if (a.has_lj) {
+      const double sg = a.sigma[k];
+      s_sig2[k] = (float)(sg * sg);
+      s_e4[k] = (float)(4.0 * a.epsilon[k]);
+      s_e24[k] = (float)(24.0 * a.epsilon[k]);
+    }
+  }
+  __syncthreads();
+  PairParams pp;
+  pc_pair_params(pp, a.has_lj, a.has_coul, a.has_lj_excl, a.has_coul_excl, a.lj_rc, a.cl_rc,
+                 a.alpha, s, ns, s_sig2, s_e4, s_e24);
+  const int slot = start + tid;
+  float ax = 0.f, ay = 0.f, az = 0.f, elj = 0.f, ecl = 0.f;
+  float w00 = 0.f, w01 = 0.f, w02 = 0.f, w11 = 0.f, w12 = 0.f, w22 = 0.f;
+  if (slot < n) {
+    const int d = i0 + slot;
+    const float4 ri = a.spos[d];
+    const int si = a.sspec[d];
+    const int xlj = EXCL && a.has_lj_excl ? a.slj[d] : 0;
+    const int xcl = EXCL && a.has_coul_excl ? a.scl[d] : 0;
+    auto visit = [&](int t, float4 rj, int sj, int sx, int sy, int sz, bool self) {
+      // FP64 pair vector d = r_j - r_i + s @ L
+      double Sx, Sy, Sz;
+      tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
+      const double exd = (double)rj.x - ((double)ri.x - Sx);
+      const double eyd = (double)rj.y - ((double)ri.y - Sy);
+      const double ezd = (double)rj.z - ((double)ri.z - Sz);
+      const double r2d = exd * exd + eyd * eyd + ezd * ezd;
+      if (!(r2d < pp.all_hi)) return;
+      const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
+      const float gsum = pc_pair_terms<EXCL>(
+          pp, g, ri, rj, sx, sy, sz, r2d, r2, self, si, sj, xlj,
+          EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
+          EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
+      ax += gsum * ex; ay += gsum * ey; az += gsum * ez;
+      const float hg = -0.5f * gsum;
+      w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
+      w11 += hg * ey * ey; w12 += hg * ey * ez; w22 += hg * ez * ez;
+    };
+    const int cnt = a.nlist[d];
+    if (cnt <= a.cap) {
+      // software pipeline: entry k + 2 and position k + 1 are in flight while
+      // entry k is evaluated
+      const uint32_t* lp = a.list + d;
+      const size_t stride = (size_t)a.n_atoms;
+      uint32_t e1 = cnt > 0 ? lp[0] : 0u, e2 = cnt > 1 ? lp[stride] : 0u;
+      float4 r1 = a.spos[i0 + (int)(e1 & 0x7fffu)];
+      for (int k = 0; k < cnt; ++k) {
+        const uint32_t e = e1;
+        const float4 rj = r1;
+        e1 = e2;
+        r1 = a.spos[i0 + (int)(e1 & 0x7fffu)];
+        e2 = k + 2 < cnt ? lp[(size_t)(k + 2) * stride] : 0u;
+        visit((int)(e & 0x7fffu), rj, (int)(e >> 27), (int)((e >> 15) & 15u) - 8,
+              (int)((e >> 19) & 15u) - 8, (int)((e >> 23) & 15u) - 8, false);
+      }
+    } else {
+      // overflowed list: the build's 125-cell stencil (cells sized for rc + skin)
+      const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
+      const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+      const int c = a.scell[d];
+      const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+      for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
+        int zz = cz + dz;
+        const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
+        zz -= sz * ncz;
+        for (int dy = -g.h[1]; dy <= g.h[1]; ++dy) {
+          int yy = cy + dy;
+          const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
+          yy -= sy * ncy;
+          for (int dx = -g.h[0]; dx <= g.h[0]; ++dx) {
+            int xx = cx + dx;
+            const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
+            xx -= sx * ncx;
+            const bool center = dx == 0 && dy == 0 && dz == 0;
+            const int nb = (zz * ncy + yy) * ncx + xx;
+            for (int t = cs[nb]; t < cs[nb + 1]; ++t)
+              visit(t, a.spos[i0 + t], a.sspec[i0 + t], sx, sy, sz, center && t == slot);
+          }
+        }
+      }
+    }
+    const int gi = i0 + a.sidx[d];
+    if (a.accumulate) {
+      a.forces[3 * gi] += ax; a.forces[3 * gi + 1] += ay; a.forces[3 * gi + 2] += az;
+    } else {
+      a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
+    }
+  }
+  const double v[PC_NPART] = {0.5 * elj, 0.5 * ecl, w00, w01, w02, w11, w12, w22};
+  pc_block_partials<VL_TILE>(v, red, part);
+}
+
+__global__ void vl_reduce_kernel(VlArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n_sys) return;
+  pc_reduce_system(s, a.tiles, a.partial, a.accumulate, a.energy, a.virial);
+}
+
+// Byte offsets of the handle's arrays.
+struct VlLayout {
+  size_t geom, spos, sref, sidx, scell, sspec, slj, scl, cstart, nlist, list, valid, need, maxd2,
+      counters, partial, total;
+  VlLayout(int n_sys, int n_atoms, int cap, int tiles) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    const size_t na = (size_t)n_atoms, ns = (size_t)n_sys;
+    geom = take(sizeof(SysGeom) * ns);
+    spos = take(sizeof(float4) * na);
+    sref = take(sizeof(float4) * na);
+    sidx = take(sizeof(int) * na);
+    scell = take(sizeof(int) * na);
+    sspec = take(sizeof(int) * na);
+    slj = take(sizeof(int) * na);
+    scl = take(sizeof(int) * na);
+    cstart = take(sizeof(int) * ns * (PC_NCMAX + 1));
+    nlist = take(sizeof(int) * na);
+    list = take(sizeof(uint32_t) * na * (size_t)cap);
+    valid = take(sizeof(int) * ns);
+    need = take(sizeof(int) * ns);
+    maxd2 = take(sizeof(unsigned) * ns);
+    counters = take(sizeof(unsigned long long) * 2);
+    partial = take(sizeof(double) * ns * (size_t)tiles * PC_NPART);
+    total = off;
+  }
+};
+
+__global__ void vl_reset_counters_kernel(unsigned long long* c) { c[0] = 0; c[1] = 0; }
+
+}  // namespace kupsmi
+
+using namespace kupsmi;
+
+extern "C" int kupsmi_verlet_create(kupsmi_verlet** out) {
+  KUPSMI_REQUIRE(out != nullptr, "null output");
+  *out = new kupsmi_verlet();
+  KUPSMI_HIP(hipGetDevice(&(*out)->device));
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_verlet_destroy(kupsmi_verlet* v) {
+  if (!v) return KUPSMI_OK;
+  if (v->mem) {
+    int cur = 0;
+    KUPSMI_HIP(hipGetDevice(&cur));
+    KUPSMI_HIP(hipSetDevice(v->device));
+    const hipError_t e = hipFree(v->mem);
+    KUPSMI_HIP(hipSetDevice(cur));
+    KUPSMI_HIP(e);
+  }
+  delete v;
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_verlet_invalidate(kupsmi_verlet* v, void* stream) {
+  KUPSMI_REQUIRE(v != nullptr, "null handle");
+  if (!v->mem || v->n_sys == 0) return KUPSMI_OK;
+  VlLayout lay(v->n_sys, v->n_atoms, v->cap, v->tiles);
+  int* valid = reinterpret_cast<int*>(static_cast<char*>(v->mem) + lay.valid);
+  vl_invalidate_kernel<<<(v->n_sys + 255) / 256, 256, 0, (hipStream_t)stream>>>(v->n_sys, valid);
+  KUPSMI_CHECK_LAUNCH();
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_verlet_counters(kupsmi_verlet* v, void* stream, unsigned long long* out) {
+  KUPSMI_REQUIRE(v != nullptr && out != nullptr, "null argument");
+  out[0] = out[1] = 0;
+  if (!v->mem) return KUPSMI_OK;
+  VlLayout lay(v->n_sys, v->n_atoms, v->cap, v->tiles);
+  KUPSMI_HIP(hipMemcpyAsync(out, static_cast<char*>(v->mem) + lay.counters,
+                            2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            (hipStream_t)stream));
+  KUPSMI_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_pair_forces_verlet(
+    kupsmi_verlet* v, void* stream, int n_sys, const int32_t* sys_off, int n_atoms,
+    int max_atoms_per_sys, const float* pos, const float* charges, const int32_t* species,
+    int n_species, const double* sigma, const double* epsilon, const double* lj_cutoff,
+    const int32_t* lj_exclusion, const double* alpha, const double* coul_cutoff,
+    const int32_t* coul_exclusion, const double* tril, double skin, int max_neighbors,
+    int accumulate, float* forces, double* energy, double* virial) {
+  KUPSMI_REQUIRE(v != nullptr, "null handle");
+  KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0, "negative size");
+  KUPSMI_REQUIRE(max_atoms_per_sys <= PC_NMAX, "systems of the pair path hold at most 16384 atoms");
+  KUPSMI_REQUIRE(n_species >= 1 && n_species <= PC_NSPEC, "n_species must be in [1, 8]");
+  KUPSMI_REQUIRE(skin >= 0.0, "skin must be non-negative");
+  KUPSMI_REQUIRE(max_neighbors >= 1, "max_neighbors must be positive");
+  const bool has_lj = sigma && epsilon && lj_cutoff;
+  const bool has_cl = alpha && coul_cutoff && charges;
+  KUPSMI_REQUIRE(has_lj || has_cl, "nothing to compute: pass LJ tables or Ewald parameters");
+  KUPSMI_REQUIRE(forces != nullptr, "forces output required");
+  if (n_sys == 0 || n_atoms == 0) return KUPSMI_OK;
+  int dev = 0;
+  KUPSMI_HIP(hipGetDevice(&dev));
+  KUPSMI_REQUIRE(dev == v->device, "the handle belongs to another device");
+  hipStream_t st = (hipStream_t)stream;
+  int tiles = (max_atoms_per_sys + VL_TILE - 1) / VL_TILE;
+  if (tiles < 1) tiles = 1;
+  if (!v->mem || v->n_sys != n_sys || v->n_atoms != n_atoms || v->cap != max_neighbors ||
+      v->tiles < tiles) {
+    if (v->mem) {
+      KUPSMI_HIP(hipStreamSynchronize(st));
+      KUPSMI_HIP(hipFree(v->mem));
+      v->mem = nullptr;
+    }
+    VlLayout lay(n_sys, n_atoms, max_neighbors, tiles);
+    void* p = nullptr;
+    KUPSMI_HIP(hipMalloc(&p, lay.total));
+    v->mem = p; v->bytes = lay.total;
+    v->n_sys = n_sys; v->n_atoms = n_atoms; v->cap = max_neighbors; v->tiles = tiles;
+    char* b = static_cast<char*>(p);
+    vl_invalidate_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(n_sys, reinterpret_cast<int*>(b + lay.valid));
+    KUPSMI_CHECK_LAUNCH();
+    vl_reset_counters_kernel<<<1, 1, 0, st>>>(reinterpret_cast<unsigned long long*>(b + lay.counters));
+    KUPSMI_CHECK_LAUNCH();
+  }
+  tiles = v->tiles;
+  VlLayout lay(n_sys, n_atoms, v->cap, tiles);
+  char* b = static_cast<char*>(v->mem);
+  VlArgs a{};
+  a.n_sys = n_sys; a.n_atoms = n_atoms; a.n_species = n_species; a.has_lj = has_lj;
+  a.has_coul = has_cl; a.has_lj_excl = lj_exclusion != nullptr;
+  a.has_coul_excl = coul_exclusion != nullptr; a.accumulate = accumulate;
+  a.tiles = tiles; a.cap = v->cap; a.skin = skin;
+  a.sys_off = sys_off; a.pos = pos; a.q = charges; a.species = species;
+  a.lj_excl = lj_exclusion; a.cl_excl = coul_exclusion; a.sigma = sigma; a.epsilon = epsilon;
+  a.lj_rc = lj_cutoff; a.alpha = alpha; a.cl_rc = coul_cutoff; a.tril = tril;
+  a.geom = reinterpret_cast<SysGeom*>(b + lay.geom);
+  a.spos = reinterpret_cast<float4*>(b + lay.spos);
+  a.sref = reinterpret_cast<float4*>(b + lay.sref);
+  a.sidx = reinterpret_cast<int*>(b + lay.sidx);
+  a.scell = reinterpret_cast<int*>(b + lay.scell);
+  a.sspec = reinterpret_cast<int*>(b + lay.sspec);
+  a.slj = reinterpret_cast<int*>(b + lay.slj);
+  a.scl = reinterpret_cast<int*>(b + lay.scl);
+  a.cstart = reinterpret_cast<int*>(b + lay.cstart);
+  a.nlist = reinterpret_cast<int*>(b + lay.nlist);
+  a.list = reinterpret_cast<uint32_t*>(b + lay.list);
+  a.valid = reinterpret_cast<int*>(b + lay.valid);
+  a.need = reinterpret_cast<int*>(b + lay.need);
+  a.maxd2 = reinterpret_cast<unsigned*>(b + lay.maxd2);
+  a.counters = reinterpret_cast<unsigned long long*>(b + lay.counters);
+  a.partial = reinterpret_cast<double*>(b + lay.partial);
+  a.forces = forces; a.energy = energy; a.virial = virial;
+  int ncu = 256;
+  KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  vl_clear_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(n_sys, a.maxd2);
+  KUPSMI_CHECK_LAUNCH();
+  dim3 grid(tiles, n_sys);
+  vl_gather_kernel<<<grid, VL_TILE, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  vl_bin_kernel<<<n_sys < ncu ? n_sys : ncu, PC_BIN_THREADS, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  vl_list_kernel<<<grid, VL_TILE, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  if (a.has_lj_excl || a.has_coul_excl)
+    vl_pair_kernel<true><<<grid, VL_TILE, 0, st>>>(a);
+  else
+    vl_pair_kernel<false><<<grid, VL_TILE, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  vl_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
+  return KUPSMI_OK;
+}

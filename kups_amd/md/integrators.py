@@ -13,6 +13,7 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from .. import _lib
@@ -117,11 +118,17 @@ class EwaldLJPropagator:
     integrate(phase 1).  Same step order as the fused LJ kernel."""
 
     def __init__(self, params: LennardJonesParameters, ewald, integrator: int = BAOAB_LANGEVIN,
-                 seed: int = 0):
+                 seed: int = 0, skin: float | None = None, max_neighbors: int | None = None):
+        """skin: reuse a Verlet list of radius cutoff + skin across steps
+        (rebuilt on the GPU per system when an atom has moved more than
+        skin / 2); None evaluates with a fresh cell list every step."""
         self.params = params
         self.ewald = ewald
         self.integrator = integrator
         self.seed = seed
+        self.skin = skin
+        self.max_neighbors = max_neighbors
+        self.verlet = None
         self._dev = None
 
     def _prep(self, state: MDState):
@@ -134,15 +141,32 @@ class EwaldLJPropagator:
             self._kmax = torch.as_tensor(self.ewald.kmax, dtype=torch.float64, device=dev).contiguous()
             self._dev = dev
         self._maxn = int(state.sizes.max()) if state.sizes.size else 0
+        if self.skin is not None and self.verlet is None:
+            from ..core.verlet import VerletList, neighbor_capacity
+
+            rc = max(float(np.max(self.ewald.cutoff)), float(state.cutoff.max().item()))
+            cap = self.max_neighbors or neighbor_capacity(state.sizes, state.tril.cpu().numpy(), rc,
+                                                          self.skin)
+            self.verlet = VerletList(self.skin, cap)
 
     def forces(self, state: MDState, e2, elr, vir=None):
         lib = _lib.load()
-        _lib.check(lib.kupsmi_pair_forces(
-            _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
-            _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.species),
-            self.params.n_species, _lib.ptr(self._sig), _lib.ptr(self._eps), _lib.ptr(state.cutoff),
-            None, _lib.ptr(self._alpha), _lib.ptr(self._rc), None, _lib.ptr(state.tril), 0,
-            _lib.ptr(state.forces), _lib.ptr(e2), _lib.ptr(vir)))
+        if self.verlet is not None:
+            _lib.check(lib.kupsmi_pair_forces_verlet(
+                self.verlet.handle, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n,
+                self._maxn, _lib.ptr(state.positions), _lib.ptr(state.charges),
+                _lib.ptr(state.species), self.params.n_species, _lib.ptr(self._sig),
+                _lib.ptr(self._eps), _lib.ptr(state.cutoff), None, _lib.ptr(self._alpha),
+                _lib.ptr(self._rc), None, _lib.ptr(state.tril), self.verlet.skin,
+                self.verlet.max_neighbors, 0, _lib.ptr(state.forces), _lib.ptr(e2), _lib.ptr(vir)))
+        else:
+            _lib.check(lib.kupsmi_pair_forces(
+                _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n,
+                self._maxn, _lib.ptr(state.positions), _lib.ptr(state.charges),
+                _lib.ptr(state.species), self.params.n_species, _lib.ptr(self._sig),
+                _lib.ptr(self._eps), _lib.ptr(state.cutoff), None, _lib.ptr(self._alpha),
+                _lib.ptr(self._rc), None, _lib.ptr(state.tril), 0, _lib.ptr(state.forces),
+                _lib.ptr(e2), _lib.ptr(vir)))
         mx, my, mz = self.ewald.grid
         _lib.check(lib.kupsmi_ewald_recip(
             _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
