@@ -9,6 +9,14 @@ positions/forces with FP64 per-system energies, neighbor set rebuilt every
 step.  A "step" is one MD step of every system; all timed steps run in one
 persistent launch (``--steps-per-launch`` splits them).
 
+Charged-melt leg (``lj_ewald_md`` object; ``--no-ewald`` skips it): 512
+NaCl-like melts of N = 8 m^3 ions (m ~ U{5..10}, seed 0, ~2.1 M ions),
+rock-salt a = 5.64 A with 0.1 A jitter, charges +-1, LJ sigma 2.5 / 4.4 A
+(eps 0.01 eV), real-space cutoff 12 A, Ewald alpha = 0.3 / A, k grid
+|n| <= 8 per axis (2600 nonzero k), BAOAB Langevin at 1200 K (gamma 1/ps,
+dt 1 fs); real space over a Verlet list with a 1 A skin, k-space structure
+factors on the matrix cores.
+
 GCMC leg (``gcmc`` object of the line; ``--no-gcmc`` skips it): the
 learned-proposal workload, 4096 rigid hosts of 1500-3000 atoms (jittered cubic
 lattice, 30 % vacancies, 25-30 A triclinic cells, charges N(0, 0.3)
@@ -57,6 +65,10 @@ def parse():
     ap.add_argument("--cpu-sample-every", type=int, default=8,
                     help="CPU baseline runs every k-th system of the batch")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-ewald", action="store_true")
+    ap.add_argument("--ewald-steps", type=int, default=30)
+    ap.add_argument("--ewald-warmup", type=int, default=5)
+    ap.add_argument("--ewald-cpu-systems", type=int, default=16)
     ap.add_argument("--no-gcmc", action="store_true")
     ap.add_argument("--gcmc-chains", type=int, default=4096)
     ap.add_argument("--gcmc-samples", type=int, default=1_000_000)
@@ -105,6 +117,92 @@ def cpu_baseline(batch, params, args):
                    f"{atoms} atoms x {args.cpu_steps} FP32 VV steps, cell list rebuilt every step, "
                    f"OpenMP over systems, {el:.1f} s; CPU restatement of kUPS, not kUPS"),
     }
+
+
+# LJ + Ewald ion-step of the charged melts: 45 FLOP per unique real-space pair
+# (161 per ion within 12 A) + 36 FLOP per k vector (S(k) 16 + force 20, 2600 k)
+FLOP_PER_ION_STEP = 45.0 * 161 + 36.0 * 2600
+NACL = {"Na": (2.5, 0.01), "Cl": (4.4, 0.01)}
+
+
+def ewald_cpu_baseline(b, args):
+    """One force evaluation (cell-list LJ, the dense O(N^2) neighbor list the
+    reference's applications use, real-space and k-space Ewald, self term) of
+    the oracle on the first systems of the batch, FP64, OpenMP over systems;
+    the integrator is negligible beside it."""
+    import oracle
+    from kups_amd.potential.lennard_jones import LennardJonesParameters
+
+    k = args.ewald_cpu_systems
+    n = int(b.sys_off[k])
+    off = b.sys_off[:k + 1]
+    tril = b.tril[:k]
+    pos = b.positions[:n].astype(np.float32).astype(np.float64)
+    sp, q = b.species[:n], b.charges[:n]
+    rc, al = np.full(k, 12.0), np.full(k, 0.3)
+    lj = LennardJonesParameters.from_dict(12.0, NACL)
+    kv = np.stack([oracle.kvecs(t, 8) for t in tril])
+    t0 = time.perf_counter()
+    oracle.lj_cells(pos, sp, off, tril, rc, lj.sigma, lj.epsilon)
+    ij, sh = oracle.neighborlist(pos, np.arange(n), off, tril, rc)
+    oracle.ewald_real_edges(pos, q, off, tril, al, rc, ij, sh)
+    oracle.ewald_recip(pos, q, off, tril, al, kv)
+    oracle.ewald_self(q, off, al)
+    el = time.perf_counter() - t0
+    return {
+        "value": n / el, "unit": "ion-steps/s", "cores": oracle.num_threads(), "kind": "port",
+        "sample": f"first {k} of {b.n_sys} melts ({n} ions), one FP64 force evaluation with a "
+                  f"dense neighbor list, {el:.1f} s; CPU restatement of kUPS, not kUPS",
+    }
+
+
+def ewald_leg(args):
+    import torch
+
+    from kups_amd import workloads
+    from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
+    from kups_amd.md.integrators import BAOAB_LANGEVIN, EwaldLJPropagator
+    from kups_amd.potential.ewald import EwaldParameters
+    from kups_amd.potential.lennard_jones import LennardJonesParameters
+
+    b = workloads.nacl_melts(n_sys=512, seed=0)
+    lj = LennardJonesParameters.from_dict(12.0, NACL)
+    p0 = maxwell_boltzmann_momenta(b.masses, b.sizes, 1200.0, np.random.default_rng(0))
+    st = make_md_state(b.positions.astype(np.float32), b.masses, b.species, b.sizes, b.tril, 12.0,
+                       time_step_fs=1.0, temperature=1200.0, momenta=p0.astype(np.float32),
+                       charges=b.charges, friction_per_fs=0.001)
+    ep = EwaldParameters.box(b.n_sys, alpha=0.3, cutoff=12.0, nmax=8)
+    prop = EwaldLJPropagator(lj, ep, BAOAB_LANGEVIN, skin=1.0)
+    prop.evaluate_forces(st)
+    if args.ewald_warmup:
+        prop(st, args.ewald_warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    obs = prop(st, args.ewald_steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ions = int(b.sizes.sum())
+    rebuilds, overflow = prop.verlet.counters()
+    value = ions * args.ewald_steps / wall
+    achieved = FLOP_PER_ION_STEP * value / 1e12
+    res = {
+        "metric": "ion-steps/s (batched LJ+Ewald MD)", "value": value, "unit": "ion-steps/s",
+        "ms_per_step": wall * 1e3 / args.ewald_steps, "steps": args.ewald_steps,
+        "warmup": args.ewald_warmup, "dtype": "fp32 pairs and k sums (fp64 accumulation)",
+        "workload": f"LJ+Ewald melts: {b.description}, rc 12 A, alpha 0.3, |n| <= 8, BAOAB 1200 K, "
+                    "dt 1 fs, Verlet skin 1 A",
+        "n_ions": ions,
+        "verlet": {"system_rebuilds_total": rebuilds, "overflowed_lists": overflow},
+        "final_potential_energy_eV": float(obs.potential_energy[:, -1].sum().item()),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "note": f"{FLOP_PER_ION_STEP:.0f} algorithmic FLOP per ion-step (whole step)"},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = ewald_cpu_baseline(b, args)
+    del prop, st
+    torch.cuda.empty_cache()
+    return res
 
 
 # MLP 32 -> 128 -> 128 -> 128 -> 1: multiply-adds per descriptor row
@@ -341,9 +439,11 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(batch, params, args)
+    del st, prop
+    torch.cuda.empty_cache()
+    if not args.no_ewald:
+        line["lj_ewald_md"] = ewald_leg(args)
     if not args.no_gcmc:
-        del st, prop
-        torch.cuda.empty_cache()
         line["gcmc"] = gcmc_leg(args)
     print(json.dumps(line), flush=True)
 

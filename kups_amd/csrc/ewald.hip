@@ -7,10 +7,11 @@
 // on the wrapped fractional coordinates, so the per-atom work is a product of
 // three small tables instead of a sincos per (atom, k).
 //
-//   ek_sk_kernel      S(k) = sum_i q_i exp(i k.r_i): one workgroup per
-//                     (system, 256 k vectors), atoms streamed through LDS in
-//                     tiles of 64 (their phase tables built once per tile),
-//                     FP32 tile sums accumulated in FP64.
+//   ek_sk_kernel      S(k) = sum_i q_i exp(i k.r_i) as an FP32 GEMM on the
+//                     matrix cores (v_mfma_f32_16x16x4_f32), one workgroup
+//                     per (system, 512-atom chunk), atoms streamed through LDS
+//                     in tiles of 64 with their per-axis phase tables;
+//                     ek_sk_reduce_kernel sums the chunks in fixed order.
 //   ek_energy_kernel  P(k) (FP64), E = K sum_k P |S|^2, the reciprocal-space
 //                     virial, and S'(k) = P(k) S(k) for the force pass.
 //   ek_force_kernel   F_i = -2 K q_i sum_k k Im(S'(k) exp(-i k.r_i)): one thread
@@ -20,7 +21,6 @@
 namespace kupsmi {
 
 constexpr int EK_THREADS = 256;
-constexpr int EK_TA = 64;          // atoms per LDS tile of the S(k) pass
 constexpr int EK_NM = 12;          // max |n| per axis
 constexpr int EK_W = 2 * EK_NM + 1;
 constexpr int EK_NKMAX = (EK_NM + 1) * EK_W * EK_W;
@@ -33,12 +33,16 @@ struct EkArgs {
   const double* tril;
   const double* alpha;
   const double* kmax;
+  int chunks;       // atom chunks per system of the S(k) pass
   double* sk;       // n_sys x nk x 2 (FP64)
+  double* skpart;   // n_sys x chunks x nk x 2: chunk partials of S
   float2* skp;      // n_sys x nk, P(k) S(k)
   float* forces;
   double* energy;
   double* virial;
 };
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -57,6 +61,30 @@ __device__ inline void unit_phases(const double* t, const float* r, float2& ux, 
   sincospif(2.0f * f2, &uz.y, &uz.x);
 }
 
+__device__ inline double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// c u^n for n = 0..m into out[0..m] (FP64 recurrence, rounded once).
+__device__ inline void powers64_pos(double2 u, double c, int m, float2* out) {
+  double2 p = make_double2(c, 0.0);
+  for (int n = 0; n <= m; ++n) {
+    out[n] = make_float2((float)p.x, (float)p.y);
+    p = zmul(p, u);
+  }
+}
+
+// u^n for n = -m..m into out[0..2m] (n = 0 at out[m]; FP64, rounded once).
+__device__ inline void powers64(double2 u, int m, float2* out) {
+  out[m] = make_float2(1.f, 0.f);
+  double2 p = make_double2(1.0, 0.0);
+  for (int n = 1; n <= m; ++n) {
+    p = zmul(p, u);
+    out[m + n] = make_float2((float)p.x, (float)p.y);
+    out[m - n] = make_float2((float)p.x, (float)-p.y);
+  }
+}
+
 // Powers u^n for n = -m..m into out[0..2m] (n = 0 at out[m]).
 __device__ inline void powers(float2 u, int m, float2* out) {
   out[m] = make_float2(1.f, 0.f);
@@ -68,66 +96,216 @@ __device__ inline void powers(float2 u, int m, float2* out) {
   }
 }
 
-__global__ __launch_bounds__(EK_THREADS) void ek_sk_kernel(EkArgs a) {
-  __shared__ float2 tx[EK_TA][EK_NM + 1];
-  __shared__ float2 ty[EK_TA][EK_W];
-  __shared__ float2 tz[EK_TA][EK_W];
-  const int s = blockIdx.y;
-  const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
-  const int tid = threadIdx.x;
-  const int k = blockIdx.x * EK_THREADS + tid;
-  const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
-  int kx = 0, ky = 0, kz = 0;
-  if (k < a.nk) {
-    kx = k / (wy * wz);
-    ky = (k / wz) % wy;   // index into ty (n1 + my)
-    kz = k % wz;          // index into tz (n2 + mz)
-  }
-  const double* t = a.tril + 6 * s;
-  double accr = 0.0, acci = 0.0;
-  for (int base = 0; base < n; base += EK_TA) {
+// S(k) as a real FP32 GEMM on the matrix cores.  With m = (n0, n1)
+// (M = (mx+1)(2my+1) rows) and A_i(m) = q_i X_i(n0) Y_i(n1):
+//   S(m, n2) = sum_i A_i(m) Z_i(n2),
+// stacked over (atom, re/im) on K = 2 x atoms and (re, im) of S on N = 2 wz:
+//   S_re = sum Re A Re Z - Im A Im Z,  S_im = sum Re A Im Z + Im A Re Z.
+// One workgroup per (system, chunk of EK_CHUNK atoms).  Per tile of TA atoms
+// the workgroup builds the per-axis phase tables (FP64, rounded once), packs
+// the GEMM operands K-major into LDS (A: 2 TA x M, B: 2 TA x N, leading
+// dimensions = 16 mod 64 floats so the four k rows of a fragment hit disjoint
+// banks), and the 4 waves, each owning every 4th 16-row tile of M and all N
+// tiles, run v_mfma_f32_16x16x4_f32 on fragments loaded straight from the
+// panels (A 16 x 4: lane -> row l % 16, k = l / 16; B 4 x 16: col l % 16,
+// k = l / 16; D: row 4 (l / 16) + r, col l % 16).  The FP32 sums of a tile
+// are added into FP64 registers; the chunk's partial S goes to HBM for a
+// fixed-order reduce.
+constexpr int EK_SK_THREADS = 256;
+constexpr int EK_WAVES = EK_SK_THREADS / 64;
+constexpr int EK_CHUNK = 512;
+
+struct EkSkShape {
+  int ta, lda, ldb;  // atoms per tile, panel leading dimensions (floats)
+  size_t lds;        // dynamic LDS bytes
+};
+
+__host__ __device__ inline int ek_ld16(int n) {  // smallest ld >= n with ld % 64 == 16
+  int ld = n - (n % 64) + 16;
+  return ld < n ? ld + 64 : ld;
+}
+
+template <int MTW, int NT>
+__global__ __launch_bounds__(EK_SK_THREADS) void ek_sk_kernel(EkArgs a, EkSkShape sh) {
+  extern __shared__ float ek_lds[];
+  __shared__ double s_li[6];
+  const int TA = sh.ta, lda = sh.lda, ldb = sh.ldb;
+  float* apk = ek_lds;                          // [2 TA][lda]
+  float* bpk = apk + (size_t)2 * TA * lda;      // [2 TA][ldb]
+  float2* tx = reinterpret_cast<float2*>(bpk + (size_t)2 * TA * ldb);  // [TA][mx + 1]
+  const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1, wx = a.mx + 1;
+  float2* ty = tx + (size_t)TA * wx;            // [TA][wy]
+  float2* tz = ty + (size_t)TA * wy;            // [TA][wz]
+  const int s = blockIdx.y, chunk = blockIdx.x;
+  const int i0 = a.sys_off[s], n_all = a.sys_off[s + 1] - i0;
+  const int c0 = chunk * EK_CHUNK;
+  if (c0 >= n_all) return;
+  const int n = n_all - c0 < EK_CHUNK ? n_all - c0 : EK_CHUNK;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int M = wx * wy, NC = 2 * wz;
+  const int mt_n = (M + 15) / 16;
+  const int row = lane & 15, kk = lane >> 4;
+  if (tid == 0) tril_inverse(a.tril + 6 * s, s_li);
+  // zero the panels once: padding rows / columns are never written again
+  for (int e = tid; e < 2 * TA * (lda + ldb); e += EK_SK_THREADS) apk[e] = 0.f;
+  double sum[MTW][NT][4];
+#pragma unroll
+  for (int t = 0; t < MTW; ++t)
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sum[t][u][r] = 0.0;
+  for (int base = 0; base < n; base += TA) {
     __syncthreads();
-    // three quarter-warps of 64 threads build the x, y, z tables of 64 atoms
-    const int role = tid / EK_TA, at = tid % EK_TA;
-    if (role < 3) {
-      const int i = base + at;
-      if (i < n) {
-        float2 ux, uy, uz;
-        unit_phases(t, a.pos + 3 * (size_t)(i0 + i), ux, uy, uz);
-        if (role == 0) {
-          const float qi = a.q[i0 + i];
-          float2 p = make_float2(qi, 0.f);
-          for (int m = 0; m <= a.mx; ++m) {
-            tx[at][m] = p;
-            p = cmul(p, ux);
-          }
-        } else if (role == 1) {
-          powers(uy, a.my, &ty[at][0]);
-        } else {
-          powers(uz, a.mz, &tz[at][0]);
-        }
+    const int cnt = n - base < TA ? n - base : TA;
+    // 1. per-axis phase tables (q folded into x); atoms past the end are zero
+    for (int e = tid; e < 3 * TA; e += EK_SK_THREADS) {
+      const int axis = e / TA, at = e % TA;
+      if (at < cnt) {
+        const float* r = a.pos + 3 * (size_t)(i0 + c0 + base + at);
+        double f0, f1, f2;
+        tmul(s_li, (double)r[0], (double)r[1], (double)r[2], f0, f1, f2);
+        double f = axis == 0 ? f0 : (axis == 1 ? f1 : f2);
+        f -= rint(f);
+        double2 u;
+        sincospi(2.0 * f, &u.y, &u.x);
+        if (axis == 0)
+          powers64_pos(u, (double)a.q[i0 + c0 + base + at], a.mx, tx + (size_t)at * wx);
+        else if (axis == 1)
+          powers64(u, a.my, ty + (size_t)at * wy);
+        else
+          powers64(u, a.mz, tz + (size_t)at * wz);
       } else {
-        if (role == 0)
-          for (int m = 0; m <= a.mx; ++m) tx[at][m] = make_float2(0.f, 0.f);
+        const int w = axis == 0 ? wx : (axis == 1 ? wy : wz);
+        float2* o = axis == 0 ? tx + (size_t)at * wx : (axis == 1 ? ty + (size_t)at * wy : tz + (size_t)at * wz);
+        for (int m = 0; m < w; ++m) o[m] = make_float2(0.f, 0.f);
       }
     }
     __syncthreads();
-    if (k < a.nk) {
-      const int cnt = n - base < EK_TA ? n - base : EK_TA;
-      float sr = 0.f, si = 0.f;
-      for (int j = 0; j < cnt; ++j) {
-        const float2 v = cmul(cmul(tx[j][kx], ty[j][ky]), tz[j][kz]);
-        sr += v.x;
-        si += v.y;
+    // 2. pack A (k = 2 at + re/im, m) and B (k, j): j < wz -> Re S, else Im S
+    for (int m = tid; m < M; m += EK_SK_THREADS) {
+      const int n0 = m / wy, n1 = m - n0 * wy;
+      const float2* px = tx + n0;
+      const float2* py = ty + n1;
+      float* pa = apk + m;
+      for (int at = 0; at < TA; ++at) {
+        const float2 A = cmul(px[(size_t)at * wx], py[(size_t)at * wy]);
+        pa[(size_t)(2 * at) * lda] = A.x;
+        pa[(size_t)(2 * at + 1) * lda] = A.y;
       }
-      accr += sr;
-      acci += si;
+    }
+    for (int e = tid; e < TA * wz; e += EK_SK_THREADS) {
+      const int at = e / wz, j = e % wz;
+      const float2 z = tz[(size_t)at * wz + j];
+      float* b0 = bpk + (size_t)(2 * at) * ldb;
+      float* b1 = b0 + ldb;
+      b0[j] = z.x;  b1[j] = -z.y;       // Re S: Re A Re Z - Im A Im Z
+      b0[wz + j] = z.y;  b1[wz + j] = z.x;  // Im S: Re A Im Z + Im A Re Z
+    }
+    __syncthreads();
+    // 3. matrix-core products over this tile's K = 2 TA
+    f32x4 acc[MTW][NT];
+#pragma unroll
+    for (int t = 0; t < MTW; ++t)
+#pragma unroll
+      for (int u = 0; u < NT; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const float* ap = apk + (size_t)kk * lda + wave * 16 + row;
+    const float* bp = bpk + (size_t)kk * ldb + row;
+    for (int k0 = 0; k0 < 2 * TA; k0 += 4) {
+      float bv[NT];
+#pragma unroll
+      for (int u = 0; u < NT; ++u) bv[u] = bp[(size_t)k0 * ldb + u * 16];
+#pragma unroll
+      for (int t = 0; t < MTW; ++t) {
+        if (wave + t * EK_WAVES >= mt_n) break;
+        const float av = ap[(size_t)k0 * lda + t * EK_WAVES * 16];
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[u], acc[t][u], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MTW; ++t)
+#pragma unroll
+      for (int u = 0; u < NT; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sum[t][u][r] += (double)acc[t][u][r];
+  }
+  // D[row 4 (l/16) + r][col l % 16] of tile (t, u): m = 16 mt + row, column j
+  double* part = a.skpart + ((size_t)s * a.chunks + chunk) * a.nk * 2;
+#pragma unroll
+  for (int t = 0; t < MTW; ++t) {
+    const int mt = wave + t * EK_WAVES;
+    if (mt >= mt_n) continue;
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const int j = u * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = mt * 16 + (lane >> 4) * 4 + r;
+        if (m >= M || j >= NC) continue;
+        const int k = m * wz + (j < wz ? j : j - wz);
+        part[2 * k + (j < wz ? 0 : 1)] = sum[t][u][r];
+      }
     }
   }
-  if (k < a.nk) {
-    double* o = a.sk + 2 * ((size_t)s * a.nk + k);
-    o[0] = accr;
-    o[1] = acci;
+}
+
+// S of each system: fixed-order sum of its chunk partials.
+__global__ void ek_sk_reduce_kernel(EkArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  if (k >= a.nk) return;
+  const int n = a.sys_off[s + 1] - a.sys_off[s];
+  const int nc = (n + EK_CHUNK - 1) / EK_CHUNK;
+  double re = 0.0, im = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    const double* p = a.skpart + (((size_t)s * a.chunks + c) * a.nk + k) * 2;
+    re += p[0];
+    im += p[1];
+  }
+  double* o = a.sk + 2 * ((size_t)s * a.nk + k);
+  o[0] = re;
+  o[1] = im;
+}
+
+template <int MTW>
+void ek_sk_launch_nt(const EkArgs& a, const EkSkShape& sh, int nt, dim3 g, hipStream_t st) {
+  switch (nt) {
+    case 1: ek_sk_kernel<MTW, 1><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
+    case 2: ek_sk_kernel<MTW, 2><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
+    case 3: ek_sk_kernel<MTW, 3><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
+    default: ek_sk_kernel<MTW, 4><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
+  }
+}
+
+// Tile size: the largest even TA <= 64 whose panels and tables fit 64 KB.
+EkSkShape ek_sk_shape(const EkArgs& a) {
+  const int wx = a.mx + 1, wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
+  const int mp = ((wx * wy + 15) / 16) * 16, np = ((2 * wz + 15) / 16) * 16;
+  EkSkShape sh;
+  sh.lda = ek_ld16(mp);
+  sh.ldb = ek_ld16(np);
+  const size_t per_atom = sizeof(float) * 2 * (size_t)(sh.lda + sh.ldb) +
+                          sizeof(float2) * (size_t)(wx + wy + wz);
+  int ta = (int)((64 * 1024) / per_atom);
+  ta = ta > 64 ? 64 : ta;
+  ta &= ~1;
+  sh.ta = ta < 2 ? 2 : ta;
+  sh.lds = per_atom * sh.ta;
+  return sh;
+}
+
+void ek_sk_launch(const EkArgs& a, dim3 g, hipStream_t st) {
+  const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
+  const int mt = ((a.mx + 1) * wy + 15) / 16, nt = (2 * wz + 15) / 16;
+  const int mtw = (mt + EK_WAVES - 1) / EK_WAVES;
+  const EkSkShape sh = ek_sk_shape(a);
+  switch (mtw) {
+    case 1: ek_sk_launch_nt<1>(a, sh, nt, g, st); break;
+    case 2: ek_sk_launch_nt<2>(a, sh, nt, g, st); break;
+    case 3: ek_sk_launch_nt<3>(a, sh, nt, g, st); break;
+    case 4: ek_sk_launch_nt<4>(a, sh, nt, g, st); break;
+    default: ek_sk_launch_nt<6>(a, sh, nt, g, st); break;  // up to 21 tiles (|n| <= 12)
   }
 }
 
@@ -267,14 +445,20 @@ extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   a.sys_off = sys_off; a.pos = pos; a.q = charges; a.tril = tril; a.alpha = alpha; a.kmax = kmax;
   a.forces = forces; a.energy = energy; a.virial = virial;
   const size_t nsk = (size_t)n_sys * a.nk;
+  a.chunks = max_atoms_per_sys > 0 ? (max_atoms_per_sys + EK_CHUNK - 1) / EK_CHUNK : 1;
   void* buf = nullptr;
   const size_t sk_bytes = structure_factor ? 0 : ((sizeof(double) * 2 * nsk + 255) & ~(size_t)255);
-  int rc = scratch(ctx, 4, sk_bytes + sizeof(float2) * nsk, &buf);
+  const size_t skp_bytes = (sizeof(float2) * nsk + 255) & ~(size_t)255;
+  const size_t part_bytes = sizeof(double) * 2 * nsk * (size_t)a.chunks;
+  int rc = scratch(ctx, 4, sk_bytes + skp_bytes + part_bytes, &buf);
   if (rc) return rc;
-  a.sk = structure_factor ? structure_factor : static_cast<double*>(buf);
-  a.skp = reinterpret_cast<float2*>(static_cast<char*>(buf) + sk_bytes);
-  dim3 g1((a.nk + EK_THREADS - 1) / EK_THREADS, n_sys);
-  ek_sk_kernel<<<g1, EK_THREADS, 0, st>>>(a);
+  char* b = static_cast<char*>(buf);
+  a.sk = structure_factor ? structure_factor : reinterpret_cast<double*>(b);
+  a.skp = reinterpret_cast<float2*>(b + sk_bytes);
+  a.skpart = reinterpret_cast<double*>(b + sk_bytes + skp_bytes);
+  ek_sk_launch(a, dim3(a.chunks, n_sys), st);
+  KUPSMI_CHECK_LAUNCH();
+  ek_sk_reduce_kernel<<<dim3((a.nk + 255) / 256, n_sys), 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   ek_energy_kernel<<<n_sys, EK_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();

@@ -89,28 +89,46 @@ __global__ __launch_bounds__(256) void ml_desc_kernel(int n_chain, int nv, const
 
 // ---------------------------------------------------- MFMA inference ----
 // One wave per 16-row tile: three 16 x 128 layers as v_mfma_f32_16x16x16_bf16
-// (A: 16 x 16 activations, B: 16 x 16 weights, FP32 accumulators), bias and
-// SiLU applied on the accumulators, activations re-staged in LDS as BF16.
-// Weights are held transposed ([out][in]) in LDS so that every B fragment is
-// one 8-byte load.
-constexpr int ML_WAVES = 4;
+// (A: 16 x 16 activations, B: 16 x 16 weights, FP32 accumulators).  The eight
+// 16-column output tiles of a layer are accumulated together, so eight
+// independent MFMA chains share each A fragment; bias and SiLU are applied on
+// the accumulators (reciprocal by v_rcp_f32: the result is rounded to BF16),
+// activations are re-staged in LDS as BF16.  Weights are held transposed
+// ([out][in]) in LDS so that every B fragment is one 8-byte load; rows are
+// padded by 8 elements so the 16 rows of a fragment fall in different banks.
+constexpr int ML_WAVES = 8;
+constexpr int ML_LD_IN = ML_IN + 8;  // padded row lengths (BF16 elements)
+constexpr int ML_LD_H = ML_H + 8;
 
-__device__ inline void ml_layer(const unsigned short* act, int kin, const unsigned short* wT,
+__device__ inline float silu_bf(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+template <int KIN, int LDK>
+__device__ inline void ml_layer(const unsigned short* act, const unsigned short* wT,
                                 const float* bias, unsigned short* out, int lane) {
   const int row = lane & 15, kq = (lane >> 4) * 4;
-  for (int nt = 0; nt < ML_H / 16; ++nt) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < kin / 16; ++ks) {
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(act + row * kin + ks * 16 + kq);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(wT + (nt * 16 + row) * kin + ks * 16 + kq);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  f32x4 acc[ML_H / 16];
+#pragma unroll
+  for (int nt = 0; nt < ML_H / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KIN / 16; ++ks) {
+    const bf16x4 a = *reinterpret_cast<const bf16x4*>(act + row * LDK + ks * 16 + kq);
+#pragma unroll
+    for (int nt = 0; nt < ML_H / 16; ++nt) {
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(wT + (nt * 16 + row) * LDK + ks * 16 + kq);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
     }
-    // D[i][j]: i = (lane / 16) * 4 + r, j = lane % 16
+  }
+  // D[i][j]: i = (lane / 16) * 4 + r, j = lane % 16 (+ 16 nt)
+#pragma unroll
+  for (int nt = 0; nt < ML_H / 16; ++nt) {
     const int j = nt * 16 + (lane & 15);
+    const float bj = bias[j];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = (lane >> 4) * 4 + r;
-      out[i * ML_H + j] = f2bf(silu(acc[r] + bias[j]));
+      out[i * ML_LD_H + j] = f2bf(silu_bf(acc[nt][r] + bj));
     }
   }
 }
@@ -118,18 +136,20 @@ __device__ inline void ml_layer(const unsigned short* act, int kin, const unsign
 __global__ __launch_bounds__(64 * ML_WAVES) void ml_infer_kernel(int n_rows, const unsigned short* desc,
                                                                 const unsigned short* wbf,
                                                                 const float* wf, float* logits) {
-  __shared__ unsigned short w1t[ML_H * ML_IN], w2t[ML_H * ML_H], w3t[ML_H * ML_H];
+  __shared__ __attribute__((aligned(16))) unsigned short w1t[ML_H * ML_LD_IN];
+  __shared__ __attribute__((aligned(16))) unsigned short w2t[ML_H * ML_LD_H], w3t[ML_H * ML_LD_H];
   __shared__ float b1[ML_H], b2[ML_H], b3[ML_H], w4[ML_H];
-  __shared__ unsigned short bufA[ML_WAVES][16 * ML_H], bufB[ML_WAVES][16 * ML_H];
+  __shared__ __attribute__((aligned(16))) unsigned short bufA[ML_WAVES][16 * ML_LD_H];
+  __shared__ __attribute__((aligned(16))) unsigned short bufB[ML_WAVES][16 * ML_LD_H];
   const int tid = threadIdx.x;
   for (int e = tid; e < ML_IN * ML_H; e += blockDim.x) {
     const int k = e / ML_H, n = e % ML_H;  // W1[k][n]
-    w1t[n * ML_IN + k] = wbf[ML_OFF_W1 + e];
+    w1t[n * ML_LD_IN + k] = wbf[ML_OFF_W1 + e];
   }
   for (int e = tid; e < ML_H * ML_H; e += blockDim.x) {
     const int k = e / ML_H, n = e % ML_H;
-    w2t[n * ML_H + k] = wbf[ML_OFF_W2 + e];
-    w3t[n * ML_H + k] = wbf[ML_OFF_W3 + e];
+    w2t[n * ML_LD_H + k] = wbf[ML_OFF_W2 + e];
+    w3t[n * ML_LD_H + k] = wbf[ML_OFF_W3 + e];
   }
   for (int e = tid; e < ML_H; e += blockDim.x) {
     b1[e] = wf[ML_OFF_B1 + e];
@@ -145,23 +165,25 @@ __global__ __launch_bounds__(64 * ML_WAVES) void ml_infer_kernel(int n_rows, con
   const int ntiles = (n_rows + 15) / 16;
   for (int tile = blockIdx.x * ML_WAVES + wave; tile < ntiles; tile += gridDim.x * ML_WAVES) {
     const int r0 = tile * 16;
-    // input tile 16 x 32 (rows past the end are zero)
-    for (int e = lane; e < 16 * ML_IN; e += 64) {
-      const int r = e / ML_IN, k = e % ML_IN;
-      A[r * ML_IN + k] = r0 + r < n_rows ? desc[(size_t)(r0 + r) * ML_IN + k] : (unsigned short)0;
+    // input tile 16 x 32: one 16-byte load per lane (rows past the end are zero)
+    {
+      const int r = lane >> 2, k8 = (lane & 3) * 8;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (r0 + r < n_rows) v = *reinterpret_cast<const uint4*>(desc + (size_t)(r0 + r) * ML_IN + k8);
+      *reinterpret_cast<uint4*>(A + r * ML_LD_IN + k8) = v;
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    ml_layer(A, ML_IN, w1t, b1, B, lane);
+    ml_layer<ML_IN, ML_LD_IN>(A, w1t, b1, B, lane);
     __builtin_amdgcn_wave_barrier();
-    ml_layer(B, ML_H, w2t, b2, A, lane);
+    ml_layer<ML_H, ML_LD_H>(B, w2t, b2, A, lane);
     __builtin_amdgcn_wave_barrier();
-    ml_layer(A, ML_H, w3t, b3, B, lane);
+    ml_layer<ML_H, ML_LD_H>(A, w3t, b3, B, lane);
     __builtin_amdgcn_wave_barrier();
     // output: 4 lanes per row, 32 products each, shuffle reduction
     const int row = lane >> 2, part = lane & 3;
     float s = 0.f;
-    for (int j = part * 32; j < part * 32 + 32; ++j) s += bf2f(B[row * ML_H + j]) * w4[j];
+    for (int j = part * 32; j < part * 32 + 32; ++j) s += bf2f(B[row * ML_LD_H + j]) * w4[j];
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
     if (part == 0 && r0 + row < n_rows) logits[r0 + row] = s + b4;
