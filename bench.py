@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--cpu-sample-every", type=int, default=8,
                     help="CPU baseline runs every k-th system of the batch")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="1 GiB device copy (counter calibration) + one MD launch, no JSON line")
     ap.add_argument("--no-ewald", action="store_true")
     ap.add_argument("--ewald-steps", type=int, default=30)
     ap.add_argument("--ewald-warmup", type=int, default=5)
@@ -366,6 +368,14 @@ def main():
 
     params = LennardJonesParameters.from_dict(
         workloads.AR_CUTOFF, {"Ar": (workloads.AR_SIGMA, workloads.AR_EPS)})
+    if args.traffic_probe:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
+        # calibrate the counters against 1 GiB read + 1 GiB written
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     batch = workloads.batched_lj_boxes(n_sys=args.n_sys, seed=0)
     p0 = maxwell_boltzmann_momenta(batch.masses, batch.sizes, 94.4, np.random.default_rng(0))
     st = make_md_state(batch.positions.astype(np.float32), batch.masses, batch.species,
@@ -373,6 +383,12 @@ def main():
                        temperature=94.4, momenta=p0.astype(np.float32))
     prop = make_velocity_verlet_step(params)
     prop.evaluate_forces(st)
+    if args.traffic_probe:
+        prop(st, args.steps, record=True)
+        torch.cuda.synchronize()
+        print(f"traffic probe: {int(batch.sizes.sum())} atoms, {args.steps} steps in one launch",
+              flush=True)
+        return
     if args.warmup:
         prop(st, args.warmup, record=False)
     torch.cuda.synchronize()

@@ -121,8 +121,15 @@ int kupsmi_lj_forces(kupsmi_ctx* ctx, void* stream, int n_sys,
  * A steps wrap positions to [-0.5, 0.5) fractional (104-141); O uses
  * p <- c1 p + sqrt(kT (1 - c1^2)) sqrt(m) eta with eta from Philox4x32-10
  * keyed by (seed, global atom index, step), see kupsmi_normals.
- * The neighbor set is rebuilt every step (cell list in LDS), exactly the set
- * of kupsmi_neighborlist with particle exclusion.
+ * The neighbor set of every step is exactly the set of kupsmi_neighborlist
+ * with particle exclusion (the reference rebuilds it every step).  With
+ * skin == 0 it is rebuilt every step from a cell list in LDS.  With skin > 0
+ * (and n_steps >= 2) each system keeps Verlet lists of radius cutoff + skin
+ * (max_neighbors entries per atom, in HBM) and rebuilds them whenever an
+ * atom may have moved by skin/2 since the last build; every step still
+ * tests every listed pair against the cutoff, so the pair set is unchanged.
+ * Systems thinner than 2 cutoffs, and any system whose lists overflow,
+ * use the per-step cell list instead.
  * In/out: pos, mom, forces (F at the current positions on entry: the
  * reference initialises it to zero, md/data.py:188).
  * Per system: mass per atom (f32), dt, kT (eV), gamma (1/internal time).
@@ -136,9 +143,9 @@ int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n_steps,
                  int n_atoms, float* pos, float* mom, float* forces,
                  const float* mass, const int32_t* species, int n_species,
                  const double* sigma, const double* epsilon, const double* tril,
-                 const double* cutoff, const double* dt, const double* kT,
-                 const double* gamma, double* pot_energy, double* kin_energy,
-                 double* virial);
+                 const double* cutoff, double skin, int max_neighbors,
+                 const double* dt, const double* kT, const double* gamma,
+                 double* pot_energy, double* kin_energy, double* virial);
 
 /*
  * The standard normals eta[n_atoms x 3] that kupsmi_md_lj's O step uses at

@@ -12,19 +12,32 @@
 // set of core/neighborlist.py:692-786 with particle exclusion).
 #include "common.hpp"
 
+// Multiply-adds are fused only within one source expression: the forces of
+// a step are then the same bits whether or not that step also accumulates
+// the virial (a separate instantiation of the pair loops).
+#pragma clang fp contract(on)
+
 namespace kupsmi {
 
 constexpr int MD_THREADS = 1024;
 constexpr int MD_NMAX = KUPSMI_LDS_MAX_ATOMS;
 constexpr int MD_SLOTS = MD_NMAX / MD_THREADS;
-constexpr int MD_MAX_NC_AXIS = 10;  // <= 1000 cells per system
-constexpr int MD_NCMAX = 1024;
+// cells are at least rc/2 wide (stencil of +-2 cells per axis)
+constexpr int MD_SUB = 2;
+constexpr int MD_MAX_NC_AXIS = 13;  // <= 2197 cells per system
+constexpr int MD_NCMAX = MD_MAX_NC_AXIS * MD_MAX_NC_AXIS * MD_MAX_NC_AXIS;
+constexpr int MD_CPT = (MD_NCMAX + MD_THREADS - 1) / MD_THREADS;
 constexpr int MD_NSPEC = 8;
 // relative half width of the FP32 cutoff band re-decided in FP64
 constexpr float CUT_BAND = 4e-6f;
 
 struct MdArgs {
   int integrator, n_steps, n_sys, n_species, want_virial;
+  // Verlet lists: skin > 0 and nblist != NULL enable them; nbmax entries per
+  // atom, MD_NMAX atoms per workgroup
+  float skin;
+  int nbmax;
+  unsigned short* nblist;
   int64_t step0;
   uint64_t seed;
   const int32_t* sys_off;
@@ -81,16 +94,37 @@ __global__ __launch_bounds__(1024) void order_kernel(int n_sys, const int32_t* s
 }
 
 // Geometry of every system (cells per axis, image stencil, inverse cell).
+// Cells and images are sized for the list radius cutoff + skin; rc / rc2 /
+// rc2f hold the cutoff itself, and `rep` tells whether the cutoff itself
+// needs periodic images (a box thinner than 2 rc: no Verlet list).
 __global__ void geom_kernel(int n_sys, const double* tril, const double* cutoff, int max_nc,
-                            SysGeom* geom) {
+                            double skin, SysGeom* geom) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n_sys) make_geom(tril + 6 * s, cutoff[s], max_nc, geom[s]);
+  if (s >= n_sys) return;
+  const double rc = cutoff[s];
+  SysGeom g0;
+  make_geom(tril + 6 * s, rc, max_nc, g0, MD_SUB);
+  SysGeom g;
+  make_geom(tril + 6 * s, rc + (skin > 0.0 ? skin : 0.0), max_nc, g, MD_SUB);
+  g.rc = rc;
+  g.rc2 = rc * rc;
+  g.rc2f = (float)g.rc2;
+  g.rep = g0.rep;
+  geom[s] = g;
 }
 
-// Exclusive scan of cnt[0..n) into start[0..n] (n <= 1024) by the block.
+// Exclusive scan of cnt[0..n) into start[0..n] (n <= MD_NCMAX) by the
+// block: MD_CPT consecutive cells per thread, then a scan of the thread sums.
 __device__ inline void block_scan_cells(const int* cnt, int* start, int n, int* wsum) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  int own = tid < n ? cnt[tid] : 0;
+  const int c0 = tid * MD_CPT;
+  int loc[MD_CPT];
+  int own = 0;
+#pragma unroll
+  for (int k = 0; k < MD_CPT; ++k) {
+    loc[k] = c0 + k < n ? cnt[c0 + k] : 0;
+    own += loc[k];
+  }
   int v = own;
   for (int o = 1; o < 64; o <<= 1) {
     int y = __shfl_up(v, o, 64);
@@ -108,8 +142,12 @@ __device__ inline void block_scan_cells(const int* cnt, int* start, int n, int* 
   }
   __syncthreads();
   int ex = v - own + (wv > 0 ? wsum[wv - 1] : 0);
-  if (tid < n) start[tid] = ex;
-  if (tid == n - 1) start[n] = ex + own;
+#pragma unroll
+  for (int k = 0; k < MD_CPT; ++k) {
+    if (c0 + k < n) start[c0 + k] = ex;
+    ex += loc[k];
+    if (c0 + k == n - 1) start[n] = ex;
+  }
   if (n == 0 && tid == 0) start[0] = 0;
 }
 
@@ -142,26 +180,291 @@ __device__ inline bool band_keep(const SysGeom& g, float4 ri, float4 rj, int sx,
   return d0 * d0 + d1 * d1 + d2 * d2 < g.rc2;
 }
 
+// floor(x / n) for a small integer x and 1 <= n <= MD_MAX_NC_AXIS: the
+// quotient of (x + 1/2) is at least 1/(2n) away from every integer, far more
+// than the rounding of the FP32 product.
+__device__ inline int floor_div_small(int x, float inv_n) {
+  return (int)floorf(((float)x + 0.5f) * inv_n);
+}
+
+// ---- per-atom force walks (one lane per atom) ------------------------------
+
+// LJ terms of one candidate with edge vector e = r_j + s@L - r_i and FP32 r^2.
+// The keep decision is folded into 1/r^2 (zero for a rejected candidate).
+template <bool VIR>
+__device__ inline void md_lj_terms(float ex, float ey, float ez, float r2, bool keep, float4 pp,
+                                   float& ax, float& ay, float& az, float& ei, float (&w)[6]) {
+  const float ir2 = keep ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+  float c6 = pp.x * ir2;
+  c6 = c6 * c6 * c6;
+  ei += pp.y * c6 * (c6 - 1.0f);
+  const float g2 = pp.z * c6 * (1.0f - 2.0f * c6) * ir2;  // u'(r) / r
+  ax += g2 * ex; ay += g2 * ey; az += g2 * ez;
+  if (VIR) {
+    const float hg = -0.5f * g2;
+    w[0] += hg * ex * ex; w[1] += hg * ex * ey; w[2] += hg * ex * ez;
+    w[3] += hg * ey * ey; w[4] += hg * ey * ez; w[5] += hg * ez * ez;
+  }
+}
+
+// FP64 re-decision of a band pair on its nearest image: s = rint(f_i - f_j)
+// per axis, kept when |(f_i - f_j - s) @ L|^2 < rc^2 (the reference's
+// fractional formula, core/neighborlist.py:542-564, 589).  Out of line: the
+// rare call claims no registers in the pair loops.
+__device__ __attribute__((noinline)) bool md_band_nearest(const SysGeom& g, float4 ri, float4 rj) {
+  double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
+  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
+  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
+  const double t0 = fi0 - fj0, t1 = fi1 - fj1, t2 = fi2 - fj2;
+  tmul(g.L, t0 - rint(t0), t1 - rint(t1), t2 - rint(t2), d0, d1, d2);
+  return d0 * d0 + d1 * d1 + d2 * d2 < g.rc2;
+}
+
+__device__ __attribute__((noinline)) bool md_band_shift(const SysGeom& g, float4 ri, float4 rj,
+                                                        int sx, int sy, int sz) {
+  return band_keep(g, ri, rj, sx, sy, sz);
+}
+
+// Forces on the atom in sorted slot `slot` from every candidate of its cell
+// stencil.  Atoms are sorted by cell index (z ny + y) nx + x, so for a
+// stencil row (dz, dy) the cells cx-h .. cx+h of one image shift sx are one
+// contiguous run of sp[]: the row is walked as one run per image.
+template <bool MULTI, bool VIR>
+__device__ inline void md_stencil_forces(const SysGeom& g, const float4* sp, const int* cstart,
+                                         const float4* s_par, int ns, int slot, float rc2_lo,
+                                         float rc2_hi, float& ax, float& ay, float& az, float& ei,
+                                         float (&w)[6]) {
+  const float4 ri = sp[slot];
+  const int wi = __float_as_int(ri.w);
+  const int c = (wi >> 8) & 0xfff;
+  const int si = wi & 0xff;
+  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+  const float ix = 1.0f / (float)ncx, iy = 1.0f / (float)ncy, iz = 1.0f / (float)ncz;
+  const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+  const int h0 = g.h[0], h1 = g.h[1], h2 = g.h[2];
+  const float4 p0 = s_par[0];
+  const int sx_lo = floor_div_small(cx - h0, ix), sx_hi = floor_div_small(cx + h0, ix);
+  for (int dz = -h2; dz <= h2; ++dz) {
+    int zz = cz + dz;
+    const int sz = floor_div_small(zz, iz);
+    zz -= sz * ncz;
+    for (int dy = -h1; dy <= h1; ++dy) {
+      int yy = cy + dy;
+      const int sy = floor_div_small(yy, iy);
+      yy -= sy * ncy;
+      const int row = (zz * ncy + yy) * ncx;
+      for (int sx = sx_lo; sx <= sx_hi; ++sx) {
+        const int x0 = max(cx - h0 - sx * ncx, 0), x1 = min(cx + h0 - sx * ncx, ncx - 1);
+        float Sx, Sy, Sz;
+        tmul(g.Lf, (float)sx, (float)sy, (float)sz, Sx, Sy, Sz);
+        const float qx = ri.x - Sx, qy = ri.y - Sy, qz = ri.z - Sz;
+        const int self = (sx | sy | sz) == 0 ? slot : -1;
+        const int tb = cstart[row + x0], te = cstart[row + x1 + 1];
+        for (int t = tb; t < te; ++t) {
+          const float4 rj = sp[t];
+          const float ex = rj.x - qx, ey = rj.y - qy, ez = rj.z - qz;
+          const float r2 = ex * ex + ey * ey + ez * ez;
+          const bool lo = r2 < rc2_lo;
+          bool keep = lo && t != self;
+          if (__builtin_expect(!lo && r2 < rc2_hi, 0)) keep = md_band_shift(g, ri, rj, sx, sy, sz);
+          const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj.w) & 0xff)] : p0;
+          md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
+        }
+      }
+    }
+  }
+}
+
+// Verlet list of the atom in sorted slot `slot`: every atom j with an image
+// within the list radius, once.  The stencil may visit j under several image
+// shifts; only the visit whose shift is the per-axis nearest image,
+// s = -rint((r_j - r_i) @ L^-1) (one FP32 evaluation per pair, the same for
+// every visit), is kept, and it is kept when that image lies within the list
+// radius (orthorhombic cells, where it is the nearest image) or within the
+// slab |f - s| < (rc + skin) / perp on every axis (triclinic cells, a
+// superset of the sphere).  An atom within the cutoff at a later step (every
+// atom moved by less than skin/2) is therefore in the list.  Entries are
+// sorted slots, entry-major (nl[k * MD_NMAX]).  Returns the count, which may
+// exceed nbmax (entries past nbmax are not written).
+__device__ inline int md_build_list(const SysGeom& g, const float4* sp, const int* cstart, int slot,
+                                    unsigned short* nl, int nbmax, float list_r2, bool ortho,
+                                    float fx_max, float fy_max, float fz_max) {
+  const float4 ri = sp[slot];
+  const int c = (__float_as_int(ri.w) >> 8) & 0xfff;
+  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+  const float ix = 1.0f / (float)ncx, iy = 1.0f / (float)ncy, iz = 1.0f / (float)ncz;
+  const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+  const int h0 = g.h[0], h1 = g.h[1], h2 = g.h[2];
+  const int sx_lo = floor_div_small(cx - h0, ix), sx_hi = floor_div_small(cx + h0, ix);
+  int cnt = 0;
+  for (int dz = -h2; dz <= h2; ++dz) {
+    int zz = cz + dz;
+    const int sz = floor_div_small(zz, iz);
+    zz -= sz * ncz;
+    for (int dy = -h1; dy <= h1; ++dy) {
+      int yy = cy + dy;
+      const int sy = floor_div_small(yy, iy);
+      yy -= sy * ncy;
+      const int row = (zz * ncy + yy) * ncx;
+      for (int sx = sx_lo; sx <= sx_hi; ++sx) {
+        const int x0 = max(cx - h0 - sx * ncx, 0), x1 = min(cx + h0 - sx * ncx, ncx - 1);
+        float Sx, Sy, Sz;
+        tmul(g.Lf, (float)sx, (float)sy, (float)sz, Sx, Sy, Sz);
+        const float qx = ri.x - Sx, qy = ri.y - Sy, qz = ri.z - Sz;
+        const int tb = cstart[row + x0], te = cstart[row + x1 + 1];
+        for (int t = tb; t < te; ++t) {
+          if (t == slot) continue;  // self images are never within 2 rc
+          const float4 rj = sp[t];
+          const float ex = rj.x - qx, ey = rj.y - qy, ez = rj.z - qz;
+          const float r2 = ex * ex + ey * ey + ez * ez;
+          float f0, f1, f2;
+          tmul(g.Lif, rj.x - ri.x, rj.y - ri.y, rj.z - ri.z, f0, f1, f2);
+          const float n0 = rintf(f0), n1 = rintf(f1), n2 = rintf(f2);
+          const bool nearest = (int)n0 == -sx && (int)n1 == -sy && (int)n2 == -sz;
+          const bool near = ortho ? r2 < list_r2
+                                  : (fabsf(f0 - n0) < fx_max && fabsf(f1 - n1) < fy_max &&
+                                     fabsf(f2 - n2) < fz_max);
+          if (nearest && near) {
+            if (cnt < nbmax) nl[(size_t)cnt * MD_NMAX] = (unsigned short)t;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  return cnt;
+}
+
+// Forces on the atom in sorted slot `slot` from its Verlet list: each entry
+// on its per-axis nearest image (the only image within the cutoff when every
+// perpendicular width is at least 2 rc).  Four entries per trip, their list
+// words and positions loaded before any is used.
+template <bool MULTI, bool VIR>
+__device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const unsigned short* nl,
+                                      int cnt, int slot, const float4* s_par, int ns,
+                                      float rc2_lo, float rc2_hi, float& ax, float& ay, float& az,
+                                      float& ei, float (&w)[6]) {
+  const float4 ri = sp[slot];
+  const int si = __float_as_int(ri.w) & 0xff;
+  const float4 p0 = s_par[0];
+  for (int k = 0; k < cnt; k += 4) {
+    int j[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = nl[(size_t)min(k + u, cnt - 1) * MD_NMAX];
+    float4 rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rj[u] = sp[j[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float ex = rj[u].x - ri.x, ey = rj[u].y - ri.y, ez = rj[u].z - ri.z;
+      float f0, f1, f2;
+      tmul(g.Lif, ex, ey, ez, f0, f1, f2);
+      float sx, sy, sz;
+      tmul(g.Lf, rintf(f0), rintf(f1), rintf(f2), sx, sy, sz);
+      ex -= sx; ey -= sy; ez -= sz;
+      const float r2 = ex * ex + ey * ey + ez * ez;
+      const bool valid = k + u < cnt;
+      const bool lo = r2 < rc2_lo;
+      bool keep = valid && lo;
+      if (__builtin_expect(valid && !lo && r2 < rc2_hi, 0)) keep = md_band_nearest(g, ri, rj[u]);
+      const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj[u].w) & 0xff)] : p0;
+      md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
+    }
+  }
+}
+
+// Forces, closing B(dt/2) (+ the next step's opening B(dt/2)) and kinetic
+// energy of every slot of the thread; the forces of the last step go to HBM.
+template <bool MULTI, bool VIR>
+__device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const float4* sp,
+                                      float4* pb, const int* cstart, const short* nbcount,
+                                      const unsigned short* nbl, bool use_list,
+                                      const float4* s_par, int ns, int n, int i0, float rc2_lo,
+                                      float rc2_hi, float hdt, float dtf, bool last, float& epot,
+                                      float& ekin, float (&w)[6]) {
+  const int tid = threadIdx.x;
+#pragma unroll 1
+  for (int k = 0; k < MD_SLOTS; ++k) {
+    const int slot = tid + k * MD_THREADS;
+    if (slot >= n) continue;
+    float ax = 0.f, ay = 0.f, az = 0.f, ei = 0.f;
+    if (use_list)
+      md_list_forces<MULTI, VIR>(g, sp, nbl + slot, nbcount[slot], slot, s_par, ns, rc2_lo, rc2_hi,
+                                 ax, ay, az, ei, w);
+    else
+      md_stencil_forces<MULTI, VIR>(g, sp, cstart, s_par, ns, slot, rc2_lo, rc2_hi, ax, ay, az, ei,
+                                    w);
+    epot += 0.5f * ei;
+    float4 p = pb[slot];
+    const int gi = i0 + __float_as_int(p.w);
+    if (a.n_steps > 0 || a.kin) {
+      const float im = 1.0f / a.mass[gi];
+      const float vx = p.x + ax * hdt, vy = p.y + ay * hdt, vz = p.z + az * hdt;
+      ekin += 0.5f * (vx * vx + vy * vy + vz * vz) * im;
+    }
+    if (a.n_steps > 0) {
+      const float kick = last ? hdt : dtf;
+      p.x += ax * kick; p.y += ay * kick; p.z += az * kick;
+      pb[slot] = p;
+    }
+    if (last) {
+      a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
+    }
+  }
+}
+
+// Maximum of v over the workgroup, returned in every thread.
+__device__ inline float block_max_all(float v, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  __syncthreads();
+  if (lane == 0) red[wv] = (double)v;
+  __syncthreads();
+  double m = red[0];
+#pragma unroll
+  for (int k = 1; k < MD_THREADS / 64; ++k) m = fmax(m, red[k]);
+  return (float)m;
+}
+
+// buf[dst[k]] = buf[tid + k * MD_THREADS] for every dst[k] >= 0, in place:
+// every read completes at the barrier before any write.
+// The copies use a native 4-float vector: a float4 struct copy becomes a
+// memcpy through private memory that is not promoted to registers.
+__device__ inline void permute_slots(float4* buf, const int (&dst)[MD_SLOTS], int tid) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  v4f* vb = reinterpret_cast<v4f*>(buf);
+  v4f t[MD_SLOTS];
+#pragma unroll
+  for (int k = 0; k < MD_SLOTS; ++k) t[k] = vb[min(tid + k * MD_THREADS, MD_NMAX - 1)];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < MD_SLOTS; ++k)
+    if (dst[k] >= 0) vb[dst[k]] = t[k];
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
   __shared__ float4 sp[MD_NMAX];   // sorted x,y,z ; w = bits(species | cell << 8)
   __shared__ float4 pb[MD_NMAX];   // sorted px,py,pz ; w = bits(local id)
-  __shared__ short keys[MD_NMAX];  // previous slot of each provisional entry
+  // while sorting: previous slot of each provisional entry; between list
+  // builds: the Verlet list length of each slot
+  __shared__ short keys[MD_NMAX];
   __shared__ int cstart[MD_NCMAX + 1];
   __shared__ int ccount[MD_NCMAX];
   __shared__ int wsum[MD_THREADS / 64];
   __shared__ double red[MD_THREADS / 64];
-  __shared__ float s_sig2[MD_NSPEC * MD_NSPEC], s_e4[MD_NSPEC * MD_NSPEC],
-      s_e24[MD_NSPEC * MD_NSPEC];
+  __shared__ float4 s_par[MD_NSPEC * MD_NSPEC];  // sigma^2, 4 eps, 24 eps
   __shared__ SysGeom g;
   __shared__ int s_q;
 
   const int tid = threadIdx.x;
   const int ns = a.n_species;
+  unsigned short* nbl =
+      a.nblist ? a.nblist + (size_t)blockIdx.x * (size_t)a.nbmax * MD_NMAX : nullptr;
   for (int k = tid; k < ns * ns; k += MD_THREADS) {
     double sg = a.sigma[k];
-    s_sig2[k] = (float)(sg * sg);
-    s_e4[k] = (float)(4.0 * a.epsilon[k]);
-    s_e24[k] = (float)(24.0 * a.epsilon[k]);
+    s_par[k] = make_float4((float)(sg * sg), (float)(4.0 * a.epsilon[k]),
+                           (float)(24.0 * a.epsilon[k]), 0.0f);
   }
 
   for (;;) {
@@ -210,14 +513,34 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
       c1 = (float)e1;
       c2 = (float)sqrt(a.kT[s] * (1.0 - e1 * e1));
     }
+    // Verlet lists for boxes whose perpendicular widths are all >= 2 rc
+    // (one image per pair within the cutoff) and runs of >= 2 steps
+    bool use_list = nbl != nullptr && a.skin > 0.0f && g.rep == 0 && a.n_steps >= 2;
+    bool listed = false;
+    const double lr = g.rc + (double)a.skin;
+    const float list_r2 = (float)(lr * lr) * (1.0f + 1e-5f);
+    const bool ortho = g.L[1] == 0.0 && g.L[3] == 0.0 && g.L[4] == 0.0;
+    // triclinic list slab: |f| < (rc + skin) / perp per axis, perp = 1 / |row of L^-T|
+    const float fx_max = (float)(lr * sqrt(g.Li[0] * g.Li[0] + g.Li[1] * g.Li[1] + g.Li[3] * g.Li[3])) * 1.00001f;
+    const float fy_max = (float)(lr * sqrt(g.Li[2] * g.Li[2] + g.Li[4] * g.Li[4])) * 1.00001f;
+    const float fz_max = (float)(lr * fabs(g.Li[5])) * 1.00001f;
+    // per slot: sum of the step displacements since the list was built, an
+    // upper bound of the displacement
+    float dacc[MD_SLOTS];
+#pragma unroll
+    for (int k = 0; k < MD_SLOTS; ++k) dacc[k] = 0.0f;
 
     const int total = a.n_steps > 0 ? a.n_steps : 1;
     for (int st = 0; st < total; ++st) {
       const bool last = st == total - 1;
       // every force loop of the previous step has finished reading sp[]
       __syncthreads();
+      // reload the LDS-resident geometry in every step instead of keeping
+      // the hoisted copies live across the whole step loop
+      asm volatile("" ::: "memory");
       // ---- [B(dt/2) at st = 0] A(...) [O A] in place, wrap, cell of every atom
       int cel[MD_SLOTS];
+      float dmax = 0.0f;
 #pragma unroll
       for (int k = 0; k < MD_SLOTS; ++k) {
         const int slot = tid + k * MD_THREADS;
@@ -235,6 +558,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
           }
           const float ta = a.integrator == 1 ? hdt : dtf;
           r.x += p.x * im * ta; r.y += p.y * im * ta; r.z += p.z * im * ta;
+          float moved = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * im * ta;
           if (a.integrator == 1) {
             float f0, f1, f2;
             wrap_shift(g.L, g.Lif, r.x, r.y, r.z, f0, f1, f2);
@@ -245,8 +569,11 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
             p.y = c1 * p.y + sm * e1;
             p.z = c1 * p.z + sm * e2;
             r.x += p.x * im * hdt; r.y += p.y * im * hdt; r.z += p.z * im * hdt;
+            moved += sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) * im * hdt;
           }
           pb[slot] = p;
+          dacc[k] += moved;
+          dmax = fmaxf(dmax, dacc[k]);
         }
         float f0, f1, f2;
         wrap_shift(g.L, g.Lif, r.x, r.y, r.z, f0, f1, f2);
@@ -254,123 +581,84 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
         r.w = __int_as_float((__float_as_int(r.w) & 0xff) | (cel[k] << 8));
         sp[slot] = r;
       }
-      // ---- stable counting sort into cells (rank by previous slot)
-      for (int c = tid; c < ncell; c += MD_THREADS) ccount[c] = 0;
-      __syncthreads();
+      // the lists stay valid while no atom can have moved by skin/2
+      bool sort_now = true;
+      if (use_list && listed) sort_now = block_max_all(dmax, red) > 0.5f * a.skin;
+      if (sort_now) {
+        // ---- stable counting sort into cells (rank by previous slot)
+        __syncthreads();
+        for (int c = tid; c < ncell; c += MD_THREADS) ccount[c] = 0;
+        __syncthreads();
 #pragma unroll
-      for (int k = 0; k < MD_SLOTS; ++k)
-        if (tid + k * MD_THREADS < n) atomicAdd(&ccount[cel[k]], 1);
-      __syncthreads();
-      block_scan_cells(ccount, cstart, ncell, wsum);
-      __syncthreads();
-      for (int c = tid; c < ncell; c += MD_THREADS) ccount[c] = 0;
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < MD_SLOTS; ++k) {
-        const int slot = tid + k * MD_THREADS;
-        if (slot < n) keys[cstart[cel[k]] + atomicAdd(&ccount[cel[k]], 1)] = (short)slot;
-      }
-      __syncthreads();
-      {
-        int dst[MD_SLOTS];
-        float4 tr[MD_SLOTS], tp[MD_SLOTS];
-#pragma unroll
-        for (int k = 0; k < MD_SLOTS; ++k) {
-          const int slot = tid + k * MD_THREADS;
-          dst[k] = 0;
-          if (slot < n) {
-            const int b = cstart[cel[k]], e = cstart[cel[k] + 1];
-            int rank = 0;
-            for (int t = b; t < e; ++t) rank += keys[t] < slot;
-            dst[k] = b + rank;
-            tr[k] = sp[slot];
-            tp[k] = pb[slot];
-          }
-        }
+        for (int k = 0; k < MD_SLOTS; ++k)
+          if (tid + k * MD_THREADS < n) atomicAdd(&ccount[cel[k]], 1);
+        __syncthreads();
+        block_scan_cells(ccount, cstart, ncell, wsum);
+        __syncthreads();
+        for (int c = tid; c < ncell; c += MD_THREADS) ccount[c] = 0;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < MD_SLOTS; ++k) {
-          if (tid + k * MD_THREADS < n) {
-            sp[dst[k]] = tr[k];
-            pb[dst[k]] = tp[k];
-          }
+          const int slot = tid + k * MD_THREADS;
+          if (slot < n) keys[cstart[cel[k]] + atomicAdd(&ccount[cel[k]], 1)] = (short)slot;
         }
-      }
-      __syncthreads();
-      // ---- forces: every slot walks its stencil of (cell, image) pairs,
-      // then the kick and the kinetic energy of that slot
-      float epot = 0.0f, ekin = 0.0f;
-      float w00 = 0.f, w01 = 0.f, w02 = 0.f, w11 = 0.f, w12 = 0.f, w22 = 0.f;
-      const bool do_w = last && a.want_virial;
-      const int h0 = g.h[0], h1 = g.h[1], h2 = g.h[2];
-#pragma unroll 1
-      for (int k = 0; k < MD_SLOTS; ++k) {
-        const int slot = tid + k * MD_THREADS;
-        if (slot >= n) continue;
-        const float4 ri = sp[slot];
-        const int wi = __float_as_int(ri.w);
-        const int c = wi >> 8;
-        const int si = wi & 0xff;
-        const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-        float ax = 0.f, ay = 0.f, az = 0.f, ei = 0.f;
-        for (int dz = -h2; dz <= h2; ++dz) {
-          int zz = cz + dz;
-          const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
-          zz -= sz * ncz;
-          for (int dy = -h1; dy <= h1; ++dy) {
-            int yy = cy + dy;
-            const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
-            yy -= sy * ncy;
-            for (int dx = -h0; dx <= h0; ++dx) {
-              int xx = cx + dx;
-              const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
-              xx -= sx * ncx;
-              // image shift s @ L; edge vector d = r_j + s@L - r_i
-              float Sx, Sy, Sz;
-              tmul(g.Lf, (float)sx, (float)sy, (float)sz, Sx, Sy, Sz);
-              const float qx = ri.x - Sx, qy = ri.y - Sy, qz = ri.z - Sz;
-              const bool center = dx == 0 && dy == 0 && dz == 0;
-              const int nb = (zz * ncy + yy) * ncx + xx;
-              const int tb = cstart[nb], te = cstart[nb + 1];
-              for (int t = tb; t < te; ++t) {
-                const float4 rj = sp[t];
-                const float ex = rj.x - qx, ey = rj.y - qy, ez = rj.z - qz;
-                const float r2 = ex * ex + ey * ey + ez * ez;
-                if (!(r2 < rc2_hi) || (center && t == slot)) continue;
-                if (r2 > rc2_lo && !band_keep(g, ri, rj, sx, sy, sz)) continue;
-                const int p = si * ns + (__float_as_int(rj.w) & 0xff);
-                const float ir2 = 1.0f / r2;
-                float c6 = s_sig2[p] * ir2;
-                c6 = c6 * c6 * c6;
-                ei += s_e4[p] * c6 * (c6 - 1.0f);
-                const float g2 = s_e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;  // 2 u'
-                ax += g2 * ex; ay += g2 * ey; az += g2 * ez;
-                if (do_w) {
-                  const float hg = -0.5f * g2;
-                  w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
-                  w11 += hg * ey * ey; w12 += hg * ey * ez; w22 += hg * ez * ez;
-                }
-              }
+        __syncthreads();
+        {
+          // destination of every slot, then the positions and the momenta
+          // permuted one array at a time: only MD_SLOTS float4 are held in
+          // registers across a barrier (the 128-VGPR budget of a 1024-thread
+          // workgroup has no room for both)
+          int dst[MD_SLOTS];
+#pragma unroll
+          for (int k = 0; k < MD_SLOTS; ++k) {
+            const int slot = tid + k * MD_THREADS;
+            dst[k] = -1;
+            if (slot < n) {
+              const int b = cstart[cel[k]], e = cstart[cel[k] + 1];
+              int rank = 0;
+              for (int t = b; t < e; ++t) rank += keys[t] < slot;
+              dst[k] = b + rank;
             }
           }
+          permute_slots(sp, dst, tid);
+          permute_slots(pb, dst, tid);
         }
-        epot += 0.5f * ei;
-        // closing B(dt/2) (+ the next step's opening B(dt/2)), kinetic energy
-        float4 p = pb[slot];
-        const int gi = i0 + __float_as_int(p.w);
-        if (a.n_steps > 0 || a.kin) {
-          const float im = 1.0f / a.mass[gi];
-          const float vx = p.x + ax * hdt, vy = p.y + ay * hdt, vz = p.z + az * hdt;
-          ekin += 0.5f * (vx * vx + vy * vy + vz * vz) * im;
+        if (use_list) {
+          // ---- Verlet lists of the new order; an overflowing list turns
+          // the lists off for the rest of the run (the stencil is exact)
+          int over = 0;
+#pragma unroll 1
+          for (int k = 0; k < MD_SLOTS; ++k) {
+            const int slot = tid + k * MD_THREADS;
+            dacc[k] = 0.0f;
+            if (slot >= n) continue;
+            const int cnt = md_build_list(g, sp, cstart, slot, nbl + slot, a.nbmax, list_r2, ortho,
+                                          fx_max, fy_max, fz_max);
+            keys[slot] = (short)min(cnt, a.nbmax);
+            over |= cnt > a.nbmax;
+          }
+          use_list = !__syncthreads_or(over);
+          listed = use_list;
         }
-        if (a.n_steps > 0) {
-          const float kick = last ? hdt : dtf;
-          p.x += ax * kick; p.y += ay * kick; p.z += az * kick;
-          pb[slot] = p;
-        }
-        if (last) {
-          a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
-        }
+      }
+      // ---- forces, kick and kinetic energy
+      float epot = 0.0f, ekin = 0.0f;
+      float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      const bool do_w = last && a.want_virial;
+      if (ns == 1) {
+        if (do_w)
+          md_force_phase<false, true>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
+                                      rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
+        else
+          md_force_phase<false, false>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
+                                       rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
+      } else {
+        if (do_w)
+          md_force_phase<true, true>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
+                                     rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
+        else
+          md_force_phase<true, false>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
+                                      rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
       }
       if (a.pot) {
         double e = block_sum((double)epot, red);
@@ -381,10 +669,9 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
         if (tid == 0) a.kin[(size_t)s * total + st] = e;
       }
       if (do_w && a.virial) {
-        const float wv[6] = {w00, w01, w02, w11, w12, w22};
         double out[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) out[c] = block_sum((double)wv[c], red);
+        for (int c = 0; c < 6; ++c) out[c] = block_sum((double)w[c], red);
         if (tid == 0) {
           double* W = a.virial + 9 * (size_t)s;
           W[0] = out[0]; W[1] = out[1]; W[2] = out[2];
@@ -394,7 +681,8 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
       }
     }
     // ---- store the system back in its original atom order (forces were
-    // written by the last force pass)
+    // written by the last force pass; every slot is stored by the thread
+    // that kicked it)
     if (a.n_steps > 0) {
 #pragma unroll
       for (int k = 0; k < MD_SLOTS; ++k) {
@@ -440,7 +728,8 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
   int* queue = reinterpret_cast<int*>(order + a.n_sys);
   order_kernel<<<1, 1024, 0, stream>>>(a.n_sys, a.sys_off, order, queue);
   KUPSMI_CHECK_LAUNCH();
-  geom_kernel<<<(a.n_sys + 255) / 256, 256, 0, stream>>>(a.n_sys, a.tril, a.cutoff, MD_MAX_NC_AXIS, geom);
+  geom_kernel<<<(a.n_sys + 255) / 256, 256, 0, stream>>>(a.n_sys, a.tril, a.cutoff, MD_MAX_NC_AXIS,
+                                                         (double)a.skin, geom);
   KUPSMI_CHECK_LAUNCH();
   a.order = order;
   a.queue = queue;
@@ -449,6 +738,14 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
   KUPSMI_HIP(hipGetDevice(&dev));
   KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   int grid = a.n_sys < ncu ? a.n_sys : ncu;
+  a.nblist = nullptr;
+  if (a.skin > 0.0f && a.nbmax > 0 && a.n_steps >= 2) {
+    // one list region (nbmax entries x MD_NMAX slots) per resident workgroup
+    void* lb = nullptr;
+    rc = scratch(ctx, 6, sizeof(unsigned short) * (size_t)grid * a.nbmax * MD_NMAX, &lb);
+    if (rc) return rc;
+    a.nblist = static_cast<unsigned short*>(lb);
+  }
   md_lj_kernel<<<grid, MD_THREADS, 0, stream>>>(a);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
@@ -463,11 +760,14 @@ extern "C" int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n
                             int n_atoms, float* pos, float* mom, float* forces,
                             const float* mass, const int32_t* species, int n_species,
                             const double* sigma, const double* epsilon, const double* tril,
-                            const double* cutoff, const double* dt, const double* kT,
-                            const double* gamma, double* pot_energy, double* kin_energy,
-                            double* virial) {
+                            const double* cutoff, double skin, int max_neighbors,
+                            const double* dt, const double* kT, const double* gamma,
+                            double* pot_energy, double* kin_energy, double* virial) {
   KUPSMI_REQUIRE(n_atoms >= 0, "n_atoms < 0");
   KUPSMI_REQUIRE(n_steps == 0 || (dt != nullptr), "dt required");
+  KUPSMI_REQUIRE(skin >= 0.0 && max_neighbors >= 0, "skin and max_neighbors must be >= 0");
+  KUPSMI_REQUIRE(skin == 0.0 || (max_neighbors > 0 && max_neighbors <= 32767),
+                 "a skin needs max_neighbors in [1, 32767]");
   KUPSMI_REQUIRE(integrator != 1 || n_steps == 0 || (kT && gamma), "kT and gamma required for BAOAB");
   MdArgs a{};
   a.integrator = integrator; a.n_steps = n_steps; a.n_sys = n_sys; a.n_species = n_species;
@@ -476,6 +776,7 @@ extern "C" int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n
   a.sys_off = sys_off; a.pos = pos; a.mom = mom; a.forces = forces; a.mass = mass;
   a.species = species; a.sigma = sigma; a.epsilon = epsilon; a.tril = tril; a.cutoff = cutoff;
   a.dt = dt; a.kT = kT; a.gamma = gamma; a.pot = pot_energy; a.kin = kin_energy; a.virial = virial;
+  a.skin = (float)skin; a.nbmax = max_neighbors;
   return md_launch(ctx, (hipStream_t)stream, a);
 }
 

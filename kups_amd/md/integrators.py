@@ -6,7 +6,10 @@ propagators with the reference's step order (273-317: B(dt/2) A(dt) F B(dt/2);
 104-141).  Where the reference composes MomentumStep/PositionStep/
 StochasticStep around a Potential, here one persistent HIP launch
 (``kupsmi_md_lj``) runs any number of steps of every system, each system
-resident in LDS with its cell list rebuilt every step.
+resident in LDS.  The neighbor set of every step is the reference's (every
+pair within the cutoff, rebuilt every step); the kernel finds it through
+Verlet lists of radius cutoff + skin kept in HBM and rebuilt when an atom may
+have moved by skin/2 (``skin=0``: a cell list in LDS every step).
 """
 
 from __future__ import annotations
@@ -17,6 +20,7 @@ import numpy as np
 import torch
 
 from .. import _lib
+from ..core.verlet import neighbor_capacity
 from ..potential.lennard_jones import LennardJonesParameters
 from .data import MDState
 
@@ -33,11 +37,36 @@ class MDObservables:
 
 @dataclass
 class FusedLJPropagator:
-    """n MD steps of every system of an MDState in one launch."""
+    """n MD steps of every system of an MDState in one launch.
+
+    skin: Verlet-list skin (length units of the positions); None picks 0.3 x
+    the largest LJ sigma, 0 rebuilds a cell list every step.  It changes
+    the cost, not the pair set.  max_neighbors: list entries per atom (None:
+    from the densest system); a system whose lists overflow falls back to the
+    per-step cell list."""
 
     params: LennardJonesParameters
     integrator: int = VELOCITY_VERLET
     seed: int = 0
+    skin: float | None = None
+    max_neighbors: int | None = None
+
+    def skin_length(self) -> float:
+        if self.skin is None:
+            return 0.3 * float(np.max(self.params.sigma))
+        if self.skin < 0:
+            raise ValueError("skin must be >= 0")
+        return float(self.skin)
+
+    def _capacity(self, state: MDState, skin: float) -> int:
+        if self.max_neighbors is not None:
+            return int(self.max_neighbors)
+        key = (id(state), state.n_sys, skin)
+        if getattr(self, "_cap_key", None) != key:
+            cut = float(state.cutoff.max().item()) if state.n_sys else 0.0
+            cap = neighbor_capacity(state.sizes, state.tril.cpu().numpy(), cut, skin)
+            self._cap_key, self._cap = key, min(cap, 32767)
+        return self._cap
 
     def _tables(self, dev):
         return (torch.as_tensor(self.params.sigma, dtype=torch.float64, device=dev).contiguous(),
@@ -63,13 +92,15 @@ class FusedLJPropagator:
             kin = torch.empty((state.n_sys, n_rec), dtype=torch.float64, device=dev) if n_steps else None
             vir = torch.empty((state.n_sys, 3, 3), dtype=torch.float64, device=dev)
         kT = state.kT.contiguous()
+        skin = self.skin_length() if n_steps >= 2 else 0.0
+        cap = self._capacity(state, skin) if skin > 0 else 0
         _lib.check(_lib.load().kupsmi_md_lj(
             _lib.ctx(), _lib.stream_ptr(), int(self.integrator), int(n_steps), int(state.step),
             int(self.seed) & (2**64 - 1), state.n_sys, _lib.ptr(state.sys_off), state.n,
             _lib.ptr(state.positions), _lib.ptr(state.momenta), _lib.ptr(state.forces),
             _lib.ptr(state.masses), _lib.ptr(state.species), self.params.n_species,
             _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(state.tril), _lib.ptr(state.cutoff),
-            _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction),
+            skin, cap, _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction),
             _lib.ptr(pot), _lib.ptr(kin), _lib.ptr(vir)))
         state.step += n_steps
         if not record:
@@ -79,14 +110,16 @@ class FusedLJPropagator:
         return MDObservables(pot, kin, vir)
 
 
-def make_velocity_verlet_step(params: LennardJonesParameters, seed: int = 0) -> FusedLJPropagator:
+def make_velocity_verlet_step(params: LennardJonesParameters, seed: int = 0,
+                              skin: float | None = None) -> FusedLJPropagator:
     """NVE velocity Verlet (md/integrators.py:273-317)."""
-    return FusedLJPropagator(params, VELOCITY_VERLET, seed)
+    return FusedLJPropagator(params, VELOCITY_VERLET, seed, skin)
 
 
-def make_baoab_langevin_step(params: LennardJonesParameters, seed: int = 0) -> FusedLJPropagator:
+def make_baoab_langevin_step(params: LennardJonesParameters, seed: int = 0,
+                             skin: float | None = None) -> FusedLJPropagator:
     """NVT BAOAB Langevin (md/integrators.py:405-453); O step noise from Philox4x32-10."""
-    return FusedLJPropagator(params, BAOAB_LANGEVIN, seed)
+    return FusedLJPropagator(params, BAOAB_LANGEVIN, seed, skin)
 
 
 def normals(seed: int, step: int, n_atoms: int, device=None) -> torch.Tensor:

@@ -5,6 +5,8 @@ forces within 1e-5 relative of the FP64 oracle on the same FP32 inputs
 (forces as ||F_gpu - F_ref|| / ||F_ref||, energies per system).
 """
 
+import copy
+
 import numpy as np
 import pytest
 import torch
@@ -14,8 +16,9 @@ from kups_amd import workloads
 from kups_amd.core.batch import ParticleBatch, SystemBatch
 from kups_amd.core.neighborlist import CellListNeighborList
 from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
-from kups_amd.md.integrators import (FusedLJPropagator, make_baoab_langevin_step,
-                                     make_velocity_verlet_step, normals)
+from kups_amd.md.integrators import (VELOCITY_VERLET, FusedLJPropagator,
+                                     make_baoab_langevin_step, make_velocity_verlet_step,
+                                     normals)
 from kups_amd.potential.lennard_jones import LennardJonesParameters, LennardJonesPotential
 
 pytestmark = pytest.mark.gpu
@@ -132,6 +135,31 @@ def test_lj_forces_vs_oracle(gpu, sizes):
                                atol=1e-5 * np.abs(w).max())
 
 
+MIX = LennardJonesParameters.from_dict(
+    workloads.AR_CUTOFF, {"Ar": (workloads.AR_SIGMA, workloads.AR_EPS), "Kr": (3.63, 0.0143),
+                          "Ne": (2.78, 0.00307)})
+
+
+@pytest.mark.parametrize("sizes", [[700, 64, 90], [3000, 1500]])
+def test_lj_forces_three_species_vs_oracle(gpu, sizes):
+    """Mixed species (Lorentz-Berthelot pair tables) through the MD kernel's
+    force evaluation: energies, forces and the virial against the oracle."""
+    b = _lj_batch(sizes, seed=11)
+    species = np.random.default_rng(5).integers(0, 3, b.n).astype(np.int32)
+    pos32 = b.positions.astype(np.float32)
+    system = np.repeat(np.arange(b.n_sys), b.sizes).astype(np.int32)
+    pb = ParticleBatch(torch.as_tensor(pos32, device=gpu), torch.as_tensor(system, device=gpu),
+                       species=torch.as_tensor(species, device=gpu))
+    sb = SystemBatch(torch.as_tensor(b.tril, device=gpu))
+    out = LennardJonesPotential(MIX)(pb, sb)
+    cut = np.full(b.n_sys, workloads.AR_CUTOFF)
+    e, f, w = oracle.lj_cells(_f32(pos32), species, b.sys_off, b.tril, cut, MIX.sigma, MIX.epsilon)
+    _check_energy(out.total_energies.cpu().numpy(), e, b.sizes, 1e-5)
+    _check_forces(-out.position_gradients.cpu().numpy(), f)
+    np.testing.assert_allclose(out.virial.cpu().numpy(), w, rtol=1e-5,
+                               atol=1e-5 * np.abs(w).max())
+
+
 def _md_state(b, seed=0, temperature=94.4, dt_fs=2.0, friction=0.0):
     rng = np.random.default_rng(seed)
     p = maxwell_boltzmann_momenta(b.masses, b.sizes, temperature, rng)
@@ -182,6 +210,84 @@ def test_md_baoab_shared_noise_vs_oracle(gpu):
                               b.tril, cut, AR.sigma, AR.epsilon, dt, kT, gam, eta, frc)
         _check_energy(obs.potential_energy[:, k].cpu().numpy(), e, b.sizes, 1e-5)
     _check_forces(st.momenta.cpu().numpy(), mom, tol=1e-4)
+
+
+def _periodic_diff(d, b):
+    """Position differences reduced to the nearest lattice image."""
+    out = np.empty_like(d, dtype=np.float64)
+    for s in range(b.n_sys):
+        a, z = b.sys_off[s], b.sys_off[s + 1]
+        t = b.tril[s]
+        M = np.array([[t[0], 0, 0], [t[1], t[2], 0], [t[3], t[4], t[5]]])
+        f = d[a:z] @ np.linalg.inv(M)
+        out[a:z] = (f - np.round(f)) @ M
+    return out
+
+
+def _sheared(b, xy=0.12, xz=-0.07, yz=0.09):
+    """The batch in triclinic cells of the same volume: fractional
+    coordinates kept, L10/L20/L21 = xy/xz/yz x the box length."""
+    b2 = copy.deepcopy(b)
+    tril = b.tril.copy()
+    L = tril[:, 0]
+    tril[:, 1], tril[:, 3], tril[:, 4] = xy * L, xz * L, yz * L
+    for s in range(b.n_sys):
+        a, z = b.sys_off[s], b.sys_off[s + 1]
+        f = b.positions[a:z] / L[s]
+        M = np.array([[tril[s, 0], 0, 0], [tril[s, 1], tril[s, 2], 0],
+                      [tril[s, 3], tril[s, 4], tril[s, 5]]])
+        b2.positions[a:z] = f @ M
+    b2.tril = tril
+    return b2
+
+
+@pytest.mark.parametrize("integ", ["vv", "baoab"])
+@pytest.mark.parametrize("shape", ["cubic", "triclinic"])
+def test_md_verlet_skin_matches_cell_list_every_step(gpu, integ, shape):
+    """Verlet lists (skin > 0, rebuilt on the skin/2 displacement trigger)
+    give the pair set of a cell list rebuilt every step: the trajectories
+    agree to FP32 summation order, and the energies of every step match."""
+    # 300 atoms: 2 rc <= box < 2 (rc + 1.2 sigma), the lists are built on an
+    # image grid; 80 atoms: box < 2 rc, no lists
+    b = _lj_batch([1500, 864, 300, 80], seed=12)
+    if shape == "triclinic":
+        b = _sheared(b)
+    make = make_velocity_verlet_step if integ == "vv" else make_baoab_langevin_step
+    runs = []
+    for skin in (0.0, 0.3 * workloads.AR_SIGMA, 1.2 * workloads.AR_SIGMA):
+        st = _md_state(b, friction=0.01 if integ == "baoab" else 0.0, temperature=300.0)
+        prop = make(AR, seed=7, skin=skin)
+        prop.evaluate_forces(st)
+        obs = prop(st, 60)
+        runs.append((st.positions.cpu().numpy(), st.momenta.cpu().numpy(),
+                     obs.potential_energy.cpu().numpy(), obs.virial.cpu().numpy()))
+    for pos, mom, e, w in runs[1:]:
+        assert np.abs(_periodic_diff(pos - runs[0][0], b)).max() < 2e-4
+        _check_forces(mom, runs[0][1], tol=1e-4)
+        for k in range(e.shape[1]):
+            _check_energy(e[:, k], runs[0][2][:, k], b.sizes, 2e-5)
+        np.testing.assert_allclose(w, runs[0][3], rtol=1e-4, atol=1e-4 * np.abs(w).max())
+    # final state against the oracle
+    pos, _, e, _ = runs[1]
+    cut = np.full(b.n_sys, workloads.AR_CUTOFF)
+    eo, _, _ = oracle.lj_cells(_f32(pos), b.species, b.sys_off, b.tril, cut, AR.sigma, AR.epsilon)
+    _check_energy(e[:, -1], eo, b.sizes, 1e-5)
+
+
+def test_md_verlet_overflow_falls_back_to_cell_list(gpu):
+    """Lists far too short for the neighbourhood: the system switches to the
+    per-step cell list and the run matches skin = 0."""
+    b = _lj_batch([864, 500], seed=13)
+    runs = []
+    for skin, cap in ((0.0, None), (1.0, 8)):
+        st = _md_state(b)
+        prop = FusedLJPropagator(AR, VELOCITY_VERLET, 0, skin, cap)
+        prop.evaluate_forces(st)
+        obs = prop(st, 20)
+        runs.append((st.positions.cpu().numpy(), obs.potential_energy.cpu().numpy()))
+    assert np.abs(_periodic_diff(runs[1][0] - runs[0][0], b)).max() < 1e-4
+    for k in range(runs[0][1].shape[1]):
+        _check_energy(runs[1][1][:, k], runs[0][1][:, k], b.sizes, 1e-5)
 
 
 def test_normals_statistics(gpu):
