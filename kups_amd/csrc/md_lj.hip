@@ -339,7 +339,7 @@ __device__ inline int md_build_list(const SysGeom& g, const float4* sp, const in
 // on its per-axis nearest image (the only image within the cutoff when every
 // perpendicular width is at least 2 rc).  Four entries per trip, their list
 // words and positions loaded before any is used.
-template <bool MULTI, bool VIR>
+template <bool MULTI, bool VIR, bool ORTHO>
 __device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const unsigned short* nl,
                                       int cnt, int slot, const float4* s_par, int ns,
                                       float rc2_lo, float rc2_hi, float& ax, float& ay, float& az,
@@ -357,11 +357,17 @@ __device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       float ex = rj[u].x - ri.x, ey = rj[u].y - ri.y, ez = rj[u].z - ri.z;
-      float f0, f1, f2;
-      tmul(g.Lif, ex, ey, ez, f0, f1, f2);
-      float sx, sy, sz;
-      tmul(g.Lf, rintf(f0), rintf(f1), rintf(f2), sx, sy, sz);
-      ex -= sx; ey -= sy; ez -= sz;
+      if (ORTHO) {
+        ex -= rintf(ex * g.Lif[0]) * g.Lf[0];
+        ey -= rintf(ey * g.Lif[2]) * g.Lf[2];
+        ez -= rintf(ez * g.Lif[5]) * g.Lf[5];
+      } else {
+        float f0, f1, f2;
+        tmul(g.Lif, ex, ey, ez, f0, f1, f2);
+        float sx, sy, sz;
+        tmul(g.Lf, rintf(f0), rintf(f1), rintf(f2), sx, sy, sz);
+        ex -= sx; ey -= sy; ez -= sz;
+      }
       const float r2 = ex * ex + ey * ey + ez * ez;
       const bool valid = k + u < cnt;
       const bool lo = r2 < rc2_lo;
@@ -375,7 +381,7 @@ __device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const 
 
 // Forces, closing B(dt/2) (+ the next step's opening B(dt/2)) and kinetic
 // energy of every slot of the thread; the forces of the last step go to HBM.
-template <bool MULTI, bool VIR>
+template <bool MULTI, bool VIR, bool ORTHO>
 __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const float4* sp,
                                       float4* pb, const int* cstart, const short* nbcount,
                                       const unsigned short* nbl, bool use_list,
@@ -389,8 +395,8 @@ __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const f
     if (slot >= n) continue;
     float ax = 0.f, ay = 0.f, az = 0.f, ei = 0.f;
     if (use_list)
-      md_list_forces<MULTI, VIR>(g, sp, nbl + slot, nbcount[slot], slot, s_par, ns, rc2_lo, rc2_hi,
-                                 ax, ay, az, ei, w);
+      md_list_forces<MULTI, VIR, ORTHO>(g, sp, nbl + slot, nbcount[slot], slot, s_par, ns, rc2_lo,
+                                        rc2_hi, ax, ay, az, ei, w);
     else
       md_stencil_forces<MULTI, VIR>(g, sp, cstart, s_par, ns, slot, rc2_lo, rc2_hi, ax, ay, az, ei,
                                     w);
@@ -645,21 +651,17 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
       float epot = 0.0f, ekin = 0.0f;
       float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       const bool do_w = last && a.want_virial;
+#define MD_PHASE(M, V, O)                                                                     \
+  md_force_phase<M, V, O>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0, rc2_lo, \
+                          rc2_hi, hdt, dtf, last, epot, ekin, w)
       if (ns == 1) {
-        if (do_w)
-          md_force_phase<false, true>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
-                                      rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
-        else
-          md_force_phase<false, false>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
-                                       rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
+        if (do_w) { if (ortho) MD_PHASE(false, true, true); else MD_PHASE(false, true, false); }
+        else { if (ortho) MD_PHASE(false, false, true); else MD_PHASE(false, false, false); }
       } else {
-        if (do_w)
-          md_force_phase<true, true>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
-                                     rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
-        else
-          md_force_phase<true, false>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
-                                      rc2_lo, rc2_hi, hdt, dtf, last, epot, ekin, w);
+        if (do_w) { if (ortho) MD_PHASE(true, true, true); else MD_PHASE(true, true, false); }
+        else { if (ortho) MD_PHASE(true, false, true); else MD_PHASE(true, false, false); }
       }
+#undef MD_PHASE
       if (a.pot) {
         double e = block_sum((double)epot, red);
         if (tid == 0) a.pot[(size_t)s * total + st] = e;
