@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-every", type=int, default=8,
                     help="CPU baseline runs every k-th system of the batch")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="1 GiB device copy (counter calibration) + one MD launch, no JSON line")
     ap.add_argument("--no-ewald", action="store_true")
@@ -78,7 +78,7 @@ def parse():
     ap.add_argument("--gcmc-steps", type=int, default=20000, help="timed trials per chain and proposal")
     ap.add_argument("--gcmc-chunk", type=int, default=500, help="trials per launch (loading samples)")
     ap.add_argument("--gcmc-cpu-chains", type=int, default=64)
-    ap.add_argument("--gcmc-cpu-steps", type=int, default=1000)
+    ap.add_argument("--gcmc-cpu-steps", type=int, default=25000)
     return ap.parse_args()
 
 
@@ -433,7 +433,8 @@ def main():
         "config": {
             "workload": "batched LJ MD: 2048 cubic PBC boxes, N~U{64..4096} (seed 0), "
                         "rho*=0.8, jitter 0.05 sigma, argon LJ rc=2.5 sigma, NVE velocity "
-                        "Verlet dt=2 fs, neighbor set rebuilt every step",
+                        "Verlet dt=2 fs, the exact cutoff pair set every step (Verlet lists, "
+                        "skin 0.2 sigma, rebuilt on the skin/2 displacement bound)",
             "n_systems": batch.n_sys,
             "n_atoms": atoms,
             "steps_per_launch": spl,
@@ -448,6 +449,7 @@ def main():
             "kernel": "md_lj_kernel",
             "note": "FP32 FLOP ceiling (157.3 TF/s is the FP32 peak of both the vector "
                     "ALUs and the matrix cores; this kernel is VALU work, state LDS-resident, "
+                    "Verlet lists in HBM; "
                     f"{FLOP_PER_ATOM_STEP:.0f} algorithmic FLOP per atom-step)",
             "kernel_ms_per_launch": kern_ms,
         },

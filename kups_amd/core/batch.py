@@ -89,6 +89,20 @@ def system_offsets(system: torch.Tensor, n_sys: int) -> tuple[torch.Tensor, np.n
     return torch.as_tensor(off.astype(np.int32), device=system.device), sizes
 
 
+def segment_sum(values: torch.Tensor, system: torch.Tensor, n_sys: int) -> torch.Tensor:
+    """Per-system FP64 sums of ``values`` (n, ...) over particles sorted by
+    system: differences of one running sum at the system boundaries, found by
+    binary search (no atomics, which serialise when millions of particles add
+    into a few thousand systems)."""
+    dev = values.device
+    bounds = torch.arange(n_sys + 1, dtype=system.dtype, device=dev)
+    off = torch.searchsorted(system.contiguous(), bounds)
+    c = torch.zeros((values.shape[0] + 1,) + tuple(values.shape[1:]), dtype=torch.float64,
+                    device=dev)
+    torch.cumsum(values.double(), 0, out=c[1:])
+    return c[off[1:]] - c[off[:-1]]
+
+
 def offsets_from_sizes(sizes, device) -> torch.Tensor:
     sizes = np.asarray(sizes, np.int64)
     off = np.zeros(len(sizes) + 1, np.int64)

@@ -11,6 +11,8 @@ from __future__ import annotations
 
 import torch
 
+from ..core.batch import segment_sum
+
 
 def particle_kinetic_energy(momenta: torch.Tensor, masses: torch.Tensor) -> torch.Tensor:
     """K_i = |p_i|^2 / (2 m_i) (md/observables.py:17-34)."""
@@ -20,8 +22,7 @@ def particle_kinetic_energy(momenta: torch.Tensor, masses: torch.Tensor) -> torc
 def kinetic_energy(momenta: torch.Tensor, masses: torch.Tensor, system: torch.Tensor,
                    n_sys: int) -> torch.Tensor:
     """Per-system kinetic energy (FP64)."""
-    out = torch.zeros(n_sys, dtype=torch.float64, device=momenta.device)
-    return out.index_add_(0, system.long(), particle_kinetic_energy(momenta, masses))
+    return segment_sum(particle_kinetic_energy(momenta, masses), system, n_sys)
 
 
 def volume(tril: torch.Tensor) -> torch.Tensor:

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ..core.batch import ParticleBatch, SystemBatch, system_offsets
+from ..core.batch import ParticleBatch, SystemBatch, segment_sum, system_offsets
 from ..core.constants import TO_STANDARD_UNITS
 
 
@@ -160,8 +160,7 @@ class EwaldOut:
 
 def self_energy(q: torch.Tensor, system: torch.Tensor, alpha: torch.Tensor, n_sys: int):
     """-K alpha/sqrt(pi) sum q^2 per system (ewald.py:277-302), FP64."""
-    q2 = torch.zeros(n_sys, dtype=torch.float64, device=q.device)
-    q2.index_add_(0, system.long(), q.double() ** 2)
+    q2 = segment_sum(q.double() ** 2, system, n_sys)
     return -TO_STANDARD_UNITS * alpha * q2 / math.sqrt(math.pi)
 
 
