@@ -275,19 +275,34 @@ __device__ inline void md_stencil_forces(const SysGeom& g, const float4* sp, con
   }
 }
 
+// List entries per atom as allocated: the capacity rounded up to whole
+// groups of four.
+__host__ __device__ inline int md_list_cap(int nbmax) { return (nbmax + 3) & ~3; }
+
+// Word index of list entry e of slot s: entries are grouped by four, the four
+// entries of a group contiguous (one 8-byte load), groups entry-major over
+// the slots (a wave's loads of one group are contiguous).
+__device__ inline size_t md_list_word(int e, int s) {
+  return ((size_t)(e >> 2) * MD_NMAX + s) * 4 + (e & 3);
+}
+
 // Verlet list of the atom in sorted slot `slot`: every atom j with an image
-// within the list radius, once.  The stencil may visit j under several image
-// shifts; only the visit whose shift is the per-axis nearest image,
+// within the list radius, once.  When every axis has >= 5 cells, the +-2-cell
+// stencil visits each (j, image) once and distinct images of j cannot both be
+// within rc + skin <= perp / 2, so the stencil image is tested directly
+// (DEDUP = false).  Otherwise the stencil may visit j under several images;
+// only the visit whose shift is the per-axis nearest image,
 // s = -rint((r_j - r_i) @ L^-1) (one FP32 evaluation per pair, the same for
 // every visit), is kept, and it is kept when that image lies within the list
 // radius (orthorhombic cells, where it is the nearest image) or within the
 // slab |f - s| < (rc + skin) / perp on every axis (triclinic cells, a
-// superset of the sphere).  An atom within the cutoff at a later step (every
-// atom moved by less than skin/2) is therefore in the list.  Entries are
-// sorted slots, entry-major (nl[k * MD_NMAX]).  Returns the count, which may
-// exceed nbmax (entries past nbmax are not written).
+// superset of the sphere).  Either way an atom within the cutoff at a later
+// step (every atom moved by less than skin/2) is in the list.  The last group
+// is padded with the atom's own slot (masked off by the count).  Returns the
+// count, which may exceed cap (entries past cap are not written).
+template <bool DEDUP>
 __device__ inline int md_build_list(const SysGeom& g, const float4* sp, const int* cstart, int slot,
-                                    unsigned short* nl, int nbmax, float list_r2, bool ortho,
+                                    unsigned short* nbl, int cap, float list_r2, bool ortho,
                                     float fx_max, float fy_max, float fz_max) {
   const float4 ri = sp[slot];
   const int c = (__float_as_int(ri.w) >> 8) & 0xfff;
@@ -317,21 +332,28 @@ __device__ inline int md_build_list(const SysGeom& g, const float4* sp, const in
           const float4 rj = sp[t];
           const float ex = rj.x - qx, ey = rj.y - qy, ez = rj.z - qz;
           const float r2 = ex * ex + ey * ey + ez * ez;
-          float f0, f1, f2;
-          tmul(g.Lif, rj.x - ri.x, rj.y - ri.y, rj.z - ri.z, f0, f1, f2);
-          const float n0 = rintf(f0), n1 = rintf(f1), n2 = rintf(f2);
-          const bool nearest = (int)n0 == -sx && (int)n1 == -sy && (int)n2 == -sz;
-          const bool near = ortho ? r2 < list_r2
-                                  : (fabsf(f0 - n0) < fx_max && fabsf(f1 - n1) < fy_max &&
-                                     fabsf(f2 - n2) < fz_max);
-          if (nearest && near) {
-            if (cnt < nbmax) nl[(size_t)cnt * MD_NMAX] = (unsigned short)t;
+          bool take;
+          if (DEDUP) {
+            float f0, f1, f2;
+            tmul(g.Lif, rj.x - ri.x, rj.y - ri.y, rj.z - ri.z, f0, f1, f2);
+            const float n0 = rintf(f0), n1 = rintf(f1), n2 = rintf(f2);
+            const bool nearest = (int)n0 == -sx && (int)n1 == -sy && (int)n2 == -sz;
+            const bool near = ortho ? r2 < list_r2
+                                    : (fabsf(f0 - n0) < fx_max && fabsf(f1 - n1) < fy_max &&
+                                       fabsf(f2 - n2) < fz_max);
+            take = nearest && near;
+          } else {
+            take = r2 < list_r2;
+          }
+          if (take) {
+            if (cnt < cap) nbl[md_list_word(cnt, slot)] = (unsigned short)t;
             ++cnt;
           }
         }
       }
     }
   }
+  for (int e = cnt; e < min((cnt + 3) & ~3, cap); ++e) nbl[md_list_word(e, slot)] = (unsigned short)slot;
   return cnt;
 }
 
@@ -340,17 +362,22 @@ __device__ inline int md_build_list(const SysGeom& g, const float4* sp, const in
 // perpendicular width is at least 2 rc).  Four entries per trip, their list
 // words and positions loaded before any is used.
 template <bool MULTI, bool VIR, bool ORTHO>
-__device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const unsigned short* nl,
+__device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const unsigned short* nbl,
                                       int cnt, int slot, const float4* s_par, int ns,
                                       float rc2_lo, float rc2_hi, float& ax, float& ay, float& az,
                                       float& ei, float (&w)[6]) {
   const float4 ri = sp[slot];
   const int si = __float_as_int(ri.w) & 0xff;
   const float4 p0 = s_par[0];
-  for (int k = 0; k < cnt; k += 4) {
-    int j[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) j[u] = nl[(size_t)min(k + u, cnt - 1) * MD_NMAX];
+  // group q of this slot: one 8-byte word of four entries; the next group's
+  // word is loaded one trip ahead
+  const uint2* nl = reinterpret_cast<const uint2*>(nbl) + slot;
+  const int ng = (cnt + 3) >> 2;
+  uint2 cur = ng > 0 ? nl[0] : make_uint2(0u, 0u);
+  for (int q = 0; q < ng; ++q) {
+    const uint2 nxt = nl[(size_t)min(q + 1, ng - 1) * MD_NMAX];
+    const int j[4] = {(int)(cur.x & 0xffffu), (int)(cur.x >> 16), (int)(cur.y & 0xffffu),
+                      (int)(cur.y >> 16)};
     float4 rj[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) rj[u] = sp[j[u]];
@@ -369,13 +396,14 @@ __device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const 
         ex -= sx; ey -= sy; ez -= sz;
       }
       const float r2 = ex * ex + ey * ey + ez * ez;
-      const bool valid = k + u < cnt;
+      const bool valid = 4 * q + u < cnt;
       const bool lo = r2 < rc2_lo;
       bool keep = valid && lo;
       if (__builtin_expect(valid && !lo && r2 < rc2_hi, 0)) keep = md_band_nearest(g, ri, rj[u]);
       const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj[u].w) & 0xff)] : p0;
       md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
     }
+    cur = nxt;
   }
 }
 
@@ -395,8 +423,8 @@ __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const f
     if (slot >= n) continue;
     float ax = 0.f, ay = 0.f, az = 0.f, ei = 0.f;
     if (use_list)
-      md_list_forces<MULTI, VIR, ORTHO>(g, sp, nbl + slot, nbcount[slot], slot, s_par, ns, rc2_lo,
-                                        rc2_hi, ax, ay, az, ei, w);
+      md_list_forces<MULTI, VIR, ORTHO>(g, sp, nbl, nbcount[slot], slot, s_par, ns, rc2_lo, rc2_hi,
+                                        ax, ay, az, ei, w);
     else
       md_stencil_forces<MULTI, VIR>(g, sp, cstart, s_par, ns, slot, rc2_lo, rc2_hi, ax, ay, az, ei,
                                     w);
@@ -466,7 +494,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
   const int tid = threadIdx.x;
   const int ns = a.n_species;
   unsigned short* nbl =
-      a.nblist ? a.nblist + (size_t)blockIdx.x * (size_t)a.nbmax * MD_NMAX : nullptr;
+      a.nblist ? a.nblist + (size_t)blockIdx.x * (size_t)md_list_cap(a.nbmax) * MD_NMAX : nullptr;
   for (int k = tid; k < ns * ns; k += MD_THREADS) {
     double sg = a.sigma[k];
     s_par[k] = make_float4((float)(sg * sg), (float)(4.0 * a.epsilon[k]),
@@ -526,6 +554,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
     const double lr = g.rc + (double)a.skin;
     const float list_r2 = (float)(lr * lr) * (1.0f + 1e-5f);
     const bool ortho = g.L[1] == 0.0 && g.L[3] == 0.0 && g.L[4] == 0.0;
+    const bool dedup = g.nc[0] < 5 || g.nc[1] < 5 || g.nc[2] < 5;
     // triclinic list slab: |f| < (rc + skin) / perp per axis, perp = 1 / |row of L^-T|
     const float fx_max = (float)(lr * sqrt(g.Li[0] * g.Li[0] + g.Li[1] * g.Li[1] + g.Li[3] * g.Li[3])) * 1.00001f;
     const float fy_max = (float)(lr * sqrt(g.Li[2] * g.Li[2] + g.Li[4] * g.Li[4])) * 1.00001f;
@@ -638,10 +667,13 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
             const int slot = tid + k * MD_THREADS;
             dacc[k] = 0.0f;
             if (slot >= n) continue;
-            const int cnt = md_build_list(g, sp, cstart, slot, nbl + slot, a.nbmax, list_r2, ortho,
-                                          fx_max, fy_max, fz_max);
-            keys[slot] = (short)min(cnt, a.nbmax);
-            over |= cnt > a.nbmax;
+            const int cap = md_list_cap(a.nbmax);
+            const int cnt = dedup ? md_build_list<true>(g, sp, cstart, slot, nbl, cap, list_r2, ortho,
+                                                        fx_max, fy_max, fz_max)
+                                  : md_build_list<false>(g, sp, cstart, slot, nbl, cap, list_r2,
+                                                         ortho, fx_max, fy_max, fz_max);
+            keys[slot] = (short)min(cnt, cap);
+            over |= cnt > cap;
           }
           use_list = !__syncthreads_or(over);
           listed = use_list;
@@ -744,7 +776,8 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
   if (a.skin > 0.0f && a.nbmax > 0 && a.n_steps >= 2) {
     // one list region (nbmax entries x MD_NMAX slots) per resident workgroup
     void* lb = nullptr;
-    rc = scratch(ctx, 6, sizeof(unsigned short) * (size_t)grid * a.nbmax * MD_NMAX, &lb);
+    rc = scratch(ctx, 6, sizeof(unsigned short) * (size_t)grid * md_list_cap(a.nbmax) * MD_NMAX,
+                 &lb);
     if (rc) return rc;
     a.nblist = static_cast<unsigned short*>(lb);
   }
