@@ -434,7 +434,7 @@ def main():
             "workload": "batched LJ MD: 2048 cubic PBC boxes, N~U{64..4096} (seed 0), "
                         "rho*=0.8, jitter 0.05 sigma, argon LJ rc=2.5 sigma, NVE velocity "
                         "Verlet dt=2 fs, the exact cutoff pair set every step (Verlet lists, "
-                        "skin 0.2 sigma, rebuilt on the skin/2 displacement bound)",
+                        "skin 0.3 sigma, rebuilt on the skin/2 displacement bound)",
             "n_systems": batch.n_sys,
             "n_atoms": atoms,
             "steps_per_launch": spl,

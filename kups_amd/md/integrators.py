@@ -39,7 +39,7 @@ class MDObservables:
 class FusedLJPropagator:
     """n MD steps of every system of an MDState in one launch.
 
-    skin: Verlet-list skin (length units of the positions); None picks 0.2 x
+    skin: Verlet-list skin (length units of the positions); None picks 0.3 x
     the largest LJ sigma, 0 rebuilds a cell list every step.  It changes
     the cost, not the pair set.  max_neighbors: list entries per atom (None:
     from the densest system); a system whose lists overflow falls back to the
@@ -53,7 +53,7 @@ class FusedLJPropagator:
 
     def skin_length(self) -> float:
         if self.skin is None:
-            return 0.2 * float(np.max(self.params.sigma))
+            return 0.3 * float(np.max(self.params.sigma))
         if self.skin < 0:
             raise ValueError("skin must be >= 0")
         return float(self.skin)
