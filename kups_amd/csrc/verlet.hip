@@ -225,7 +225,89 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
   if (cnt > a.cap) atomicAdd(a.counters + 1, 1ull);
 }
 
+// erfc(x), x >= 0, by the Chebyshev fit of Press, Teukolsky, Vetterling and
+// Flannery, Numerical Recipes in C, 2nd ed. (1992), section 6.2 (erfcc):
+// fractional error below 1.2e-7 in exact arithmetic.  In FP32 the rounding of
+// the exponent (about -13 at x = 3.6, the usual alpha rc) adds ~1e-6
+// relative where erfc itself is ~4e-7, i.e. ~1e-12 eV per pair.
+__device__ inline float vl_erfc(float x) {
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * x);
+  float p = 0.17087277f;
+  p = fmaf(p, t, -0.82215223f);
+  p = fmaf(p, t, 1.48851587f);
+  p = fmaf(p, t, -1.13520398f);
+  p = fmaf(p, t, 0.27886807f);
+  p = fmaf(p, t, -0.18628806f);
+  p = fmaf(p, t, 0.09678418f);
+  p = fmaf(p, t, 0.37409196f);
+  p = fmaf(p, t, 1.00002368f);
+  p = fmaf(p, t, -1.26551223f);
+  return t * __expf(p - x * x);
+}
+
+// FP64 re-decision of a pair within VL_BAND32 of a cutoff (the reference's
+// fractional formula).  Out of line: the rare call claims no registers in
+// the list loop.
+__device__ __attribute__((noinline)) bool vl_keep64(const SysGeom& g, float4 ri, float4 rj, int sx,
+                                                    int sy, int sz, double rc2) {
+  return keep64(g, ri, rj, sx, sy, sz, rc2);
+}
+
+__device__ __attribute__((noinline)) bool vl_min_image(const SysGeom& g, float4 ri, float4 rj,
+                                                       int sx, int sy, int sz) {
+  return is_min_image(g, ri, rj, sx, sy, sz);
+}
+
+// relative half width of the FP32 cutoff band re-decided in FP64: the FP32
+// pair vector of positions within 1.5 box lengths of the origin is good to
+// ~4e-6 A per component, i.e. ~2e-4 A^2 in r^2 at rc = 12 A, well inside
+// 4e-6 rc^2 = 6e-4 A^2
+constexpr float VL_BAND32 = 4e-6f;
+
+// LJ and real-space Ewald terms of one list entry, d = r_j + s@L - r_i in
+// FP32.  Each cutoff test is folded into its 1/r factor (zero for a rejected
+// pair), so the loop has no data-dependent branch but the rare band case;
+// exclusion ids compare on the minimum image only (core/neighborlist.py:
+// 731-735).  Adds the pair energies to elj / ecl and returns u'(r) / r.
 template <bool EXCL>
+__device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)[4],
+                                       const SysGeom& g, float4 ri, float4 rj, int sx, int sy,
+                                       int sz, float r2, int si, int sj, int xlj_i, int xlj_j,
+                                       int xcl_i, int xcl_j, float& elj, float& ecl) {
+  const float K = 14.399645351950548f;  // HARTREE * BOHR, eV A
+  float gsum = 0.f;
+  bool min_img = true;
+  if (EXCL && g.rep && ((pp.has_lj_excl && xlj_j == xlj_i) || (pp.has_coul_excl && xcl_j == xcl_i)))
+    min_img = vl_min_image(g, ri, rj, sx, sy, sz);
+  if (pp.has_lj) {
+    const bool lo = r2 < band[0];
+    bool keep = lo;
+    if (__builtin_expect(!lo && r2 < band[1], 0)) keep = vl_keep64(g, ri, rj, sx, sy, sz, pp.lj_rc2);
+    if (EXCL && pp.has_lj_excl && min_img && xlj_j == xlj_i) keep = false;
+    const int p = si * pp.ns + sj;
+    const float ir2 = keep ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+    float c6 = pp.sig2[p] * ir2;
+    c6 = c6 * c6 * c6;
+    elj += pp.e4[p] * c6 * (c6 - 1.0f);
+    gsum += pp.e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;
+  }
+  if (pp.has_coul) {
+    const bool lo = r2 < band[2];
+    bool keep = lo;
+    if (__builtin_expect(!lo && r2 < band[3], 0)) keep = vl_keep64(g, ri, rj, sx, sy, sz, pp.cl_rc2);
+    if (EXCL && pp.has_coul_excl && min_img && xcl_j == xcl_i) keep = false;
+    const float ir = keep ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    const float x = pp.alpha * (r2 * ir);
+    const float ec = vl_erfc(x);
+    const float qq = ri.w * rj.w * K;
+    ecl += qq * ec * ir;
+    // dU/dr = qq (-2a/sqrt(pi) e^{-a^2 r^2} - erfc / r) / r; u'/r = dU/dr / r
+    gsum += qq * (-pp.two_a_sqrtpi * __expf(-x * x) - ec * ir) * ir * ir;
+  }
+  return gsum;
+}
+
+template <bool EXCL, bool VIR>
 __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   __shared__ SysGeom g;
   __shared__ float s_sig2[PC_NSPEC * PC_NSPEC], s_e4[PC_NSPEC * PC_NSPEC],
@@ -254,6 +336,8 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   PairParams pp;
   pc_pair_params(pp, a.has_lj, a.has_coul, a.has_lj_excl, a.has_coul_excl, a.lj_rc, a.cl_rc,
                  a.alpha, s, ns, s_sig2, s_e4, s_e24);
+  const float band[4] = {(float)pp.lj_rc2 * (1.0f - VL_BAND32), (float)pp.lj_rc2 * (1.0f + VL_BAND32),
+                         (float)pp.cl_rc2 * (1.0f - VL_BAND32), (float)pp.cl_rc2 * (1.0f + VL_BAND32)};
   const int slot = start + tid;
   float ax = 0.f, ay = 0.f, az = 0.f, elj = 0.f, ecl = 0.f;
   float w00 = 0.f, w01 = 0.f, w02 = 0.f, w11 = 0.f, w12 = 0.f, w22 = 0.f;
@@ -263,44 +347,86 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
     const int si = a.sspec[d];
     const int xlj = EXCL && a.has_lj_excl ? a.slj[d] : 0;
     const int xcl = EXCL && a.has_coul_excl ? a.scl[d] : 0;
-    auto visit = [&](int t, float4 rj, int sj, int sx, int sy, int sz, bool self) {
-      // FP64 pair vector d = r_j - r_i + s @ L
-      double Sx, Sy, Sz;
-      tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
-      const double exd = (double)rj.x - ((double)ri.x - Sx);
-      const double eyd = (double)rj.y - ((double)ri.y - Sy);
-      const double ezd = (double)rj.z - ((double)ri.z - Sz);
-      const double r2d = exd * exd + eyd * eyd + ezd * ezd;
-      if (!(r2d < pp.all_hi)) return;
-      const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
-      const float gsum = pc_pair_terms<EXCL>(
-          pp, g, ri, rj, sx, sy, sz, r2d, r2, self, si, sj, xlj,
-          EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
-          EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
+    auto accumulate = [&](float gsum, float ex, float ey, float ez) {
       ax += gsum * ex; ay += gsum * ey; az += gsum * ez;
-      const float hg = -0.5f * gsum;
-      w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
-      w11 += hg * ey * ey; w12 += hg * ey * ez; w22 += hg * ez * ez;
+      if (VIR) {
+        const float hg = -0.5f * gsum;
+        w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
+        w11 += hg * ey * ey; w12 += hg * ey * ez; w22 += hg * ez * ez;
+      }
     };
     const int cnt = a.nlist[d];
     if (cnt <= a.cap) {
-      // software pipeline: entry k + 2 and position k + 1 are in flight while
-      // entry k is evaluated
+      // four entries per trip; while group q is evaluated, the positions of
+      // group q + 1 and the list words of group q + 2 are in flight (indices
+      // past the end are clamped to the last entry and masked off)
       const uint32_t* lp = a.list + d;
       const size_t stride = (size_t)a.n_atoms;
-      uint32_t e1 = cnt > 0 ? lp[0] : 0u, e2 = cnt > 1 ? lp[stride] : 0u;
-      float4 r1 = a.spos[i0 + (int)(e1 & 0x7fffu)];
-      for (int k = 0; k < cnt; ++k) {
-        const uint32_t e = e1;
-        const float4 rj = r1;
-        e1 = e2;
-        r1 = a.spos[i0 + (int)(e1 & 0x7fffu)];
-        e2 = k + 2 < cnt ? lp[(size_t)(k + 2) * stride] : 0u;
-        visit((int)(e & 0x7fffu), rj, (int)(e >> 27), (int)((e >> 15) & 15u) - 8,
-              (int)((e >> 19) & 15u) - 8, (int)((e >> 23) & 15u) - 8, false);
+      const int last = cnt - 1;
+      auto words = [&](int k0, uint32_t (&e)[4]) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = lp[(size_t)min(k0 + u, last) * stride];
+      };
+      auto positions = [&](const uint32_t (&e)[4], float4 (&r)[4]) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) r[u] = a.spos[i0 + (int)(e[u] & 0x7fffu)];
+      };
+      uint32_t ea[4], eb[4];
+      float4 ra[4];
+      if (cnt > 0) {
+        words(0, ea);
+        words(4, eb);
+        positions(ea, ra);
+      }
+      for (int k0 = 0; k0 < cnt; k0 += 4) {
+        float4 rb[4];
+        uint32_t ec[4];
+        positions(eb, rb);
+        words(k0 + 8, ec);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t e = ea[u];
+          const float4 rj = ra[u];
+          const int t = (int)(e & 0x7fffu);
+          const int sx = (int)((e >> 15) & 15u) - 8, sy = (int)((e >> 19) & 15u) - 8,
+                    sz = (int)((e >> 23) & 15u) - 8;
+          float Sx, Sy, Sz;
+          tmul(g.Lf, (float)sx, (float)sy, (float)sz, Sx, Sy, Sz);
+          const float ex = (rj.x + Sx) - ri.x, ey = (rj.y + Sy) - ri.y, ez = (rj.z + Sz) - ri.z;
+          // a masked-off entry gets a huge finite r^2 (above every cutoff
+          // band; times the zero 1/r of a rejected pair it stays 0)
+          const float r2 = k0 + u < cnt ? ex * ex + ey * ey + ez * ez : 3.0e38f;
+          const float gsum = vl_entry_terms<EXCL>(
+              pp, band, g, ri, rj, sx, sy, sz, r2, si, (int)(e >> 27),
+              xlj, EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
+              EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
+          accumulate(gsum, ex, ey, ez);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          ea[u] = eb[u];
+          eb[u] = ec[u];
+          ra[u] = rb[u];
+        }
       }
     } else {
       // overflowed list: the build's 125-cell stencil (cells sized for rc + skin)
+      auto visit = [&](int t, float4 rj, int sj, int sx, int sy, int sz, bool self) {
+        // FP64 pair vector d = r_j - r_i + s @ L
+        double Sx, Sy, Sz;
+        tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
+        const double exd = (double)rj.x - ((double)ri.x - Sx);
+        const double eyd = (double)rj.y - ((double)ri.y - Sy);
+        const double ezd = (double)rj.z - ((double)ri.z - Sz);
+        const double r2d = exd * exd + eyd * eyd + ezd * ezd;
+        if (!(r2d < pp.all_hi)) return;
+        const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
+        const float gsum = pc_pair_terms<EXCL>(
+            pp, g, ri, rj, sx, sy, sz, r2d, r2, self, si, sj, xlj,
+            EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
+            EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
+        accumulate(gsum, ex, ey, ez);
+      };
       const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
       const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
       const int c = a.scell[d];
@@ -500,10 +626,14 @@ extern "C" int kupsmi_pair_forces_verlet(
   KUPSMI_CHECK_LAUNCH();
   vl_list_kernel<<<grid, VL_TILE, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
-  if (a.has_lj_excl || a.has_coul_excl)
-    vl_pair_kernel<true><<<grid, VL_TILE, 0, st>>>(a);
-  else
-    vl_pair_kernel<false><<<grid, VL_TILE, 0, st>>>(a);
+  const bool excl = a.has_lj_excl || a.has_coul_excl, vir = a.virial != nullptr;
+  if (excl) {
+    if (vir) vl_pair_kernel<true, true><<<grid, VL_TILE, 0, st>>>(a);
+    else vl_pair_kernel<true, false><<<grid, VL_TILE, 0, st>>>(a);
+  } else {
+    if (vir) vl_pair_kernel<false, true><<<grid, VL_TILE, 0, st>>>(a);
+    else vl_pair_kernel<false, false><<<grid, VL_TILE, 0, st>>>(a);
+  }
   KUPSMI_CHECK_LAUNCH();
   vl_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
