@@ -118,6 +118,11 @@ int kupsmi_lj_forces(kupsmi_ctx* ctx, void* stream, int n_sys,
  * make_md_step_from_state (md/integrators.py:904-962):
  *   integrator 0: velocity Verlet  B(dt/2) A(dt) F B(dt/2)      (273-317)
  *   integrator 1: BAOAB Langevin   B(dt/2) A(dt/2) O A(dt/2) F B(dt/2) (405-453)
+ *   integrator 2: CSVR + VV        T B(dt/2) A(dt) F B(dt/2)          (470-635)
+ *                 T: momenta scaled by the stochastic velocity rescaling
+ *                 factor (Bussi, Donadio, Parrinello 2007) of each system,
+ *                 one R1 ~ N(0,1) for the batch and R2 ~ chi^2(3N - 4) per
+ *                 system from Philox4x32-10 (seed, step), as the oracle draws.
  * A steps wrap positions to [-0.5, 0.5) fractional (104-141); O uses
  * p <- c1 p + sqrt(kT (1 - c1^2)) sqrt(m) eta with eta from Philox4x32-10
  * keyed by (seed, global atom index, step), see kupsmi_normals.
@@ -132,7 +137,8 @@ int kupsmi_lj_forces(kupsmi_ctx* ctx, void* stream, int n_sys,
  * use the per-step cell list instead.
  * In/out: pos, mom, forces (F at the current positions on entry: the
  * reference initialises it to zero, md/data.py:188).
- * Per system: mass per atom (f32), dt, kT (eV), gamma (1/internal time).
+ * Per system: mass per atom (f32), dt, kT (eV), gamma (1/internal time),
+ * tau (CSVR coupling time, internal units; NULL unless integrator 2).
  * Outputs: pot_energy[n_sys x n_steps] and kin_energy[n_sys x n_steps] (f64,
  * after each step; either may be NULL), virial[n_sys x 9] of the last step
  * (may be NULL).  step0 is the global index of the first step (RNG stream
@@ -145,7 +151,8 @@ int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n_steps,
                  const double* sigma, const double* epsilon, const double* tril,
                  const double* cutoff, double skin, int max_neighbors,
                  const double* dt, const double* kT, const double* gamma,
-                 double* pot_energy, double* kin_energy, double* virial);
+                 const double* tau, double* pot_energy, double* kin_energy,
+                 double* virial);
 
 /*
  * The standard normals eta[n_atoms x 3] that kupsmi_md_lj's O step uses at

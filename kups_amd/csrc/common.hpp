@@ -196,6 +196,67 @@ __device__ inline void normals3(uint64_t seed, uint32_t atom, int64_t step, floa
   n2 = r1 * c1;
 }
 
+// ------------------------------------------------------------- CSVR ----
+// Stochastic velocity rescaling (Bussi, Donadio and Parrinello, J. Chem.
+// Phys. 126, 014101 (2007); md/integrators.py:470-585): per system
+//   alpha^2 = c1 + c2 (R1^2 + R2) + 2 R1 sqrt(c1 c2), c1 = exp(-dt / tau),
+//   c2 = (1 - c1) K_target / (K N_dof), K_target = N_dof kT / 2, N_dof = 3N - 3,
+// with ONE R1 ~ N(0, 1) for the whole batch (the reference draws a scalar)
+// and R2 ~ chi^2(N_dof - 1) = 2 Gamma((N_dof - 1) / 2) per system by
+// Marsaglia and Tsang (ACM TOMS 26 (2000) 363-372).  Counters: R1 from
+// {0xFFFFFFFF, t, t >> 32, CSVR_TAG_R1}; attempt k of system s from
+// {s, t, t >> 32, CSVR_TAG_R2 + k} (x = Box-Muller of words 0, 1; u = word 2);
+// shape < 1 boosts from shape + 1 with U^(1/shape), U = word 0 of attempt 255.
+constexpr uint32_t CSVR_TAG_R1 = 0x43535631u;
+constexpr uint32_t CSVR_TAG_R2 = 0x43535700u;
+
+__device__ inline double csvr_u(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
+
+__device__ inline double csvr_bm(uint32_t a, uint32_t b) {
+  return sqrt(-2.0 * log(csvr_u(a))) * cospi(2.0 * csvr_u(b));
+}
+
+__device__ inline void csvr_draw(uint64_t seed, uint32_t a, int64_t t, uint32_t tag, uint32_t c[4]) {
+  c[0] = a; c[1] = (uint32_t)t; c[2] = (uint32_t)((uint64_t)t >> 32); c[3] = tag;
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+__device__ inline double csvr_gamma(double shape, uint64_t seed, uint32_t sys, int64_t t) {
+  uint32_t c[4];
+  double boost = 1.0;
+  if (shape < 1.0) {
+    csvr_draw(seed, sys, t, CSVR_TAG_R2 + 255u, c);
+    boost = pow(csvr_u(c[0]), 1.0 / shape);
+    shape += 1.0;
+  }
+  const double d = shape - 1.0 / 3.0, cc = 1.0 / sqrt(9.0 * d);
+  for (uint32_t k = 0; k < 255u; ++k) {
+    csvr_draw(seed, sys, t, CSVR_TAG_R2 + k, c);
+    const double x = csvr_bm(c[0], c[1]);
+    double v = 1.0 + cc * x;
+    if (v <= 0.0) continue;
+    v = v * v * v;
+    if (log(csvr_u(c[2])) < 0.5 * x * x + d - d * v + d * log(v)) return boost * d * v;
+  }
+  return boost * d;
+}
+
+// alpha of system `sys` at global step t from its kinetic energy K (eV); 1
+// for a system without degrees of freedom or kinetic energy.
+__device__ inline double csvr_alpha(double K, int n_atoms, double kT, double dt, double tau,
+                                    uint64_t seed, uint32_t sys, int64_t t) {
+  const int ndof = 3 * n_atoms - 3;
+  if (ndof <= 0 || !(K > 0.0)) return 1.0;
+  uint32_t c[4];
+  csvr_draw(seed, 0xFFFFFFFFu, t, CSVR_TAG_R1, c);
+  const double r1 = csvr_bm(c[0], c[1]);
+  const double r2 = ndof - 1 > 0 ? 2.0 * csvr_gamma(0.5 * (ndof - 1), seed, sys, t) : 0.0;
+  const double c1 = exp(-dt / tau);
+  const double c2 = (1.0 - c1) * (0.5 * ndof * kT) / (K * ndof);
+  const double a2 = c1 + c2 * (r1 * r1 + r2) + 2.0 * r1 * sqrt(c1 * c2);
+  return sqrt(a2 > 0.0 ? a2 : 0.0);
+}
+
 // ---------------------------------------------------------- reductions ----
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

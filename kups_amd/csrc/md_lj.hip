@@ -56,6 +56,7 @@ struct MdArgs {
   const double* dt;
   const double* kT;
   const double* gamma;
+  const double* tau;  // CSVR coupling time per system (internal units)
   double* pot;
   double* kin;
   double* virial;
@@ -414,8 +415,8 @@ __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const f
                                       float4* pb, const int* cstart, const short* nbcount,
                                       const unsigned short* nbl, bool use_list,
                                       const float4* s_par, int ns, int n, int i0, float rc2_lo,
-                                      float rc2_hi, float hdt, float dtf, bool last, float& epot,
-                                      float& ekin, float (&w)[6]) {
+                                      float rc2_hi, float hdt, float dtf, bool last, bool split,
+                                      float& epot, float& ekin, float (&w)[6]) {
   const int tid = threadIdx.x;
 #pragma unroll 1
   for (int k = 0; k < MD_SLOTS; ++k) {
@@ -437,11 +438,11 @@ __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const f
       ekin += 0.5f * (vx * vx + vy * vy + vz * vz) * im;
     }
     if (a.n_steps > 0) {
-      const float kick = last ? hdt : dtf;
+      const float kick = last || split ? hdt : dtf;
       p.x += ax * kick; p.y += ay * kick; p.z += az * kick;
       pb[slot] = p;
     }
-    if (last) {
+    if (last || split) {
       a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
     }
   }
@@ -490,6 +491,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
   __shared__ float4 s_par[MD_NSPEC * MD_NSPEC];  // sigma^2, 4 eps, 24 eps
   __shared__ SysGeom g;
   __shared__ int s_q;
+  __shared__ float s_alpha;
 
   const int tid = threadIdx.x;
   const int ns = a.n_species;
@@ -542,6 +544,12 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
     const float dtf = a.n_steps > 0 ? (float)a.dt[s] : 0.0f;
     const float hdt = 0.5f * dtf;
     float c1 = 1.0f, c2 = 0.0f;
+    // CSVR (integrator 2): the thermostat scales the momenta at the start of
+    // every step, between the previous step's closing half kick and this
+    // step's opening one, so the two half kicks are not fused and F goes to
+    // HBM every step
+    const bool csvr = a.integrator == 2 && a.n_steps > 0;
+    double kin_prev = 0.0;  // K after the previous closing kick (thread 0)
     if (a.integrator == 1 && a.n_steps > 0) {
       double e1 = exp(-a.gamma[s] * a.dt[s]);
       c1 = (float)e1;
@@ -573,6 +581,26 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
       // reload the LDS-resident geometry in every step instead of keeping
       // the hoisted copies live across the whole step loop
       asm volatile("" ::: "memory");
+      float alpha = 1.0f;
+      if (csvr) {
+        double K = kin_prev;
+        if (st == 0) {
+          float k0 = 0.0f;
+#pragma unroll
+          for (int k = 0; k < MD_SLOTS; ++k) {
+            const int slot = tid + k * MD_THREADS;
+            if (slot >= n) continue;
+            const float4 p = pb[slot];
+            k0 += 0.5f * (p.x * p.x + p.y * p.y + p.z * p.z) / a.mass[i0 + __float_as_int(p.w)];
+          }
+          K = block_sum((double)k0, red);
+        }
+        if (tid == 0)
+          s_alpha = (float)csvr_alpha(K, n, a.kT[s], a.dt[s], a.tau[s], a.seed, (uint32_t)s,
+                                      a.step0 + st);
+        __syncthreads();
+        alpha = s_alpha;
+      }
       // ---- [B(dt/2) at st = 0] A(...) [O A] in place, wrap, cell of every atom
       int cel[MD_SLOTS];
       float dmax = 0.0f;
@@ -586,7 +614,11 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
           float4 p = pb[slot];
           const int gi = i0 + __float_as_int(p.w);
           const float im = 1.0f / a.mass[gi];
-          if (st == 0) {
+          if (csvr) {
+            p.x = alpha * p.x + a.forces[3 * gi] * hdt;
+            p.y = alpha * p.y + a.forces[3 * gi + 1] * hdt;
+            p.z = alpha * p.z + a.forces[3 * gi + 2] * hdt;
+          } else if (st == 0) {
             p.x += a.forces[3 * gi] * hdt;
             p.y += a.forces[3 * gi + 1] * hdt;
             p.z += a.forces[3 * gi + 2] * hdt;
@@ -685,7 +717,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
       const bool do_w = last && a.want_virial;
 #define MD_PHASE(M, V, O)                                                                     \
   md_force_phase<M, V, O>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0, rc2_lo, \
-                          rc2_hi, hdt, dtf, last, epot, ekin, w)
+                          rc2_hi, hdt, dtf, last, csvr, epot, ekin, w)
       if (ns == 1) {
         if (do_w) { if (ortho) MD_PHASE(false, true, true); else MD_PHASE(false, true, false); }
         else { if (ortho) MD_PHASE(false, false, true); else MD_PHASE(false, false, false); }
@@ -698,9 +730,10 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
         double e = block_sum((double)epot, red);
         if (tid == 0) a.pot[(size_t)s * total + st] = e;
       }
-      if (a.kin) {
+      if (a.kin || csvr) {
         double e = block_sum((double)ekin, red);
-        if (tid == 0) a.kin[(size_t)s * total + st] = e;
+        if (tid == 0 && a.kin) a.kin[(size_t)s * total + st] = e;
+        kin_prev = e;
       }
       if (do_w && a.virial) {
         double out[6];
@@ -750,7 +783,8 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
   KUPSMI_REQUIRE(ctx != nullptr, "null context");
   KUPSMI_REQUIRE(a.n_sys >= 0, "n_sys < 0");
   KUPSMI_REQUIRE(a.n_species >= 1 && a.n_species <= MD_NSPEC, "n_species must be in [1, 8]");
-  KUPSMI_REQUIRE(a.integrator == 0 || a.integrator == 1, "integrator must be 0 (VV) or 1 (BAOAB)");
+  KUPSMI_REQUIRE(a.integrator >= 0 && a.integrator <= 2,
+                 "integrator must be 0 (VV), 1 (BAOAB) or 2 (CSVR)");
   KUPSMI_REQUIRE(a.n_steps >= 0, "n_steps < 0");
   if (a.n_sys == 0) return KUPSMI_OK;
   void* buf = nullptr;
@@ -797,20 +831,23 @@ extern "C" int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n
                             const double* sigma, const double* epsilon, const double* tril,
                             const double* cutoff, double skin, int max_neighbors,
                             const double* dt, const double* kT, const double* gamma,
-                            double* pot_energy, double* kin_energy, double* virial) {
+                            const double* tau, double* pot_energy, double* kin_energy,
+                            double* virial) {
   KUPSMI_REQUIRE(n_atoms >= 0, "n_atoms < 0");
   KUPSMI_REQUIRE(n_steps == 0 || (dt != nullptr), "dt required");
   KUPSMI_REQUIRE(skin >= 0.0 && max_neighbors >= 0, "skin and max_neighbors must be >= 0");
   KUPSMI_REQUIRE(skin == 0.0 || (max_neighbors > 0 && max_neighbors <= 32767),
                  "a skin needs max_neighbors in [1, 32767]");
   KUPSMI_REQUIRE(integrator != 1 || n_steps == 0 || (kT && gamma), "kT and gamma required for BAOAB");
+  KUPSMI_REQUIRE(integrator != 2 || n_steps == 0 || (kT && tau), "kT and tau required for CSVR");
   MdArgs a{};
   a.integrator = integrator; a.n_steps = n_steps; a.n_sys = n_sys; a.n_species = n_species;
   a.want_virial = virial != nullptr;
   a.step0 = step0; a.seed = seed;
   a.sys_off = sys_off; a.pos = pos; a.mom = mom; a.forces = forces; a.mass = mass;
   a.species = species; a.sigma = sigma; a.epsilon = epsilon; a.tril = tril; a.cutoff = cutoff;
-  a.dt = dt; a.kT = kT; a.gamma = gamma; a.pot = pot_energy; a.kin = kin_energy; a.virial = virial;
+  a.dt = dt; a.kT = kT; a.gamma = gamma; a.tau = tau;
+  a.pot = pot_energy; a.kin = kin_energy; a.virial = virial;
   a.skin = (float)skin; a.nbmax = max_neighbors;
   return md_launch(ctx, (hipStream_t)stream, a);
 }

@@ -3,8 +3,8 @@
 Momenta are drawn from Maxwell-Boltzmann, p ~ N(0, sqrt(m kT)) per component,
 minus the mean momentum of the system (171-178); gradients start at zero
 (188), so the first half kick of the first step uses F = 0 unless the caller
-evaluates forces first.  Time step and friction are converted from fs and
-1/fs to internal units (195-205).
+evaluates forces first.  Time step, friction and the CSVR coupling time are
+converted from fs and 1/fs to internal units (195-205).
 """
 
 from __future__ import annotations
@@ -34,6 +34,8 @@ class MDState:
     temperature: torch.Tensor  # (n_sys,) f64, K
     friction: torch.Tensor  # (n_sys,) f64, 1 / internal time
     step: int = 0
+    # (n_sys,) f64, CSVR coupling time tau, internal units
+    thermostat_time_constant: torch.Tensor | None = None
     charges: torch.Tensor | None = None  # (n,) f32, for Ewald systems
     extras: dict = field(default_factory=dict)
 
@@ -66,8 +68,8 @@ def maxwell_boltzmann_momenta(masses: np.ndarray, sizes, temperature, rng: np.ra
 
 
 def make_md_state(positions, masses, species, sizes, tril, cutoff, *, time_step_fs,
-                  temperature, friction_per_fs=0.0, momenta=None, charges=None,
-                  device=None) -> MDState:
+                  temperature, friction_per_fs=0.0, thermostat_time_constant_fs=100.0,
+                  momenta=None, charges=None, device=None) -> MDState:
     dev = device or require_gpu()
     n_sys = len(sizes)
     n = int(np.sum(sizes))
@@ -91,6 +93,8 @@ def make_md_state(positions, masses, species, sizes, tril, cutoff, *, time_step_
         time_step=per_sys(np.asarray(time_step_fs, np.float64) * FEMTO_SECOND),
         temperature=per_sys(temperature),
         friction=per_sys(np.asarray(friction_per_fs, np.float64) / FEMTO_SECOND),
+        thermostat_time_constant=per_sys(
+            np.asarray(thermostat_time_constant_fs, np.float64) * FEMTO_SECOND),
         charges=None if charges is None else
         torch.as_tensor(np.asarray(charges, np.float32), device=dev).contiguous(),
     )

@@ -26,6 +26,7 @@ from .data import MDState
 
 VELOCITY_VERLET = 0
 BAOAB_LANGEVIN = 1
+CSVR = 2
 
 
 @dataclass
@@ -101,6 +102,7 @@ class FusedLJPropagator:
             _lib.ptr(state.masses), _lib.ptr(state.species), self.params.n_species,
             _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(state.tril), _lib.ptr(state.cutoff),
             skin, cap, _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction),
+            _lib.ptr(state.thermostat_time_constant),
             _lib.ptr(pot), _lib.ptr(kin), _lib.ptr(vir)))
         state.step += n_steps
         if not record:
@@ -120,6 +122,15 @@ def make_baoab_langevin_step(params: LennardJonesParameters, seed: int = 0,
                              skin: float | None = None) -> FusedLJPropagator:
     """NVT BAOAB Langevin (md/integrators.py:405-453); O step noise from Philox4x32-10."""
     return FusedLJPropagator(params, BAOAB_LANGEVIN, seed, skin)
+
+
+def make_csvr_step(params: LennardJonesParameters, seed: int = 0,
+                   skin: float | None = None) -> FusedLJPropagator:
+    """NVT velocity Verlet with the CSVR thermostat before every step
+    (md/integrators.py:470-635; Bussi, Donadio and Parrinello 2007): one
+    R1 ~ N(0, 1) for the batch and R2 ~ chi^2(3N - 4) per system, from
+    Philox4x32-10 (seed, step); coupling time from the state."""
+    return FusedLJPropagator(params, CSVR, seed, skin)
 
 
 def normals(seed: int, step: int, n_atoms: int, device=None) -> torch.Tensor:

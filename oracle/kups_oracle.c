@@ -757,6 +757,85 @@ void orc_philox(uint64_t seed, uint32_t a, int64_t t, uint32_t tag, uint32_t* ou
   gc_draw(seed, a, t, tag, out);
 }
 
+/* ------------------------------------------------------------- CSVR ---- */
+/* Stochastic velocity rescaling (md/integrators.py:470-585, Bussi, Donadio
+ * and Parrinello, J. Chem. Phys. 126, 014101 (2007)): per system
+ *   alpha^2 = c1 + c2 (R1^2 + R2) + 2 R1 sqrt(c1 c2),
+ *   c1 = exp(-dt / tau), c2 = (1 - c1) K_target / (K N_dof), K_target = N_dof kT / 2,
+ *   N_dof = 3 N - 3, R1 ~ N(0, 1) ONE draw shared by every system of the batch
+ *   (the reference draws a scalar, 552), R2 ~ chi^2(N_dof - 1) per system
+ *   (0 when N_dof - 1 <= 0), p <- alpha p.
+ * Random numbers (shared with the GPU kernel):
+ *   R1: Box-Muller of words 0, 1 of philox({0xFFFFFFFF, t, t >> 32, CSVR_TAG_R1});
+ *   R2: 2 Gamma((N_dof - 1) / 2) by Marsaglia and Tsang (ACM TOMS 26 (2000)
+ *       363-372), attempt k drawing philox({system, t, t >> 32, CSVR_TAG_R2 + k}):
+ *       x = Box-Muller(words 0, 1), u = word 2; shape < 1 boosts from shape + 1
+ *       with U^(1/shape), U = word 0 of attempt 255.
+ * Uniforms in FP64: u = (x + 0.5) 2^-32. */
+#define CSVR_TAG_R1 0x43535631u
+#define CSVR_TAG_R2 0x43535700u
+
+static double csvr_u(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
+static double csvr_bm(uint32_t a, uint32_t b) {
+  return sqrt(-2.0 * log(csvr_u(a))) * cos(2.0 * M_PI * csvr_u(b));
+}
+
+double orc_gamma_mt(double shape, uint64_t seed, uint32_t sys, int64_t t) {
+  uint32_t c[4];
+  double boost = 1.0;
+  if (shape < 1.0) {
+    gc_draw(seed, sys, t, CSVR_TAG_R2 + 255u, c);
+    boost = pow(csvr_u(c[0]), 1.0 / shape);
+    shape += 1.0;
+  }
+  const double d = shape - 1.0 / 3.0, cc = 1.0 / sqrt(9.0 * d);
+  for (uint32_t k = 0; k < 255u; ++k) {
+    gc_draw(seed, sys, t, CSVR_TAG_R2 + k, c);
+    const double x = csvr_bm(c[0], c[1]);
+    double v = 1.0 + cc * x;
+    if (v <= 0.0) continue;
+    v = v * v * v;
+    if (log(csvr_u(c[2])) < 0.5 * x * x + d - d * v + d * log(v)) return boost * d * v;
+  }
+  return boost * d;
+}
+
+double orc_csvr_r1(uint64_t seed, int64_t t) {
+  uint32_t c[4];
+  gc_draw(seed, 0xFFFFFFFFu, t, CSVR_TAG_R1, c);
+  return csvr_bm(c[0], c[1]);
+}
+
+/* alpha of one system from its kinetic energy K (eV). */
+double orc_csvr_alpha(double K, int n_atoms, double kT, double dt, double tau, double r1,
+                      uint64_t seed, uint32_t sys, int64_t t) {
+  const int ndof = 3 * n_atoms - 3;
+  if (ndof <= 0 || !(K > 0.0)) return 1.0;
+  const double r2 = ndof - 1 > 0 ? 2.0 * orc_gamma_mt(0.5 * (ndof - 1), seed, sys, t) : 0.0;
+  const double c1 = exp(-dt / tau);
+  const double c2 = (1.0 - c1) * (0.5 * ndof * kT) / (K * ndof);
+  const double a2 = c1 + c2 * (r1 * r1 + r2) + 2.0 * r1 * sqrt(c1 * c2);
+  return sqrt(a2 > 0.0 ? a2 : 0.0);
+}
+
+/* The CSVR step of every system at global step t: momenta rescaled in place,
+ * K from the FP64 momenta; alpha[s] receives the factor. */
+void orc_csvr_scale(uint64_t seed, int64_t t, int nsys, const int32_t* sys_off, const double* mass,
+                    double* mom, const double* kT, const double* dt, const double* tau,
+                    double* alpha) {
+  const double r1 = orc_csvr_r1(seed, t);
+  for (int s = 0; s < nsys; ++s) {
+    double K = 0.0;
+    for (int i = sys_off[s]; i < sys_off[s + 1]; ++i)
+      for (int a = 0; a < 3; ++a) K += 0.5 * mom[3 * i + a] * mom[3 * i + a] / mass[i];
+    const double al = orc_csvr_alpha(K, sys_off[s + 1] - sys_off[s], kT[s], dt[s], tau[s], r1,
+                                     seed, (uint32_t)s, t);
+    for (int i = sys_off[s]; i < sys_off[s + 1]; ++i)
+      for (int a = 0; a < 3; ++a) mom[3 * i + a] *= al;
+    if (alpha) alpha[s] = al;
+  }
+}
+
 /* Shoemake random quaternion (core/utils/quaternion.py:70-97), the power q^w
  * (_pow_quaternion: sign fix w >= 0, angle scaled) and the rotation matrix
  * (_quat_to_mat), applied as row-vector @ R (moves.py:190). */

@@ -36,6 +36,7 @@ def lib():
         _lib.orc_coulomb_k.restype = C.c_double
         _lib.orc_md_vv_f32.restype = C.c_int
         _lib.orc_num_threads.restype = C.c_int
+        _lib.orc_gamma_mt.restype = C.c_double
     return _lib
 
 
@@ -184,6 +185,23 @@ def md_step(integrator, pos, mom, mass, species, sys_off, tril, cutoff, sigma, e
                       _p(sys_off), _p(tril), _p(cutoff), sigma.shape[0], _p(sigma), _p(eps),
                       _p(dt), _p(kT), _p(gamma), _p(eta), _p(forces), _p(e), _p(w))
     return e, w.reshape(nsys, 3, 3)
+
+
+def gamma_mt(shape: float, seed: int, sys: int, t: int) -> float:
+    """Gamma(shape, 1) draw of the CSVR chi-square (Marsaglia-Tsang, shared stream)."""
+    return float(lib().orc_gamma_mt(C.c_double(shape), C.c_uint64(int(seed) & (2**64 - 1)),
+                                    C.c_uint32(sys), C.c_int64(t)))
+
+
+def csvr_scale(seed, t, mom, mass, sys_off, kT, dt, tau):
+    """CSVR rescaling of FP64 momenta in place at global step t; returns alpha per system."""
+    assert mom.dtype == np.float64 and mom.flags.c_contiguous
+    mass, sys_off, kT, dt, tau = _d(mass), _i(sys_off), _d(kT), _d(dt), _d(tau)
+    nsys = len(sys_off) - 1
+    alpha = np.zeros(nsys)
+    lib().orc_csvr_scale(C.c_uint64(int(seed) & (2**64 - 1)), C.c_int64(t), nsys, _p(sys_off),
+                         _p(mass), _p(mom), _p(kT), _p(dt), _p(tau), _p(alpha))
+    return alpha
 
 
 def md_vv_f32(n_steps, pos, mom, mass, species, sys_off, tril, cutoff, sigma, eps, dt, forces):

@@ -135,3 +135,42 @@ def test_oracle_mlp_gradient_finite_differences():
         q[idx] -= 2 * h
         lm = oracle.mlp_loss_and_grad(q, X, y)[0]
         assert (lp - lm) / (2 * h) == pytest.approx(g[idx], rel=1e-5, abs=1e-9)
+
+
+def test_csvr_chi_square_sampler_matches_scipy():
+    """The Marsaglia-Tsang Gamma draws of the CSVR step: 2 Gamma(k/2) is
+    chi^2(k) (Kolmogorov-Smirnov against scipy), for large and small k."""
+    from scipy import stats
+    for k in (1.0, 3.0, 40.0, 12285.0):
+        x = np.array([2.0 * oracle.gamma_mt(0.5 * k, 11, s, t)
+                      for s in range(40) for t in range(100)])
+        assert stats.kstest(x, stats.chi2(k).cdf).pvalue > 1e-3, k
+
+
+def test_csvr_keeps_the_canonical_kinetic_energy_distribution():
+    """Force-free CSVR steps (the thermostat alone) sample K from the canonical
+    Gamma(N_dof / 2, kT) law whatever tau is (the exactness property of Bussi,
+    Donadio and Parrinello): mean N_dof kT / 2, variance N_dof (kT)^2 / 2, from
+    a start far from equilibrium."""
+    rng = np.random.default_rng(3)
+    n, kT = 20, 0.025
+    mass = np.full(n, 39.948)
+    mom = rng.normal(0, 5.0, (n, 3))  # ~40x too hot
+    mom -= mom.mean(axis=0)
+    ndof = 3 * n - 3
+    ks = []
+    for t in range(6000):
+        oracle.csvr_scale(5, t, mom, mass, [0, n], [kT], [1.0], [7.0])
+        if t >= 1000:
+            ks.append(0.5 * np.sum(mom ** 2 / mass[:, None]))
+    ks = np.array(ks)
+    # effective sample size from the coupling: correlation time tau / dt = 7
+    assert abs(ks.mean() / (0.5 * ndof * kT) - 1) < 0.03
+    assert abs(ks.var() / (0.5 * ndof * kT ** 2) - 1) < 0.15
+
+
+def test_csvr_shares_r1_and_needs_dof():
+    """One R1 for the whole batch; a one-atom system (N_dof = 0) is left as it is."""
+    mom = np.ones((3, 3))
+    a = oracle.csvr_scale(1, 0, mom, np.ones(3), [0, 1, 3], [0.01, 0.01], [1.0, 1.0], [2.0, 2.0])
+    assert a[0] == 1.0 and a[1] != 1.0
