@@ -123,6 +123,13 @@ int kupsmi_lj_forces(kupsmi_ctx* ctx, void* stream, int n_sys,
  *                 factor (Bussi, Donadio, Parrinello 2007) of each system,
  *                 one R1 ~ N(0,1) for the batch and R2 ~ chi^2(3N - 4) per
  *                 system from Philox4x32-10 (seed, step), as the oracle draws.
+ *   integrator 3: CSVR + VV + stochastic cell rescaling (829-888):
+ *                 T B(dt/2) A(dt) F B(dt/2) S F, S scaling the cell and the
+ *                 positions by mu = exp(d eps / 3) (Bernetti and Bussi 2020),
+ *                 d eps = (dt/tau_P) beta (P - P0) + sqrt(2 kT beta dt /
+ *                 (tau_P V)) R with one R ~ N(0,1) for the batch, P from the
+ *                 kinetic energy and the virial of the first F; the cell list
+ *                 is rebuilt every step (skin ignored).
  * A steps wrap positions to [-0.5, 0.5) fractional (104-141); O uses
  * p <- c1 p + sqrt(kT (1 - c1^2)) sqrt(m) eta with eta from Philox4x32-10
  * keyed by (seed, global atom index, step), see kupsmi_normals.
@@ -138,7 +145,10 @@ int kupsmi_lj_forces(kupsmi_ctx* ctx, void* stream, int n_sys,
  * In/out: pos, mom, forces (F at the current positions on entry: the
  * reference initialises it to zero, md/data.py:188).
  * Per system: mass per atom (f32), dt, kT (eV), gamma (1/internal time),
- * tau (CSVR coupling time, internal units; NULL unless integrator 2).
+ * tau (CSVR coupling time, internal units; NULL unless integrator >= 2),
+ * baro[n_sys x 4] (target pressure eV/A^3, coupling time, compressibility
+ * A^3/eV, minimum scale factor; integrator 3) and tril_out (the rescaled
+ * cell, may alias tril; integrator 3, else NULL).
  * Outputs: pot_energy[n_sys x n_steps] and kin_energy[n_sys x n_steps] (f64,
  * after each step; either may be NULL), virial[n_sys x 9] of the last step
  * (may be NULL).  step0 is the global index of the first step (RNG stream
@@ -151,8 +161,8 @@ int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n_steps,
                  const double* sigma, const double* epsilon, const double* tril,
                  const double* cutoff, double skin, int max_neighbors,
                  const double* dt, const double* kT, const double* gamma,
-                 const double* tau, double* pot_energy, double* kin_energy,
-                 double* virial);
+                 const double* tau, const double* baro, double* tril_out,
+                 double* pot_energy, double* kin_energy, double* virial);
 
 /*
  * The standard normals eta[n_atoms x 3] that kupsmi_md_lj's O step uses at

@@ -30,7 +30,7 @@ _SIGNATURES = {
     "kupsmi_lj_forces": ([vp, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp], C.c_int),
     "kupsmi_md_lj": (
         [vp, vp, i32, i32, i64, u64, i32, vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp,
-         C.c_double, i32, vp, vp, vp, vp, vp, vp, vp],
+         C.c_double, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp],
         C.c_int,
     ),
     "kupsmi_normals": ([vp, u64, i64, i32, vp], C.c_int),

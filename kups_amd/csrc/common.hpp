@@ -209,6 +209,9 @@ __device__ inline void normals3(uint64_t seed, uint32_t atom, int64_t step, floa
 // shape < 1 boosts from shape + 1 with U^(1/shape), U = word 0 of attempt 255.
 constexpr uint32_t CSVR_TAG_R1 = 0x43535631u;
 constexpr uint32_t CSVR_TAG_R2 = 0x43535700u;
+// the batch-shared normal of the stochastic cell rescaling, counter
+// {0xFFFFFFFE, t, t >> 32, SCR_TAG}
+constexpr uint32_t SCR_TAG = 0x53435231u;
 
 __device__ inline double csvr_u(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
 

@@ -57,6 +57,10 @@ struct MdArgs {
   const double* kT;
   const double* gamma;
   const double* tau;  // CSVR coupling time per system (internal units)
+  // stochastic cell rescaling (integrator 3) per system: target pressure,
+  // coupling time, compressibility, minimum scale factor; the cell out
+  const double* baro;
+  double* tril_out;
   double* pot;
   double* kin;
   double* virial;
@@ -416,7 +420,7 @@ __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const f
                                       const unsigned short* nbl, bool use_list,
                                       const float4* s_par, int ns, int n, int i0, float rc2_lo,
                                       float rc2_hi, float hdt, float dtf, bool last, bool split,
-                                      float& epot, float& ekin, float (&w)[6]) {
+                                      bool kick_now, float& epot, float& ekin, float (&w)[6]) {
   const int tid = threadIdx.x;
 #pragma unroll 1
   for (int k = 0; k < MD_SLOTS; ++k) {
@@ -432,12 +436,12 @@ __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const f
     epot += 0.5f * ei;
     float4 p = pb[slot];
     const int gi = i0 + __float_as_int(p.w);
-    if (a.n_steps > 0 || a.kin) {
+    if (kick_now && (a.n_steps > 0 || a.kin)) {
       const float im = 1.0f / a.mass[gi];
       const float vx = p.x + ax * hdt, vy = p.y + ay * hdt, vz = p.z + az * hdt;
       ekin += 0.5f * (vx * vx + vy * vy + vz * vz) * im;
     }
-    if (a.n_steps > 0) {
+    if (kick_now && a.n_steps > 0) {
       const float kick = last || split ? hdt : dtf;
       p.x += ax * kick; p.y += ay * kick; p.z += az * kick;
       pb[slot] = p;
@@ -478,6 +482,10 @@ __device__ inline void permute_slots(float4* buf, const int (&dst)[MD_SLOTS], in
   __syncthreads();
 }
 
+// THERMO: 0 for velocity Verlet / BAOAB, 1 for CSVR, 2 for CSVR + cell
+// rescaling; each is its own instance, so the plain integrators carry none of
+// the thermostat's and barostat's state.
+template <int THERMO>
 __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
   __shared__ float4 sp[MD_NMAX];   // sorted x,y,z ; w = bits(species | cell << 8)
   __shared__ float4 pb[MD_NMAX];   // sorted px,py,pz ; w = bits(local id)
@@ -492,6 +500,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
   __shared__ SysGeom g;
   __shared__ int s_q;
   __shared__ float s_alpha;
+  __shared__ double s_mu, s_tril[6];
 
   const int tid = threadIdx.x;
   const int ns = a.n_species;
@@ -540,7 +549,6 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
     __syncthreads();
     const float rc2 = g.rc2f;
     const float rc2_hi = rc2 * (1.0f + CUT_BAND), rc2_lo = rc2 * (1.0f - CUT_BAND);
-    const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2], ncell = g.ncell;
     const float dtf = a.n_steps > 0 ? (float)a.dt[s] : 0.0f;
     const float hdt = 0.5f * dtf;
     float c1 = 1.0f, c2 = 0.0f;
@@ -548,7 +556,9 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
     // every step, between the previous step's closing half kick and this
     // step's opening one, so the two half kicks are not fused and F goes to
     // HBM every step
-    const bool csvr = a.integrator == 2 && a.n_steps > 0;
+    const bool npt = THERMO == 2 && a.n_steps > 0;
+    const bool csvr = THERMO >= 1 && a.n_steps > 0;
+    if (npt && tid < 6) s_tril[tid] = a.tril[6 * (size_t)s + tid];
     double kin_prev = 0.0;  // K after the previous closing kick (thread 0)
     if (a.integrator == 1 && a.n_steps > 0) {
       double e1 = exp(-a.gamma[s] * a.dt[s]);
@@ -557,7 +567,7 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
     }
     // Verlet lists for boxes whose perpendicular widths are all >= 2 rc
     // (one image per pair within the cutoff) and runs of >= 2 steps
-    bool use_list = nbl != nullptr && a.skin > 0.0f && g.rep == 0 && a.n_steps >= 2;
+    bool use_list = nbl != nullptr && a.skin > 0.0f && g.rep == 0 && a.n_steps >= 2 && !npt;
     bool listed = false;
     const double lr = g.rc + (double)a.skin;
     const float list_r2 = (float)(lr * lr) * (1.0f + 1e-5f);
@@ -579,8 +589,10 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
       // every force loop of the previous step has finished reading sp[]
       __syncthreads();
       // reload the LDS-resident geometry in every step instead of keeping
-      // the hoisted copies live across the whole step loop
+      // the hoisted copies live across the whole step loop (and because the
+      // barostat changes it)
       asm volatile("" ::: "memory");
+      const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2], ncell = g.ncell;
       float alpha = 1.0f;
       if (csvr) {
         double K = kin_prev;
@@ -648,11 +660,9 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
         r.w = __int_as_float((__float_as_int(r.w) & 0xff) | (cel[k] << 8));
         sp[slot] = r;
       }
-      // the lists stay valid while no atom can have moved by skin/2
-      bool sort_now = true;
-      if (use_list && listed) sort_now = block_max_all(dmax, red) > 0.5f * a.skin;
-      if (sort_now) {
-        // ---- stable counting sort into cells (rank by previous slot)
+      // ---- stable counting sort into cells (rank by previous slot), slots
+      // permuted in place
+      auto sort_cells = [&](const int (&cel)[MD_SLOTS], int ncell) {
         __syncthreads();
         for (int c = tid; c < ncell; c += MD_THREADS) ccount[c] = 0;
         __syncthreads();
@@ -670,26 +680,30 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
           if (slot < n) keys[cstart[cel[k]] + atomicAdd(&ccount[cel[k]], 1)] = (short)slot;
         }
         __syncthreads();
-        {
-          // destination of every slot, then the positions and the momenta
-          // permuted one array at a time: only MD_SLOTS float4 are held in
-          // registers across a barrier (the 128-VGPR budget of a 1024-thread
-          // workgroup has no room for both)
-          int dst[MD_SLOTS];
+        // destination of every slot, then the positions and the momenta
+        // permuted one array at a time: only MD_SLOTS float4 are held in
+        // registers across a barrier (the 128-VGPR budget of a 1024-thread
+        // workgroup has no room for both)
+        int dst[MD_SLOTS];
 #pragma unroll
-          for (int k = 0; k < MD_SLOTS; ++k) {
-            const int slot = tid + k * MD_THREADS;
-            dst[k] = -1;
-            if (slot < n) {
-              const int b = cstart[cel[k]], e = cstart[cel[k] + 1];
-              int rank = 0;
-              for (int t = b; t < e; ++t) rank += keys[t] < slot;
-              dst[k] = b + rank;
-            }
+        for (int k = 0; k < MD_SLOTS; ++k) {
+          const int slot = tid + k * MD_THREADS;
+          dst[k] = -1;
+          if (slot < n) {
+            const int b = cstart[cel[k]], e = cstart[cel[k] + 1];
+            int rank = 0;
+            for (int t = b; t < e; ++t) rank += keys[t] < slot;
+            dst[k] = b + rank;
           }
-          permute_slots(sp, dst, tid);
-          permute_slots(pb, dst, tid);
         }
+        permute_slots(sp, dst, tid);
+        permute_slots(pb, dst, tid);
+      };
+      // the lists stay valid while no atom can have moved by skin/2
+      bool sort_now = true;
+      if (use_list && listed) sort_now = block_max_all(dmax, red) > 0.5f * a.skin;
+      if (sort_now) {
+        sort_cells(cel, ncell);
         if (use_list) {
           // ---- Verlet lists of the new order; an overflowing list turns
           // the lists off for the rest of the run (the stencil is exact)
@@ -711,42 +725,96 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
           listed = use_list;
         }
       }
-      // ---- forces, kick and kinetic energy
+      // ---- forces, kick and kinetic energy; with the barostat a second pass
+      // recomputes forces and energies at the rescaled state (one call site,
+      // so each pair-loop variant is inlined once)
       float epot = 0.0f, ekin = 0.0f;
-      float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      const bool do_w = last && a.want_virial;
-#define MD_PHASE(M, V, O)                                                                     \
+      for (int pass = 0; pass < (npt ? 2 : 1); ++pass) {
+        const bool kick_now = pass == 0;
+        // NPT needs the virial of every first pass (the barostat's pressure)
+        const bool do_w = (last && a.want_virial && pass + 1 == (npt ? 2 : 1)) || (npt && pass == 0);
+        float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        epot = 0.0f;
+#define MD_PHASE(M, V, O)                                                                      \
   md_force_phase<M, V, O>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0, rc2_lo, \
-                          rc2_hi, hdt, dtf, last, csvr, epot, ekin, w)
-      if (ns == 1) {
-        if (do_w) { if (ortho) MD_PHASE(false, true, true); else MD_PHASE(false, true, false); }
-        else { if (ortho) MD_PHASE(false, false, true); else MD_PHASE(false, false, false); }
-      } else {
-        if (do_w) { if (ortho) MD_PHASE(true, true, true); else MD_PHASE(true, true, false); }
-        else { if (ortho) MD_PHASE(true, false, true); else MD_PHASE(true, false, false); }
-      }
+                          rc2_hi, hdt, dtf, last, csvr, kick_now, epot, ekin, w)
+        if (ns == 1) {
+          if (do_w) { if (ortho) MD_PHASE(false, true, true); else MD_PHASE(false, true, false); }
+          else { if (ortho) MD_PHASE(false, false, true); else MD_PHASE(false, false, false); }
+        } else {
+          if (do_w) { if (ortho) MD_PHASE(true, true, true); else MD_PHASE(true, true, false); }
+          else { if (ortho) MD_PHASE(true, false, true); else MD_PHASE(true, false, false); }
+        }
 #undef MD_PHASE
+        if (kick_now && (a.kin || csvr)) {
+          double e = block_sum((double)ekin, red);
+          if (tid == 0 && a.kin) a.kin[(size_t)s * total + st] = e;
+          kin_prev = e;
+        }
+        // the system virial (valid in thread 0): the output of the last
+        // step's final pass, and the pressure of the barostat
+        double ptr_w = 0.0;
+        if (do_w) {
+          double wsys[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) wsys[c] = block_sum((double)w[c], red);
+          ptr_w = wsys[0] + wsys[3] + wsys[5];
+          if (tid == 0 && last && a.want_virial && a.virial && pass + 1 == (npt ? 2 : 1)) {
+            double* W = a.virial + 9 * (size_t)s;
+            W[0] = wsys[0]; W[1] = wsys[1]; W[2] = wsys[2];
+            W[3] = wsys[1]; W[4] = wsys[3]; W[5] = wsys[4];
+            W[6] = wsys[2]; W[7] = wsys[4]; W[8] = wsys[5];
+          }
+        }
+        if (npt && pass == 0) {
+          // ---- stochastic cell rescaling (Bernetti and Bussi 2020;
+          // md/integrators.py:657-811): P = 2K / (3V) + tr(W) / (3V),
+          // d eps = (dt / tau_P) beta (P - P0) + sqrt(2 kT beta dt / (tau_P V)) R
+          // with ONE R ~ N(0, 1) for the batch, mu = exp(d eps / 3) clipped to
+          // [mu_min, 1 / mu_min]; cell and positions scaled by mu (momenta
+          // kept), then the second pass at the scaled state
+          if (tid == 0) {
+            const double* bp = a.baro + 4 * (size_t)s;
+            const double V = fabs(s_tril[0] * s_tril[2] * s_tril[5]);
+            const double P = 2.0 * kin_prev / (3.0 * V) + ptr_w / (3.0 * V);
+            uint32_t c4[4];
+            csvr_draw(a.seed, 0xFFFFFFFEu, a.step0 + st, SCR_TAG, c4);
+            const double R = csvr_bm(c4[0], c4[1]);
+            const double dt = a.dt[s], kT = a.kT[s];
+            const double de = dt / bp[1] * bp[2] * (P - bp[0]) +
+                              sqrt(2.0 * kT * bp[2] * dt / (bp[1] * V)) * R;
+            const double mu = fmin(fmax(exp(de / 3.0), bp[3]), 1.0 / bp[3]);
+            s_mu = mu;
+            for (int c = 0; c < 6; ++c) s_tril[c] *= mu;
+            make_geom(s_tril, g.rc, MD_MAX_NC_AXIS, g, MD_SUB);
+          }
+          __syncthreads();
+          asm volatile("" ::: "memory");
+          const float mu = (float)s_mu;
+          const int nx = g.nc[0], ny = g.nc[1], nz = g.nc[2];
+#pragma unroll
+          for (int k = 0; k < MD_SLOTS; ++k) {
+            const int slot = tid + k * MD_THREADS;
+            cel[k] = 0;
+            if (slot >= n) continue;
+            float4 r = sp[slot];
+            r.x *= mu; r.y *= mu; r.z *= mu;
+            float f0, f1, f2;
+            wrap_shift(g.L, g.Lif, r.x, r.y, r.z, f0, f1, f2);
+            cel[k] = (cell_index(f2, nz) * ny + cell_index(f1, ny)) * nx + cell_index(f0, nx);
+            r.w = __int_as_float((__float_as_int(r.w) & 0xff) | (cel[k] << 8));
+            sp[slot] = r;
+          }
+          sort_cells(cel, g.ncell);
+        }
+      }
       if (a.pot) {
         double e = block_sum((double)epot, red);
         if (tid == 0) a.pot[(size_t)s * total + st] = e;
       }
-      if (a.kin || csvr) {
-        double e = block_sum((double)ekin, red);
-        if (tid == 0 && a.kin) a.kin[(size_t)s * total + st] = e;
-        kin_prev = e;
-      }
-      if (do_w && a.virial) {
-        double out[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) out[c] = block_sum((double)w[c], red);
-        if (tid == 0) {
-          double* W = a.virial + 9 * (size_t)s;
-          W[0] = out[0]; W[1] = out[1]; W[2] = out[2];
-          W[3] = out[1]; W[4] = out[3]; W[5] = out[4];
-          W[6] = out[2]; W[7] = out[4]; W[8] = out[5];
-        }
-      }
     }
+    if (npt && tid == 0)
+      for (int c = 0; c < 6; ++c) a.tril_out[6 * (size_t)s + c] = s_tril[c];
     // ---- store the system back in its original atom order (forces were
     // written by the last force pass; every slot is stored by the thread
     // that kicked it)
@@ -783,8 +851,10 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
   KUPSMI_REQUIRE(ctx != nullptr, "null context");
   KUPSMI_REQUIRE(a.n_sys >= 0, "n_sys < 0");
   KUPSMI_REQUIRE(a.n_species >= 1 && a.n_species <= MD_NSPEC, "n_species must be in [1, 8]");
-  KUPSMI_REQUIRE(a.integrator >= 0 && a.integrator <= 2,
-                 "integrator must be 0 (VV), 1 (BAOAB) or 2 (CSVR)");
+  KUPSMI_REQUIRE(a.integrator >= 0 && a.integrator <= 3,
+                 "integrator must be 0 (VV), 1 (BAOAB), 2 (CSVR) or 3 (CSVR + cell rescaling)");
+  // the barostat rebuilds the cell list every step: no Verlet lists
+  if (a.integrator == 3) a.skin = 0.0f;
   KUPSMI_REQUIRE(a.n_steps >= 0, "n_steps < 0");
   if (a.n_sys == 0) return KUPSMI_OK;
   void* buf = nullptr;
@@ -815,7 +885,12 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
     if (rc) return rc;
     a.nblist = static_cast<unsigned short*>(lb);
   }
-  md_lj_kernel<<<grid, MD_THREADS, 0, stream>>>(a);
+  if (a.integrator == 3)
+    md_lj_kernel<2><<<grid, MD_THREADS, 0, stream>>>(a);
+  else if (a.integrator == 2)
+    md_lj_kernel<1><<<grid, MD_THREADS, 0, stream>>>(a);
+  else
+    md_lj_kernel<0><<<grid, MD_THREADS, 0, stream>>>(a);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }
@@ -831,22 +906,24 @@ extern "C" int kupsmi_md_lj(kupsmi_ctx* ctx, void* stream, int integrator, int n
                             const double* sigma, const double* epsilon, const double* tril,
                             const double* cutoff, double skin, int max_neighbors,
                             const double* dt, const double* kT, const double* gamma,
-                            const double* tau, double* pot_energy, double* kin_energy,
-                            double* virial) {
+                            const double* tau, const double* baro, double* tril_out,
+                            double* pot_energy, double* kin_energy, double* virial) {
   KUPSMI_REQUIRE(n_atoms >= 0, "n_atoms < 0");
   KUPSMI_REQUIRE(n_steps == 0 || (dt != nullptr), "dt required");
   KUPSMI_REQUIRE(skin >= 0.0 && max_neighbors >= 0, "skin and max_neighbors must be >= 0");
   KUPSMI_REQUIRE(skin == 0.0 || (max_neighbors > 0 && max_neighbors <= 32767),
                  "a skin needs max_neighbors in [1, 32767]");
   KUPSMI_REQUIRE(integrator != 1 || n_steps == 0 || (kT && gamma), "kT and gamma required for BAOAB");
-  KUPSMI_REQUIRE(integrator != 2 || n_steps == 0 || (kT && tau), "kT and tau required for CSVR");
+  KUPSMI_REQUIRE(integrator < 2 || n_steps == 0 || (kT && tau), "kT and tau required for CSVR");
+  KUPSMI_REQUIRE(integrator != 3 || n_steps == 0 || (baro && tril_out),
+                 "baro and tril_out required for cell rescaling");
   MdArgs a{};
   a.integrator = integrator; a.n_steps = n_steps; a.n_sys = n_sys; a.n_species = n_species;
   a.want_virial = virial != nullptr;
   a.step0 = step0; a.seed = seed;
   a.sys_off = sys_off; a.pos = pos; a.mom = mom; a.forces = forces; a.mass = mass;
   a.species = species; a.sigma = sigma; a.epsilon = epsilon; a.tril = tril; a.cutoff = cutoff;
-  a.dt = dt; a.kT = kT; a.gamma = gamma; a.tau = tau;
+  a.dt = dt; a.kT = kT; a.gamma = gamma; a.tau = tau; a.baro = baro; a.tril_out = tril_out;
   a.pot = pot_energy; a.kin = kin_energy; a.virial = virial;
   a.skin = (float)skin; a.nbmax = max_neighbors;
   return md_launch(ctx, (hipStream_t)stream, a);

@@ -3,8 +3,9 @@
 Momenta are drawn from Maxwell-Boltzmann, p ~ N(0, sqrt(m kT)) per component,
 minus the mean momentum of the system (171-178); gradients start at zero
 (188), so the first half kick of the first step uses F = 0 unless the caller
-evaluates forces first.  Time step, friction and the CSVR coupling time are
-converted from fs and 1/fs to internal units (195-205).
+evaluates forces first.  Time step, friction, the CSVR and barostat coupling
+times, the target pressure and the compressibility are converted from fs,
+1/fs, Pa and 1/Pa to internal units (195-215).
 """
 
 from __future__ import annotations
@@ -16,7 +17,7 @@ import torch
 
 from .._lib import require_gpu
 from ..core.batch import offsets_from_sizes
-from ..core.constants import BOLTZMANN_CONSTANT, FEMTO_SECOND
+from ..core.constants import BOLTZMANN_CONSTANT, FEMTO_SECOND, PASCAL
 
 
 @dataclass
@@ -36,6 +37,9 @@ class MDState:
     step: int = 0
     # (n_sys,) f64, CSVR coupling time tau, internal units
     thermostat_time_constant: torch.Tensor | None = None
+    # (n_sys, 4) f64, stochastic cell rescaling: target pressure (eV/A^3),
+    # coupling time (internal units), compressibility (A^3/eV), minimum scale
+    barostat: torch.Tensor | None = None
     charges: torch.Tensor | None = None  # (n,) f32, for Ewald systems
     extras: dict = field(default_factory=dict)
 
@@ -69,7 +73,9 @@ def maxwell_boltzmann_momenta(masses: np.ndarray, sizes, temperature, rng: np.ra
 
 def make_md_state(positions, masses, species, sizes, tril, cutoff, *, time_step_fs,
                   temperature, friction_per_fs=0.0, thermostat_time_constant_fs=100.0,
-                  momenta=None, charges=None, device=None) -> MDState:
+                  target_pressure_pa=1.0e5, pressure_coupling_time_fs=1000.0,
+                  compressibility_per_pa=2.0e-9, minimum_scale_factor=0.9, momenta=None,
+                  charges=None, device=None) -> MDState:
     dev = device or require_gpu()
     n_sys = len(sizes)
     n = int(np.sum(sizes))
@@ -95,6 +101,11 @@ def make_md_state(positions, masses, species, sizes, tril, cutoff, *, time_step_
         friction=per_sys(np.asarray(friction_per_fs, np.float64) / FEMTO_SECOND),
         thermostat_time_constant=per_sys(
             np.asarray(thermostat_time_constant_fs, np.float64) * FEMTO_SECOND),
+        barostat=torch.stack([
+            per_sys(np.asarray(target_pressure_pa, np.float64) * PASCAL),
+            per_sys(np.asarray(pressure_coupling_time_fs, np.float64) * FEMTO_SECOND),
+            per_sys(np.asarray(compressibility_per_pa, np.float64) / PASCAL),
+            per_sys(minimum_scale_factor)], dim=1).contiguous(),
         charges=None if charges is None else
         torch.as_tensor(np.asarray(charges, np.float32), device=dev).contiguous(),
     )

@@ -27,6 +27,7 @@ from .data import MDState
 VELOCITY_VERLET = 0
 BAOAB_LANGEVIN = 1
 CSVR = 2
+CSVR_NPT = 3
 
 
 @dataclass
@@ -103,6 +104,8 @@ class FusedLJPropagator:
             _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(state.tril), _lib.ptr(state.cutoff),
             skin, cap, _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction),
             _lib.ptr(state.thermostat_time_constant),
+            _lib.ptr(state.barostat) if self.integrator == CSVR_NPT else None,
+            _lib.ptr(state.tril) if self.integrator == CSVR_NPT else None,
             _lib.ptr(pot), _lib.ptr(kin), _lib.ptr(vir)))
         state.step += n_steps
         if not record:
@@ -131,6 +134,14 @@ def make_csvr_step(params: LennardJonesParameters, seed: int = 0,
     R1 ~ N(0, 1) for the batch and R2 ~ chi^2(3N - 4) per system, from
     Philox4x32-10 (seed, step); coupling time from the state."""
     return FusedLJPropagator(params, CSVR, seed, skin)
+
+
+def make_csvr_npt_step(params: LennardJonesParameters, seed: int = 0) -> FusedLJPropagator:
+    """NPT: CSVR, velocity Verlet, stochastic cell rescaling of cell and
+    positions, forces again (md/integrators.py:829-888; Bernetti and Bussi
+    2020); one R ~ N(0, 1) per step for the batch.  The cell of the state
+    (state.tril) is updated in place; the cell list is rebuilt every step."""
+    return FusedLJPropagator(params, CSVR_NPT, seed, 0.0)
 
 
 def normals(seed: int, step: int, n_atoms: int, device=None) -> torch.Tensor:

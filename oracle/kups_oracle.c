@@ -836,6 +836,43 @@ void orc_csvr_scale(uint64_t seed, int64_t t, int nsys, const int32_t* sys_off, 
   }
 }
 
+/* Stochastic cell rescaling (md/integrators.py:657-811; Bernetti and Bussi,
+ * J. Chem. Phys. 153, 114107 (2020)) of every system at global step t:
+ *   P = 2K / (3V) + tr(W) / (3V)   (W the virial, K from the momenta),
+ *   d eps = (dt / tau_P) beta (P - P0) + sqrt(2 kT beta dt / (tau_P V)) R,
+ *   mu = clip(exp(d eps / 3), mu_min, 1 / mu_min), L <- mu L, r <- mu r,
+ * with ONE R ~ N(0, 1) for the batch: Box-Muller of words 0, 1 of
+ * philox({0xFFFFFFFE, t, t >> 32, SCR_TAG}).  baro[4 s..] = P0, tau_P, beta,
+ * mu_min; mu[s] receives the factor. */
+#define SCR_TAG 0x53435231u
+
+void orc_scr_scale(uint64_t seed, int64_t t, int nsys, const int32_t* sys_off, const double* mass,
+                   const double* mom, double* pos, double* tril, const double* kT,
+                   const double* dt, const double* baro, const double* virial, double* mu) {
+  uint32_t c[4];
+  gc_draw(seed, 0xFFFFFFFEu, t, SCR_TAG, c);
+  const double R = csvr_bm(c[0], c[1]);
+  for (int s = 0; s < nsys; ++s) {
+    double K = 0.0;
+    for (int i = sys_off[s]; i < sys_off[s + 1]; ++i)
+      for (int a = 0; a < 3; ++a) K += 0.5 * mom[3 * i + a] * mom[3 * i + a] / mass[i];
+    double* L = tril + 6 * s;
+    const double* bp = baro + 4 * s;
+    const double V = fabs(L[0] * L[2] * L[5]);
+    const double* W = virial + 9 * s;
+    const double P = 2.0 * K / (3.0 * V) + (W[0] + W[4] + W[8]) / (3.0 * V);
+    const double de = dt[s] / bp[1] * bp[2] * (P - bp[0]) +
+                      sqrt(2.0 * kT[s] * bp[2] * dt[s] / (bp[1] * V)) * R;
+    double m = exp(de / 3.0);
+    if (m < bp[3]) m = bp[3];
+    if (m > 1.0 / bp[3]) m = 1.0 / bp[3];
+    for (int k = 0; k < 6; ++k) L[k] *= m;
+    for (int i = sys_off[s]; i < sys_off[s + 1]; ++i)
+      for (int a = 0; a < 3; ++a) pos[3 * i + a] *= m;
+    if (mu) mu[s] = m;
+  }
+}
+
 /* Shoemake random quaternion (core/utils/quaternion.py:70-97), the power q^w
  * (_pow_quaternion: sign fix w >= 0, angle scaled) and the rotation matrix
  * (_quat_to_mat), applied as row-vector @ R (moves.py:190). */

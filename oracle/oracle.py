@@ -204,6 +204,21 @@ def csvr_scale(seed, t, mom, mass, sys_off, kT, dt, tau):
     return alpha
 
 
+def scr_scale(seed, t, mom, pos, tril, mass, sys_off, kT, dt, baro, virial):
+    """Stochastic cell rescaling of FP64 positions and cells in place at
+    global step t (virial W per system, (n_sys, 3, 3)); returns mu per system."""
+    for a in (mom, pos, tril):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    mass, sys_off, kT, dt, baro = _d(mass), _i(sys_off), _d(kT), _d(dt), _d(baro)
+    virial = _d(np.asarray(virial).reshape(-1, 9))
+    nsys = len(sys_off) - 1
+    mu = np.zeros(nsys)
+    lib().orc_scr_scale(C.c_uint64(int(seed) & (2**64 - 1)), C.c_int64(t), nsys, _p(sys_off),
+                        _p(mass), _p(mom), _p(pos), _p(tril), _p(kT), _p(dt), _p(baro),
+                        _p(virial), _p(mu))
+    return mu
+
+
 def md_vv_f32(n_steps, pos, mom, mass, species, sys_off, tril, cutoff, sigma, eps, dt, forces):
     """n_steps FP32 velocity-Verlet LJ steps in place (the FP32 CPU baseline)."""
     for a in (pos, mom, forces):

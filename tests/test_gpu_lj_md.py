@@ -17,8 +17,8 @@ from kups_amd.core.batch import ParticleBatch, SystemBatch
 from kups_amd.core.neighborlist import CellListNeighborList
 from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
 from kups_amd.md.integrators import (VELOCITY_VERLET, FusedLJPropagator,
-                                     make_baoab_langevin_step, make_csvr_step,
-                                     make_velocity_verlet_step, normals)
+                                     make_baoab_langevin_step, make_csvr_npt_step,
+                                     make_csvr_step, make_velocity_verlet_step, normals)
 from kups_amd.potential.lennard_jones import LennardJonesParameters, LennardJonesPotential
 
 pytestmark = pytest.mark.gpu
@@ -338,6 +338,66 @@ def test_md_csvr_reaches_the_target_temperature(gpu):
     ndof = 3 * b.sizes - 3
     t_kin = 2 * kin.mean(axis=1) / (ndof * BOLTZMANN_CONSTANT)
     np.testing.assert_allclose(t_kin, 110.0, rtol=0.03)
+
+
+def _npt_state(b, **kw):
+    p0 = maxwell_boltzmann_momenta(b.masses, b.sizes, 94.4, np.random.default_rng(8))
+    return make_md_state(b.positions.astype(np.float32), b.masses, b.species, b.sizes, b.tril,
+                         workloads.AR_CUTOFF, time_step_fs=2.0, temperature=94.4,
+                         thermostat_time_constant_fs=50.0, momenta=p0.astype(np.float32), **kw)
+
+
+def test_md_npt_vs_oracle(gpu):
+    """CSVR + VV + stochastic cell rescaling + forces at the rescaled state,
+    against the oracle's CSVR step, FP64 VV, cell rescaling (same Philox draw
+    R shared by the batch) and a fresh LJ evaluation, step by step; the cell
+    is written back to the state."""
+    b = _lj_batch([864, 500, 300], seed=9)
+    # a strong coupling so the cells move visibly in a few steps
+    st = _npt_state(b, target_pressure_pa=5.0e7, pressure_coupling_time_fs=40.0,
+                    compressibility_per_pa=5.0e-9)
+    prop = make_csvr_npt_step(AR, seed=17)
+    prop.evaluate_forces(st)
+    pos = _f32(st.positions.cpu().numpy()); mom = _f32(st.momenta.cpu().numpy())
+    frc = np.ascontiguousarray(_f32(st.forces.cpu().numpy()))
+    tril = np.ascontiguousarray(b.tril.astype(np.float64))
+    mass = b.masses.astype(np.float32).astype(np.float64)
+    cut = np.full(b.n_sys, workloads.AR_CUTOFF)
+    dt = st.time_step.cpu().numpy(); kT = st.kT.cpu().numpy()
+    tau = st.thermostat_time_constant.cpu().numpy(); baro = st.barostat.cpu().numpy()
+    step0 = st.step
+    n = 6
+    obs = prop(st, n)
+    eta = np.zeros_like(pos)
+    mus = []
+    for k in range(n):
+        oracle.csvr_scale(17, step0 + k, mom, mass, b.sys_off, kT, dt, tau)
+        _, w = oracle.md_step(0, pos, mom, mass, b.species, b.sys_off, tril, cut, AR.sigma,
+                              AR.epsilon, dt, dt * 0, dt * 0, eta, frc)
+        mus.append(oracle.scr_scale(17, step0 + k, mom, pos, tril, mass, b.sys_off, kT, dt, baro, w))
+        e, f, _ = oracle.lj_cells(pos, b.species, b.sys_off, tril, cut, AR.sigma, AR.epsilon)
+        frc[:] = f
+        _check_energy(obs.potential_energy[:, k].cpu().numpy(), e, b.sizes, 1e-5)
+    assert np.abs(np.log(np.array(mus))).max() > 1e-5  # the cells did move
+    np.testing.assert_allclose(st.tril.cpu().numpy(), tril, rtol=1e-6)
+    _check_forces(st.forces.cpu().numpy(), frc, tol=1e-4)
+    _check_forces(st.momenta.cpu().numpy(), mom, tol=1e-4)
+
+
+def test_md_npt_without_compressibility_is_csvr(gpu):
+    """beta = 0: the cell never moves and the run is the CSVR run (cell list
+    every step in both), bit for bit."""
+    b = _lj_batch([864, 300], seed=10)
+    runs = []
+    for make in (lambda: make_csvr_npt_step(AR, seed=3), lambda: make_csvr_step(AR, seed=3, skin=0.0)):
+        st = _npt_state(b, compressibility_per_pa=0.0)
+        prop = make()
+        prop.evaluate_forces(st)
+        obs = prop(st, 15)
+        runs.append((st.positions.cpu().numpy(), st.momenta.cpu().numpy(),
+                     obs.potential_energy.cpu().numpy(), st.tril.cpu().numpy()))
+    for a_, b_ in zip(runs[0], runs[1]):
+        np.testing.assert_array_equal(a_, b_)
 
 
 def test_normals_statistics(gpu):
