@@ -260,6 +260,26 @@ __device__ inline double csvr_alpha(double K, int n_atoms, double kT, double dt,
   return sqrt(a2 > 0.0 ? a2 : 0.0);
 }
 
+// erfc(x), x >= 0, by the Chebyshev fit of Press, Teukolsky, Vetterling and
+// Flannery, Numerical Recipes in C, 2nd ed. (1992), section 6.2 (erfcc):
+// fractional error below 1.2e-7 in exact arithmetic.  In FP32 the rounding of
+// the exponent (about -13 at x = 3.6, the usual alpha rc) adds ~1e-6
+// relative where erfc itself is ~4e-7, i.e. ~1e-12 eV per pair.
+__device__ inline float erfc_cheb(float x) {
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * x);
+  float p = 0.17087277f;
+  p = fmaf(p, t, -0.82215223f);
+  p = fmaf(p, t, 1.48851587f);
+  p = fmaf(p, t, -1.13520398f);
+  p = fmaf(p, t, 0.27886807f);
+  p = fmaf(p, t, -0.18628806f);
+  p = fmaf(p, t, 0.09678418f);
+  p = fmaf(p, t, 0.37409196f);
+  p = fmaf(p, t, 1.00002368f);
+  p = fmaf(p, t, -1.26551223f);
+  return t * __expf(p - x * x);
+}
+
 // ---------------------------------------------------------- reductions ----
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

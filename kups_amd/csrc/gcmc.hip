@@ -351,7 +351,7 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
                                a.motif_pos[3 * tid + 2], a.motif_q[tid]);
     s_mspec[tid] = a.motif_spec[tid];
   }
-  const float rclj2 = (float)(a.rc_lj * a.rc_lj), rccl = (float)a.rc_cl;
+  const float rclj2 = (float)(a.rc_lj * a.rc_lj), rccl2 = (float)(a.rc_cl * a.rc_cl);
   const float K = 14.399645351950548f;
 
   for (int ch = blockIdx.x; ch < a.n_chain; ch += gridDim.x) {
@@ -488,15 +488,15 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
             float d[3];
             gc_minimg(cell, riv, rjv, d);
             const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            float e = 0.f;
-            if (r2 < rclj2) {
-              const int p = si * ns + sj;
-              float c6 = s_sig2[p] / r2;
-              c6 = c6 * c6 * c6;
-              e += s_e4[p] * c6 * (c6 - 1.0f);
-            }
-            const float r = sqrtf(r2);
-            if (r < rccl) e += K * ri.w * rj.w * erfcf(alpha * r) / r;
+            // each cutoff test folded into its 1/r factor (zero outside)
+            const int p = si * ns + sj;
+            const float ir2 = r2 < rclj2 ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+            float c6 = s_sig2[p] * ir2;
+            c6 = c6 * c6 * c6;
+            float e = s_e4[p] * c6 * (c6 - 1.0f);
+            const float ir = __builtin_amdgcn_rsqf(r2);
+            const float irc = r2 < rccl2 ? ir : 0.0f;
+            e += K * ri.w * rj.w * erfc_cheb(alpha * (r2 * irc)) * irc;
             if (is_new) e_new += e; else e_old += e;
           }
         }

@@ -216,10 +216,13 @@ __device__ inline void md_lj_terms(float ex, float ey, float ez, float r2, bool 
 // per axis, kept when |(f_i - f_j - s) @ L|^2 < rc^2 (the reference's
 // fractional formula, core/neighborlist.py:542-564, 589).  Out of line: the
 // rare call claims no registers in the pair loops.
-__device__ __attribute__((noinline)) bool md_band_nearest(const SysGeom& g, float4 ri, float4 rj) {
+// Positions are passed as x, y, z only, so the pair loop's loads of the
+// neighbour need not fetch the w word.
+__device__ __attribute__((noinline)) bool md_band_nearest(const SysGeom& g, float xi, float yi,
+                                                          float zi, float xj, float yj, float zj) {
   double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
-  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
-  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
+  tmul(g.Li, (double)xi, (double)yi, (double)zi, fi0, fi1, fi2);
+  tmul(g.Li, (double)xj, (double)yj, (double)zj, fj0, fj1, fj2);
   const double t0 = fi0 - fj0, t1 = fi1 - fj1, t2 = fi2 - fj2;
   tmul(g.L, t0 - rint(t0), t1 - rint(t1), t2 - rint(t2), d0, d1, d2);
   return d0 * d0 + d1 * d1 + d2 * d2 < g.rc2;
@@ -404,7 +407,8 @@ __device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const 
       const bool valid = 4 * q + u < cnt;
       const bool lo = r2 < rc2_lo;
       bool keep = valid && lo;
-      if (__builtin_expect(valid && !lo && r2 < rc2_hi, 0)) keep = md_band_nearest(g, ri, rj[u]);
+      if (__builtin_expect(valid && !lo && r2 < rc2_hi, 0))
+        keep = md_band_nearest(g, ri.x, ri.y, ri.z, rj[u].x, rj[u].y, rj[u].z);
       const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj[u].w) & 0xff)] : p0;
       md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
     }

@@ -225,26 +225,6 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
   if (cnt > a.cap) atomicAdd(a.counters + 1, 1ull);
 }
 
-// erfc(x), x >= 0, by the Chebyshev fit of Press, Teukolsky, Vetterling and
-// Flannery, Numerical Recipes in C, 2nd ed. (1992), section 6.2 (erfcc):
-// fractional error below 1.2e-7 in exact arithmetic.  In FP32 the rounding of
-// the exponent (about -13 at x = 3.6, the usual alpha rc) adds ~1e-6
-// relative where erfc itself is ~4e-7, i.e. ~1e-12 eV per pair.
-__device__ inline float vl_erfc(float x) {
-  const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * x);
-  float p = 0.17087277f;
-  p = fmaf(p, t, -0.82215223f);
-  p = fmaf(p, t, 1.48851587f);
-  p = fmaf(p, t, -1.13520398f);
-  p = fmaf(p, t, 0.27886807f);
-  p = fmaf(p, t, -0.18628806f);
-  p = fmaf(p, t, 0.09678418f);
-  p = fmaf(p, t, 0.37409196f);
-  p = fmaf(p, t, 1.00002368f);
-  p = fmaf(p, t, -1.26551223f);
-  return t * __expf(p - x * x);
-}
-
 // FP64 re-decision of a pair within VL_BAND32 of a cutoff (the reference's
 // fractional formula).  Out of line: the rare call claims no registers in
 // the list loop.
@@ -298,7 +278,7 @@ __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)
     if (EXCL && pp.has_coul_excl && min_img && xcl_j == xcl_i) keep = false;
     const float ir = keep ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     const float x = pp.alpha * (r2 * ir);
-    const float ec = vl_erfc(x);
+    const float ec = erfc_cheb(x);
     const float qq = ri.w * rj.w * K;
     ecl += qq * ec * ir;
     // dU/dr = qq (-2a/sqrt(pi) e^{-a^2 r^2} - erfc / r) / r; u'/r = dU/dr / r
