@@ -356,7 +356,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every configuration is a batch of independent "
-              "systems and this build runs no multi-GPU path (see DESIGN.md, sharding)",
+              "systems, each resident on one GPU, and there is no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -1,0 +1,142 @@
+"""``kups_mcmc_rigid`` on the HIP GCMC kernel (application/simulations/mcmc_rigid.py).
+
+``build`` turns a YAML deck into a ``RigidGCMC`` batch: every host of the deck
+(CIF, optional replication) becomes ``replicas`` independent chains with the
+deck's pressure, temperature, move weights, LJ table, Ewald settings and
+initial loading; ``run`` does the warm-up and production cycles of
+mcmc_rigid.py:299-305 (a cycle is max(largest loading of the batch,
+min_cycle_length) trial moves) and samples the energy and loading after every
+production cycle, with S(k) and the running energy re-evaluated in full
+between warm-up and production (keeping the reference's tracked-energy
+composition); ``analyze`` reduces the samples per host as
+application/mcmc/analysis.py does, replicas combined.
+
+One adsorbate species per deck (the kernel's rigid-molecule buffer holds one
+motif).  The arithmetic is all in the kernel; this module only assembles
+inputs and reads the per-cycle samples.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ..mcmc.gcmc import Motif, RigidGCMC
+from ..potential.lennard_jones import LennardJonesParameters
+from .analysis import MCMCAnalysisResult, analyze_mcmc, combine_mcmc
+from .cif import Structure, make_supercell, read_cif
+from .config import McmcConfig
+
+
+def host_structure(host) -> Structure:
+    s = read_cif(host.cif_file)
+    if host.unitcell_replication is not None:
+        s = make_supercell(s, host.unitcell_replication)
+    return s
+
+
+def _motif(ads) -> Motif:
+    q = ads.charges if ads.charges else [0.0] * len(ads.symbols)
+    return Motif.from_dict({"positions": ads.positions, "symbols": ads.symbols, "charges": q,
+                            "critical_temperature": ads.critical_temperature,
+                            "critical_pressure": ads.critical_pressure,
+                            "acentric_factor": ads.acentric_factor})
+
+
+def lj_table(config_lj, labels) -> LennardJonesParameters:
+    """The deck's LJ table restricted to the labels in use, in the table's
+    order (mixing is per pair, so the restriction changes no value)."""
+    used = set(labels)
+    missing = sorted(used - set(config_lj.parameters))
+    if missing:
+        raise ValueError(f"labels {missing} have no Lennard-Jones parameters")
+    params = {k: v for k, v in config_lj.parameters.items() if k in used}
+    return LennardJonesParameters.from_dict(config_lj.cutoff, params, config_lj.mixing_rule)
+
+
+@dataclass
+class McmcRun:
+    gcmc: RigidGCMC
+    config: McmcConfig
+    replicas: int
+    structures: list
+    energy: np.ndarray | None = None  # (n_cycles, n_chain) eV
+    loading: np.ndarray | None = None  # (n_cycles, n_chain)
+
+
+def build(config: McmcConfig, *, replicas: int = 1, seed: int | None = None,
+          device=None) -> McmcRun:
+    if len(config.adsorbates) != 1:
+        raise ValueError("the rigid GCMC kernel runs one adsorbate species per deck")
+    ads = config.adsorbates[0]
+    motif = _motif(ads)
+    structs = [host_structure(h) for h in config.hosts]
+    labels = [lab for s in structs for lab in s.labels] + list(motif.species)
+    lj = lj_table(config.lj, labels)
+    index = {lab: i for i, lab in enumerate(lj.labels)}
+    guest_species = tuple(index[sym] for sym in motif.species)
+    pos, q, sp, sizes, lats, pres, temps, n_init = [], [], [], [], [], [], [], []
+    for h, s in zip(config.hosts, structs):
+        for _ in range(replicas):
+            pos.append(s.positions)
+            q.append(s.charges)
+            sp.append(np.array([index[lab] for lab in s.labels], np.int32))
+            sizes.append(len(s))
+            lats.append(s.lattice)
+            pres.append(float(h.pressure))
+            temps.append(float(h.temperature))
+            n_init.append(int(h.init_adsorbates[0]) if h.init_adsorbates else 0)
+    run_seed = config.run.seed if seed is None else seed
+    run_seed = 0 if run_seed is None else int(run_seed)
+    gcmc = RigidGCMC(np.concatenate(pos), np.concatenate(q), np.concatenate(sp), sizes,
+                     np.stack(lats), np.array(pres), np.array(temps), motif, lj, guest_species,
+                     ewald_precision=config.ewald.precision, ewald_cutoff=config.ewald.real_cutoff,
+                     weights=config.run.weights, gmax=int(config.max_num_adsorbates),
+                     seed=run_seed, tail_correction=bool(config.lj.tail_correction),
+                     device=device)
+    if any(n_init):
+        gcmc.insert_random(n_init, seed=run_seed)
+    return McmcRun(gcmc, config, replicas, structs)
+
+
+def _cycle_length(gcmc: RigidGCMC, min_len: int) -> int:
+    return max(int(gcmc.loading.max().item()) if gcmc.n_chain else 0, int(min_len))
+
+
+def run(r: McmcRun, *, num_warmup_cycles: int | None = None,
+        num_cycles: int | None = None) -> McmcRun:
+    """Warm-up then production cycles; per-cycle energy and loading kept."""
+    cfg = r.config.run
+    n_warm = cfg.num_warmup_cycles if num_warmup_cycles is None else int(num_warmup_cycles)
+    n_prod = cfg.num_cycles if num_cycles is None else int(num_cycles)
+    g = r.gcmc
+    for _ in range(n_warm):
+        g.run(_cycle_length(g, cfg.min_cycle_length))
+    if n_warm:
+        # random initial molecules may overlap: the early energies are huge and
+        # their FP32 pair terms leave rounding in the running FP64 energy
+        g.resync()
+    e = torch.empty((n_prod, g.n_chain), dtype=torch.float64, device=g.dev)
+    n = torch.empty((n_prod, g.n_chain), dtype=torch.int32, device=g.dev)
+    for c in range(n_prod):
+        g.run(_cycle_length(g, cfg.min_cycle_length))
+        e[c].copy_(g.d["energy"])
+        n[c].copy_(g.loading)
+    r.energy = e.cpu().numpy()
+    r.loading = n.cpu().numpy()
+    return r
+
+
+def analyze(r: McmcRun, n_blocks: int | None = None) -> list[MCMCAnalysisResult]:
+    """One result per host of the deck, its replicas combined."""
+    if r.energy is None:
+        raise ValueError("run the chains first")
+    out = []
+    for h_i, h in enumerate(r.config.hosts):
+        cols = range(h_i * r.replicas, (h_i + 1) * r.replicas)
+        res = [analyze_mcmc(r.energy[:, c], r.loading[:, c], float(h.temperature), n_blocks)
+               for c in cols]
+        out.append(combine_mcmc(res))
+    return out

@@ -69,6 +69,18 @@ def tail_matrix(lj: LennardJonesParameters, volume: float) -> np.ndarray:
     return T * (lj.epsilon > 0)
 
 
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    """Uniform rotation matrix from a uniform unit quaternion (Shoemake's
+    subgroup algorithm, Graphics Gems III), as Quaternion.random."""
+    u1, u2, u3 = rng.random(3)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    w, x, y, z = (a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                  b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3))
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
 class RigidGCMC:
     """GCMC of one rigid guest species in a batch of rigid hosts."""
 
@@ -76,7 +88,7 @@ class RigidGCMC:
                  pressure, temperature, motif: Motif, lj: LennardJonesParameters,
                  guest_species: tuple[int, ...], *, ewald_precision: float = 1e-6,
                  ewald_cutoff: float = 12.0, weights=(1.0, 1.0, 1.0, 3.0), gmax: int = 64,
-                 seed: int = 0, device=None):
+                 seed: int = 0, tail_correction: bool = True, device=None):
         dev = device or _lib.require_gpu()
         self.dev = dev
         self.motif = motif
@@ -135,6 +147,8 @@ class RigidGCMC:
         self.guest_species = np.asarray(guest_species, np.int32)
         self.mol_const = np.array([molecule_constant(motif, a) for a in self.alpha])
         self.tail = np.stack([tail_matrix(lj, v) for v in vol])
+        if not tail_correction:
+            self.tail[:] = 0.0
         counts = np.zeros((n, self.n_species), np.int32)
         hs = np.asarray(host_species, np.int32)
         for s in range(n):
@@ -251,8 +265,8 @@ class RigidGCMC:
         counts = self.d["counts"].cpu().numpy().astype(float)
         e_tail = np.einsum("sab,sa,sb->s", self.tail, counts, counts)
         # intramolecular exclusion correction of every guest molecule; the
-        # atomic view above already holds E_sr(intra) of each guest (the
-        # reference's composition, see the full-evaluation note in DESIGN)
+        # atomic view above already holds E_sr(intra) of each guest, as the
+        # reference's full evaluation does (ewald.py:946-967 over all pairs)
         e_excl = ng * (self.mol_const - self._self_mol())
         return (out.short_range + out.long_range + out.self_energy).cpu().numpy() + e_excl + \
             e2[:, 0].cpu().numpy() + e_tail
@@ -263,15 +277,35 @@ class RigidGCMC:
 
     def recompute_state(self):
         """Structure factors of the current configuration (GPU) and the tracked
-        energy set to the full evaluation."""
+        energy set to the full evaluation (the composition origin moves to the
+        current loading)."""
+        self._ng_ref = self.d["n_guest"].cpu().numpy().copy()
+        if self._structure_factors():
+            self.d["energy"].copy_(torch.as_tensor(self.full_energy(), device=self.dev))
+
+    def resync(self):
+        """Re-evaluate S(k) and the tracked energies in full while keeping the
+        composition of the incremental bookkeeping: a guest inserted since the
+        composition origin carries no intramolecular real-space term, one
+        removed keeps it (the reference's tracked-energy convention).  Clears
+        the rounding that FP32 pair terms leave in the FP64 running energy,
+        which matters after a start from overlapping molecules, where the
+        early energies are astronomically large."""
+        ng = self.d["n_guest"].cpu().numpy()
+        if self._structure_factors():
+            e = self.full_energy() - (ng - self._ng_ref) * self.sr_intramolecular()
+            self.d["energy"].copy_(torch.as_tensor(e, device=self.dev))
+
+    def _structure_factors(self) -> bool:
+        """S(k) of the current configuration into the chain state; False (and
+        S = 0, E = 0) when the batch holds no atom."""
         pos, q, sp, grp, sizes = self.configuration()
         dev = self.dev
         n = self.n_chain
-        self._ng_ref = self.d["n_guest"].cpu().numpy().copy()
         if len(pos) == 0:
             self.d["sk"].zero_()
             self.d["energy"].zero_()
-            return
+            return False
         off = offsets_from_sizes(sizes, dev)
         p = torch.as_tensor(pos, dtype=torch.float32, device=dev)
         qq = torch.as_tensor(q, dtype=torch.float32, device=dev)
@@ -284,7 +318,7 @@ class RigidGCMC:
             0, None, None, None, _lib.ptr(sk)))
         gi = torch.as_tensor(self._kgrid_index, device=dev)
         self.d["sk"].copy_(torch.gather(sk, 1, gi[:, :, None].expand(-1, -1, 2)))
-        self.d["energy"].copy_(torch.as_tensor(self.full_energy(), device=dev))
+        return True
 
     def energy_error(self) -> np.ndarray:
         """Tracked energy minus a full re-evaluation, per chain (eV).  The full
@@ -294,6 +328,32 @@ class RigidGCMC:
         ng = self.d["n_guest"].cpu().numpy()
         full = self.full_energy() - (ng - self._ng_ref) * self.sr_intramolecular()
         return self.d["energy"].cpu().numpy() - full
+
+    def insert_random(self, n_init, seed: int = 0):
+        """Place n_init[s] molecules in chain s, each at a uniform fractional
+        centre with a uniformly random orientation and no overlap test
+        (application/mcmc/data.py:293-340), then rebuild S(k) and the tracked
+        energies from a full evaluation."""
+        n_init = np.broadcast_to(np.asarray(n_init, np.int64), (self.n_chain,))
+        ng = self.d["n_guest"].cpu().numpy()
+        if np.any(ng + n_init > self.gmax):
+            raise ValueError(f"initial loading exceeds the {self.gmax}-molecule buffer")
+        rng = np.random.default_rng(seed)
+        guest = self.d["guest"].cpu().numpy()
+        counts = self.d["counts"].cpu().numpy()
+        lat = np.zeros((self.n_chain, 3, 3))
+        tr = self.d["tril"].cpu().numpy()
+        lat[:, [0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]] = tr
+        for s in range(self.n_chain):
+            for _ in range(int(n_init[s])):
+                com = rng.uniform(-0.5, 0.5, 3) @ lat[s]
+                guest[s, ng[s]] = com + self.motif.positions @ random_rotation(rng).T
+                ng[s] += 1
+                np.add.at(counts[s], self.guest_species, 1)
+        self.d["guest"].copy_(torch.as_tensor(guest))
+        self.d["n_guest"].copy_(torch.as_tensor(ng))
+        self.d["counts"].copy_(torch.as_tensor(counts))
+        self.recompute_state()
 
     _STATE = ("guest", "n_guest", "counts", "sk", "energy", "width", "hist", "hist_idx", "stats")
 

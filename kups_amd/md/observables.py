@@ -43,3 +43,29 @@ def instantaneous_pressure(kinetic: torch.Tensor, cauchy_stress: torch.Tensor,
     d = cauchy_stress.shape[-1]
     tr = torch.diagonal(cauchy_stress, dim1=-2, dim2=-1).sum(-1)
     return 2.0 * kinetic / (d * vol) + tr / d
+
+
+def lattice_gradient_stress(positions: torch.Tensor, forces: torch.Tensor, virial: torch.Tensor,
+                            tril: torch.Tensor, system: torch.Tensor) -> torch.Tensor:
+    """-(dU/dh) / V with the Cartesian positions held fixed, per system (n_sys, 3, 3).
+
+    This is the stress that the reference's MD application logs
+    (application/md/data.py:101-107, observables/stress.py:63-66): the lattice
+    vectors enter the energy only through the image shifts, so it is the box
+    part of the virial alone and depends on where the atoms sit (positions
+    are taken wrapped to [-0.5, 0.5) fractional, unitcell.py:139-171).  From the
+    virial theorem h^T dU/dh = -W + sum_i r_i (x) F_i with the kernels' pair
+    virial W, so dU/dh = h^-T (sum_i r_i (x) F_i - W).  The physical stress is
+    ``stress_via_virial``."""
+    n_sys = tril.shape[0]
+    L = torch.zeros((n_sys, 3, 3), dtype=torch.float64, device=tril.device)
+    L[:, [0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]] = tril.double()
+    Linv = torch.linalg.inv(L)
+    sys = system.long()
+    r = positions.double()
+    s = torch.einsum("ic,icd->id", r, Linv[sys])
+    s = s - torch.floor(s + 0.5)
+    rw = torch.einsum("ic,icd->id", s, L[sys])
+    rF = segment_sum((rw[:, :, None] * forces.double()[:, None, :]).reshape(-1, 9), system, n_sys)
+    G = Linv.transpose(-1, -2) @ (rF.reshape(n_sys, 3, 3) - virial.reshape(n_sys, 3, 3).double())
+    return -G / volume(tril).double()[:, None, None]
