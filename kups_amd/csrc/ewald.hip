@@ -10,7 +10,8 @@
 //   ek_sk_kernel      S(k) = sum_i q_i exp(i k.r_i) as an FP32 GEMM on the
 //                     matrix cores (v_mfma_f32_16x16x4_f32), one workgroup
 //                     per (system, 512-atom chunk), atoms streamed through LDS
-//                     in tiles of 64 with their per-axis phase tables;
+//                     in tiles of 8..64 (what 64 KB holds) with their per-axis
+//                     phase tables, operands k-contiguous for float4 fragments;
 //                     ek_sk_reduce_kernel sums the chunks in fixed order.
 //   ek_energy_kernel  P(k) (FP64), E = K sum_k P |S|^2, the reciprocal-space
 //                     virial, and S'(k) = P(k) S(k) for the force pass.
@@ -65,23 +66,25 @@ __device__ inline double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// c u^n for n = 0..m into out[0..m] (FP64 recurrence, rounded once).
-__device__ inline void powers64_pos(double2 u, double c, int m, float2* out) {
+// c u^n for n = 0..m into out[0], out[st], .., out[m st] (FP64 recurrence,
+// rounded once).
+__device__ inline void powers64_pos(double2 u, double c, int m, float2* out, int st = 1) {
   double2 p = make_double2(c, 0.0);
   for (int n = 0; n <= m; ++n) {
-    out[n] = make_float2((float)p.x, (float)p.y);
+    out[n * st] = make_float2((float)p.x, (float)p.y);
     p = zmul(p, u);
   }
 }
 
-// u^n for n = -m..m into out[0..2m] (n = 0 at out[m]; FP64, rounded once).
-__device__ inline void powers64(double2 u, int m, float2* out) {
-  out[m] = make_float2(1.f, 0.f);
+// u^n for n = -m..m into out[0..2m] with stride st (n = 0 at out[m st];
+// FP64, rounded once).
+__device__ inline void powers64(double2 u, int m, float2* out, int st = 1) {
+  out[m * st] = make_float2(1.f, 0.f);
   double2 p = make_double2(1.0, 0.0);
   for (int n = 1; n <= m; ++n) {
     p = zmul(p, u);
-    out[m + n] = make_float2((float)p.x, (float)p.y);
-    out[m - n] = make_float2((float)p.x, (float)-p.y);
+    out[(m + n) * st] = make_float2((float)p.x, (float)p.y);
+    out[(m - n) * st] = make_float2((float)p.x, (float)-p.y);
   }
 }
 
@@ -102,63 +105,69 @@ __device__ inline void powers(float2 u, int m, float2* out) {
 // stacked over (atom, re/im) on K = 2 x atoms and (re, im) of S on N = 2 wz:
 //   S_re = sum Re A Re Z - Im A Im Z,  S_im = sum Re A Im Z + Im A Re Z.
 // One workgroup per (system, chunk of EK_CHUNK atoms).  Per tile of TA atoms
-// the workgroup builds the per-axis phase tables (FP64, rounded once), packs
-// the GEMM operands K-major into LDS (A: 2 TA x M, B: 2 TA x N, leading
-// dimensions = 16 mod 64 floats so the four k rows of a fragment hit disjoint
-// banks), and the 4 waves, each owning every 4th 16-row tile of M and all N
-// tiles, run v_mfma_f32_16x16x4_f32 on fragments loaded straight from the
-// panels (A 16 x 4: lane -> row l % 16, k = l / 16; B 4 x 16: col l % 16,
-// k = l / 16; D: row 4 (l / 16) + r, col l % 16).  The FP32 sums of a tile
-// are added into FP64 registers; the chunk's partial S goes to HBM for a
-// fixed-order reduce.
+// the workgroup builds the per-axis phase tables (FP64, rounded once; stored
+// n-major so that two atoms are one 16-byte word), then packs both GEMM
+// operands k-contiguous into LDS: A as [m][k] and B as [column][k], row pitch
+// ldk = 2 TA + 4 floats (a quad count that is odd, so sixteen rows at the same
+// k land on sixteen disjoint quads of banks).  K is summed in blocks of 16:
+// lane (r = l % 16, q = l / 16) loads one float4 of A row r and one of B column
+// r at k = k0 + 4q .. k0 + 4q + 3, and component j feeds the j-th
+// v_mfma_f32_16x16x4_f32 of the block (each MFMA sums k0 + 4q + j over q; A and
+// B use the same k order, so the four MFMAs together sum the whole block).
+// D of an MFMA: lane l holds row 4 (l / 16) + r, column l % 16.  The 4 waves
+// each own every 4th 16-row tile of M and all N tiles; their FP32 sums are
+// added into FP64 registers after at most 64 atoms (K = 128), and the chunk's
+// partial S goes to HBM for a fixed-order reduce.
 constexpr int EK_SK_THREADS = 256;
 constexpr int EK_WAVES = EK_SK_THREADS / 64;
 constexpr int EK_CHUNK = 512;
 
 struct EkSkShape {
-  int ta, lda, ldb;  // atoms per tile, panel leading dimensions (floats)
-  size_t lds;        // dynamic LDS bytes
+  int ta, ldk;  // atoms per tile (multiple of 8), panel row pitch (floats)
+  size_t lds;   // dynamic LDS bytes
 };
-
-__host__ __device__ inline int ek_ld16(int n) {  // smallest ld >= n with ld % 64 == 16
-  int ld = n - (n % 64) + 16;
-  return ld < n ? ld + 64 : ld;
-}
 
 template <int MTW, int NT>
 __global__ __launch_bounds__(EK_SK_THREADS) void ek_sk_kernel(EkArgs a, EkSkShape sh) {
   extern __shared__ float ek_lds[];
   __shared__ double s_li[6];
-  const int TA = sh.ta, lda = sh.lda, ldb = sh.ldb;
-  float* apk = ek_lds;                          // [2 TA][lda]
-  float* bpk = apk + (size_t)2 * TA * lda;      // [2 TA][ldb]
-  float2* tx = reinterpret_cast<float2*>(bpk + (size_t)2 * TA * ldb);  // [TA][mx + 1]
+  const int TA = sh.ta, ldk = sh.ldk, TH = TA / 2;
   const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1, wx = a.mx + 1;
-  float2* ty = tx + (size_t)TA * wx;            // [TA][wy]
-  float2* tz = ty + (size_t)TA * wy;            // [TA][wz]
+  const int M = wx * wy, NC = 2 * wz;
+  const int mt_n = (M + 15) / 16;
+  float* apk = ek_lds;                                   // [16 mt_n][ldk]
+  float* bpk = apk + (size_t)16 * mt_n * ldk;            // [16 NT][ldk]
+  float2* tx = reinterpret_cast<float2*>(bpk + (size_t)16 * NT * ldk);  // [wx][TA]
+  float2* ty = tx + (size_t)wx * TA;                     // [wy][TA]
+  float2* tz = ty + (size_t)wy * TA;                     // [wz][TA]
   const int s = blockIdx.y, chunk = blockIdx.x;
   const int i0 = a.sys_off[s], n_all = a.sys_off[s + 1] - i0;
   const int c0 = chunk * EK_CHUNK;
   if (c0 >= n_all) return;
   const int n = n_all - c0 < EK_CHUNK ? n_all - c0 : EK_CHUNK;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int M = wx * wy, NC = 2 * wz;
-  const int mt_n = (M + 15) / 16;
-  const int row = lane & 15, kk = lane >> 4;
+  const int row = lane & 15, kq = lane >> 4;
   if (tid == 0) tril_inverse(a.tril + 6 * s, s_li);
   // zero the panels once: padding rows / columns are never written again
-  for (int e = tid; e < 2 * TA * (lda + ldb); e += EK_SK_THREADS) apk[e] = 0.f;
+  for (int e = tid; e < 16 * (mt_n + NT) * ldk; e += EK_SK_THREADS) apk[e] = 0.f;
   double sum[MTW][NT][4];
+  f32x4 acc[MTW][NT];
 #pragma unroll
   for (int t = 0; t < MTW; ++t)
 #pragma unroll
-    for (int u = 0; u < NT; ++u)
+    for (int u = 0; u < NT; ++u) {
+      acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int r = 0; r < 4; ++r) sum[t][u][r] = 0.0;
+    }
+  // tiles between FP64 flushes: at most 64 atoms in an FP32 partial
+  const int flush_every = TA >= 64 ? 1 : 64 / TA;
+  int since_flush = 0;
   for (int base = 0; base < n; base += TA) {
     __syncthreads();
     const int cnt = n - base < TA ? n - base : TA;
-    // 1. per-axis phase tables (q folded into x); atoms past the end are zero
+    // 1. per-axis phase tables, n-major (q folded into x); atoms past the end
+    //    are zero
     for (int e = tid; e < 3 * TA; e += EK_SK_THREADS) {
       const int axis = e / TA, at = e % TA;
       if (at < cnt) {
@@ -170,66 +179,67 @@ __global__ __launch_bounds__(EK_SK_THREADS) void ek_sk_kernel(EkArgs a, EkSkShap
         double2 u;
         sincospi(2.0 * f, &u.y, &u.x);
         if (axis == 0)
-          powers64_pos(u, (double)a.q[i0 + c0 + base + at], a.mx, tx + (size_t)at * wx);
+          powers64_pos(u, (double)a.q[i0 + c0 + base + at], a.mx, tx + at, TA);
         else if (axis == 1)
-          powers64(u, a.my, ty + (size_t)at * wy);
+          powers64(u, a.my, ty + at, TA);
         else
-          powers64(u, a.mz, tz + (size_t)at * wz);
+          powers64(u, a.mz, tz + at, TA);
       } else {
         const int w = axis == 0 ? wx : (axis == 1 ? wy : wz);
-        float2* o = axis == 0 ? tx + (size_t)at * wx : (axis == 1 ? ty + (size_t)at * wy : tz + (size_t)at * wz);
-        for (int m = 0; m < w; ++m) o[m] = make_float2(0.f, 0.f);
+        float2* o = axis == 0 ? tx + at : (axis == 1 ? ty + at : tz + at);
+        for (int m = 0; m < w; ++m) o[(size_t)m * TA] = make_float2(0.f, 0.f);
       }
     }
     __syncthreads();
-    // 2. pack A (k = 2 at + re/im, m) and B (k, j): j < wz -> Re S, else Im S
-    for (int m = tid; m < M; m += EK_SK_THREADS) {
+    // 2. pack A[m][k] (k = 2 at + re/im), two atoms per item
+    for (int e = tid; e < M * TH; e += EK_SK_THREADS) {
+      const int m = e / TH, p = e - m * TH;
       const int n0 = m / wy, n1 = m - n0 * wy;
-      const float2* px = tx + n0;
-      const float2* py = ty + n1;
-      float* pa = apk + m;
-      for (int at = 0; at < TA; ++at) {
-        const float2 A = cmul(px[(size_t)at * wx], py[(size_t)at * wy]);
-        pa[(size_t)(2 * at) * lda] = A.x;
-        pa[(size_t)(2 * at + 1) * lda] = A.y;
-      }
+      const float4 x = *reinterpret_cast<const float4*>(tx + (size_t)n0 * TA + 2 * p);
+      const float4 y = *reinterpret_cast<const float4*>(ty + (size_t)n1 * TA + 2 * p);
+      const float2 A0 = cmul(make_float2(x.x, x.y), make_float2(y.x, y.y));
+      const float2 A1 = cmul(make_float2(x.z, x.w), make_float2(y.z, y.w));
+      *reinterpret_cast<float4*>(apk + (size_t)m * ldk + 4 * p) = make_float4(A0.x, A0.y, A1.x, A1.y);
     }
-    for (int e = tid; e < TA * wz; e += EK_SK_THREADS) {
-      const int at = e / wz, j = e % wz;
-      const float2 z = tz[(size_t)at * wz + j];
-      float* b0 = bpk + (size_t)(2 * at) * ldb;
-      float* b1 = b0 + ldb;
-      b0[j] = z.x;  b1[j] = -z.y;       // Re S: Re A Re Z - Im A Im Z
-      b0[wz + j] = z.y;  b1[wz + j] = z.x;  // Im S: Re A Im Z + Im A Re Z
+    //    and B[j][k]: j < wz -> Re S (Re Z, -Im Z), else Im S (Im Z, Re Z)
+    for (int e = tid; e < wz * TH; e += EK_SK_THREADS) {
+      const int j = e / TH, p = e - j * TH;
+      const float4 z = *reinterpret_cast<const float4*>(tz + (size_t)j * TA + 2 * p);
+      *reinterpret_cast<float4*>(bpk + (size_t)j * ldk + 4 * p) = make_float4(z.x, -z.y, z.z, -z.w);
+      *reinterpret_cast<float4*>(bpk + (size_t)(wz + j) * ldk + 4 * p) = make_float4(z.y, z.x, z.w, z.z);
     }
     __syncthreads();
-    // 3. matrix-core products over this tile's K = 2 TA
-    f32x4 acc[MTW][NT];
+    // 3. matrix-core products over this tile's K = 2 TA, 16 k per block
+    const float* ap = apk + (size_t)(wave * 16 + row) * ldk + 4 * kq;
+    const float* bp = bpk + (size_t)row * ldk + 4 * kq;
+    for (int k0 = 0; k0 < 2 * TA; k0 += 16) {
+      f32x4 bv[NT];
 #pragma unroll
-    for (int t = 0; t < MTW; ++t)
-#pragma unroll
-      for (int u = 0; u < NT; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const float* ap = apk + (size_t)kk * lda + wave * 16 + row;
-    const float* bp = bpk + (size_t)kk * ldb + row;
-    for (int k0 = 0; k0 < 2 * TA; k0 += 4) {
-      float bv[NT];
-#pragma unroll
-      for (int u = 0; u < NT; ++u) bv[u] = bp[(size_t)k0 * ldb + u * 16];
+      for (int u = 0; u < NT; ++u)
+        bv[u] = *reinterpret_cast<const f32x4*>(bp + (size_t)u * 16 * ldk + k0);
 #pragma unroll
       for (int t = 0; t < MTW; ++t) {
         if (wave + t * EK_WAVES >= mt_n) break;
-        const float av = ap[(size_t)k0 * lda + t * EK_WAVES * 16];
+        const f32x4 av =
+            *reinterpret_cast<const f32x4*>(ap + (size_t)t * EK_WAVES * 16 * ldk + k0);
 #pragma unroll
-        for (int u = 0; u < NT; ++u)
-          acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[u], acc[t][u], 0, 0, 0);
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int u = 0; u < NT; ++u)
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j], bv[u][j], acc[t][u], 0, 0, 0);
       }
     }
+    if (++since_flush == flush_every || base + TA >= n) {
+      since_flush = 0;
 #pragma unroll
-    for (int t = 0; t < MTW; ++t)
+      for (int t = 0; t < MTW; ++t)
 #pragma unroll
-      for (int u = 0; u < NT; ++u)
+        for (int u = 0; u < NT; ++u) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) sum[t][u][r] += (double)acc[t][u][r];
+          for (int r = 0; r < 4; ++r) sum[t][u][r] += (double)acc[t][u][r];
+          acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
   }
   // D[row 4 (l/16) + r][col l % 16] of tile (t, u): m = 16 mt + row, column j
   double* part = a.skpart + ((size_t)s * a.chunks + chunk) * a.nk * 2;
@@ -278,20 +288,21 @@ void ek_sk_launch_nt(const EkArgs& a, const EkSkShape& sh, int nt, dim3 g, hipSt
   }
 }
 
-// Tile size: the largest even TA <= 64 whose panels and tables fit 64 KB.
+// Tile size: the largest TA in {64, 56, .., 8} whose panels and tables fit
+// 64 KB (two workgroups per CU); ldk = 2 TA + 4 has an odd quad count for TA
+// a multiple of 4.
 EkSkShape ek_sk_shape(const EkArgs& a) {
   const int wx = a.mx + 1, wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
-  const int mp = ((wx * wy + 15) / 16) * 16, np = ((2 * wz + 15) / 16) * 16;
-  EkSkShape sh;
-  sh.lda = ek_ld16(mp);
-  sh.ldb = ek_ld16(np);
-  const size_t per_atom = sizeof(float) * 2 * (size_t)(sh.lda + sh.ldb) +
-                          sizeof(float2) * (size_t)(wx + wy + wz);
-  int ta = (int)((64 * 1024) / per_atom);
-  ta = ta > 64 ? 64 : ta;
-  ta &= ~1;
-  sh.ta = ta < 2 ? 2 : ta;
-  sh.lds = per_atom * sh.ta;
+  const int rows = 16 * ((wx * wy + 15) / 16) + 16 * ((2 * wz + 15) / 16);
+  EkSkShape sh{};
+  for (int ta = 64; ta >= 8; ta -= 8) {
+    const int ldk = 2 * ta + 4;
+    const size_t lds = sizeof(float) * (size_t)rows * ldk + sizeof(float2) * (size_t)(wx + wy + wz) * ta;
+    sh.ta = ta;
+    sh.ldk = ldk;
+    sh.lds = lds;
+    if (lds <= 64 * 1024) break;
+  }
   return sh;
 }
 
