@@ -129,11 +129,18 @@ class MCMCAnalysisResult:
     loading: BlockAverageResult
     heat_of_adsorption: BlockAverageResult
     total_heat_of_adsorption: BlockAverageResult
+    pressure: BlockAverageResult | None = None
+    pressure_potential: BlockAverageResult | None = None
+    pressure_tail_correction: BlockAverageResult | None = None
+    pressure_ideal_gas: BlockAverageResult | None = None
 
 
-def analyze_mcmc(energy, counts, temperature: float,
-                 n_blocks: int | None = None) -> MCMCAnalysisResult:
-    """One system's per-cycle energy (eV) and adsorbate counts (n, n_species)."""
+def analyze_mcmc(energy, counts, temperature: float, n_blocks: int | None = None,
+                 pressure: dict | None = None) -> MCMCAnalysisResult:
+    """One system's per-cycle energy (eV) and adsorbate counts (n, n_species);
+    ``pressure``: per-sample 'potential', 'tail' and 'ideal' terms (eV/A^3),
+    block-averaged with the energy's block count, total = their sum
+    (application/mcmc/analysis.py:142-151)."""
     u = np.asarray(energy, np.float64)
     n = np.asarray(counts, np.float64)
     if n.ndim == 1:
@@ -156,13 +163,27 @@ def analyze_mcmc(energy, counts, temperature: float,
         w = nb / nb.sum(axis=1, keepdims=True)
     hoa_res = block_average_from_blocks(hoa)
     tot = block_average_from_blocks((hoa * w).sum(axis=1))
-    return MCMCAnalysisResult(e_res, load, hoa_res, tot)
+    out = MCMCAnalysisResult(e_res, load, hoa_res, tot)
+    if pressure is not None:
+        nb = min(n_blocks, len(pressure["potential"]))
+        parts = {k: np.asarray(v, np.float64) for k, v in pressure.items()}
+        out.pressure = block_average(parts["potential"] + parts["tail"] + parts["ideal"], nb)
+        out.pressure_potential = block_average(parts["potential"], nb)
+        out.pressure_tail_correction = block_average(parts["tail"], nb)
+        out.pressure_ideal_gas = block_average(parts["ideal"], nb)
+    return out
+
+
+_MCMC_FIELDS = ("energy", "loading", "heat_of_adsorption", "total_heat_of_adsorption",
+                "pressure", "pressure_potential", "pressure_tail_correction", "pressure_ideal_gas")
 
 
 def combine_mcmc(results: list[MCMCAnalysisResult]) -> MCMCAnalysisResult:
-    return MCMCAnalysisResult(*(combine_replicas([getattr(r, k) for r in results]) for k in
-                                ("energy", "loading", "heat_of_adsorption",
-                                 "total_heat_of_adsorption")))
+    vals = []
+    for k in _MCMC_FIELDS:
+        parts = [getattr(r, k) for r in results]
+        vals.append(None if any(p is None for p in parts) else combine_replicas(parts))
+    return MCMCAnalysisResult(*vals)
 
 
 def z_score(mean, sem, expected_mean, expected_sem) -> np.ndarray:

@@ -64,6 +64,8 @@ class McmcRun:
     structures: list
     energy: np.ndarray | None = None  # (n_cycles, n_chain) eV
     loading: np.ndarray | None = None  # (n_cycles, n_chain)
+    # (n_samples, n_chain) eV/A^3 when the deck sets compute_stress
+    pressure: dict | None = None
 
 
 def build(config: McmcConfig, *, replicas: int = 1, seed: int | None = None,
@@ -106,8 +108,10 @@ def _cycle_length(gcmc: RigidGCMC, min_len: int) -> int:
 
 
 def run(r: McmcRun, *, num_warmup_cycles: int | None = None,
-        num_cycles: int | None = None) -> McmcRun:
-    """Warm-up then production cycles; per-cycle energy and loading kept."""
+        num_cycles: int | None = None, stress_every: int = 1) -> McmcRun:
+    """Warm-up then production cycles; per-cycle energy and loading kept,
+    and with ``compute_stress`` the guest pressure terms every
+    ``stress_every`` cycles (RigidGCMC.guest_pressure)."""
     cfg = r.config.run
     n_warm = cfg.num_warmup_cycles if num_warmup_cycles is None else int(num_warmup_cycles)
     n_prod = cfg.num_cycles if num_cycles is None else int(num_cycles)
@@ -120,12 +124,18 @@ def run(r: McmcRun, *, num_warmup_cycles: int | None = None,
         g.resync()
     e = torch.empty((n_prod, g.n_chain), dtype=torch.float64, device=g.dev)
     n = torch.empty((n_prod, g.n_chain), dtype=torch.int32, device=g.dev)
+    terms = {"potential": [], "tail": [], "ideal": []}
     for c in range(n_prod):
         g.run(_cycle_length(g, cfg.min_cycle_length))
         e[c].copy_(g.d["energy"])
         n[c].copy_(g.loading)
+        if r.config.compute_stress and c % stress_every == 0:
+            for k, v in g.guest_pressure().items():
+                terms[k].append(v)
     r.energy = e.cpu().numpy()
     r.loading = n.cpu().numpy()
+    if r.config.compute_stress and terms["potential"]:
+        r.pressure = {k: np.stack(v) for k, v in terms.items()}
     return r
 
 
@@ -136,7 +146,9 @@ def analyze(r: McmcRun, n_blocks: int | None = None) -> list[MCMCAnalysisResult]
     out = []
     for h_i, h in enumerate(r.config.hosts):
         cols = range(h_i * r.replicas, (h_i + 1) * r.replicas)
-        res = [analyze_mcmc(r.energy[:, c], r.loading[:, c], float(h.temperature), n_blocks)
+        res = [analyze_mcmc(r.energy[:, c], r.loading[:, c], float(h.temperature), n_blocks,
+                            None if r.pressure is None else
+                            {k: v[:, c] for k, v in r.pressure.items()})
                for c in cols]
         out.append(combine_mcmc(res))
     return out

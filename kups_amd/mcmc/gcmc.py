@@ -19,10 +19,10 @@ import numpy as np
 import torch
 
 from .. import _lib
-from ..core.batch import ParticleBatch, SystemBatch, offsets_from_sizes
+from ..core.batch import ParticleBatch, SystemBatch, offsets_from_sizes, segment_sum
 from ..core.constants import BOLTZMANN_CONSTANT, PASCAL, TO_STANDARD_UNITS
 from ..potential.ewald import EwaldParameters, EwaldPotential, estimate_ewald_parameters, kvec_grid
-from ..potential.lennard_jones import LennardJonesParameters
+from ..potential.lennard_jones import LennardJonesParameters, global_tail_correction_pressure
 from .fugacity import peng_robinson_log_fugacity
 
 MOVES = ("translation", "rotation", "reinsertion", "exchange")
@@ -147,6 +147,7 @@ class RigidGCMC:
         self.guest_species = np.asarray(guest_species, np.int32)
         self.mol_const = np.array([molecule_constant(motif, a) for a in self.alpha])
         self.tail = np.stack([tail_matrix(lj, v) for v in vol])
+        self.tail_correction = bool(tail_correction)
         if not tail_correction:
             self.tail[:] = 0.0
         counts = np.zeros((n, self.n_species), np.int32)
@@ -354,6 +355,86 @@ class RigidGCMC:
         self.d["n_guest"].copy_(torch.as_tensor(ng))
         self.d["counts"].copy_(torch.as_tensor(counts))
         self.recompute_state()
+
+    def guest_pressure(self) -> dict:
+        """Pressure terms of every chain as the reference's MCMC logging
+        computes them (application/simulations/mcmc_rigid.py:385-431):
+
+        * ``potential``: molecular virial pressure of the guest-only system
+          (observables/stress.py:84-120, molecules at their geometric
+          centres): P = (tr W - sum_i d_i . F_i) / 3V with W and F of LJ,
+          real-space and reciprocal Ewald over the guests alone, d_i the
+          minimum-image offset of site i from its molecule's centre.  Pair
+          terms inside one molecule drop out of this trace exactly
+          (r_ij = d_i - d_j), so those pairs are excluded here;
+        * ``tail``: the global LJ tail pressure of all species counts
+          (lennard_jones.py:316-340);
+        * ``ideal``: N kT / V over the molecules (observables/pressure.py).
+
+        Returns FP64 numpy arrays (n_chain,) under those keys (eV/A^3)."""
+        d = self.d
+        dev = self.dev
+        n = self.n_chain
+        M = len(self.motif.charges)
+        ng = d["n_guest"].long()
+        vol = torch.as_tensor(self.volume, dtype=torch.float64, device=dev)
+        kT = torch.as_tensor(self.temperature * BOLTZMANN_CONSTANT, dtype=torch.float64, device=dev)
+        ideal = (ng.double() * kT / vol).cpu().numpy()
+        counts = d["counts"].cpu().numpy()
+        tail = global_tail_correction_pressure(self.lj, counts, self.volume) * \
+            float(self.tail_correction)
+        pot = np.zeros(n)
+        total = int(ng.sum().item())
+        if total == 0:
+            return {"potential": pot, "tail": tail, "ideal": ideal}
+        # guest-only configuration, chain-sorted, molecules contiguous
+        keep = torch.arange(self.gmax, device=dev)[None, :] < ng[:, None]          # (n, gmax)
+        g = d["guest"][keep]                                                      # (mols, M, 3)
+        chain_of_mol = torch.arange(n, device=dev)[:, None].expand(n, self.gmax)[keep]
+        tril = d["tril"]
+        L = torch.zeros((n, 3, 3), dtype=torch.float64, device=dev)
+        L[:, [0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]] = tril
+        Linv = torch.linalg.inv(L)
+        Lm, Lim = L[chain_of_mol], Linv[chain_of_mol]
+        # offsets from the first site on the minimum image, then from the centre
+        rel = g - g[:, :1]
+        f = torch.einsum("mkc,mcd->mkd", rel, Lim)
+        rel = torch.einsum("mkc,mcd->mkd", f - torch.round(f), Lm)
+        dvec = rel - rel.mean(dim=1, keepdim=True)                                # (mols, M, 3)
+        pos = g.reshape(-1, 3)
+        sizes = (ng * M).cpu().numpy()
+        off = offsets_from_sizes(sizes, dev)
+        q = torch.as_tensor(self.motif.charges, dtype=torch.float32, device=dev).repeat(g.shape[0])
+        sp = torch.as_tensor(self.guest_species, dtype=torch.int32, device=dev).repeat(g.shape[0])
+        mol_id = torch.arange(g.shape[0], device=dev, dtype=torch.int32).repeat_interleave(M)
+        p32 = pos.to(torch.float32).contiguous()
+        forces = torch.zeros((total * M, 3), dtype=torch.float32, device=dev)
+        e2 = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+        vir = torch.zeros((n, 3, 3), dtype=torch.float64, device=dev)
+        alpha = torch.as_tensor(self.alpha, dtype=torch.float64, device=dev)
+        rc_c = torch.full((n,), self.ewald_cutoff, dtype=torch.float64, device=dev)
+        rc_lj = torch.full((n,), self.lj.cutoff, dtype=torch.float64, device=dev)
+        sig = torch.as_tensor(self.lj.sigma, dtype=torch.float64, device=dev)
+        eps = torch.as_tensor(self.lj.epsilon, dtype=torch.float64, device=dev)
+        lib = _lib.load()
+        maxn = int(sizes.max())
+        _lib.check(lib.kupsmi_pair_forces(
+            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), total * M, maxn, _lib.ptr(p32),
+            _lib.ptr(q), _lib.ptr(sp), self.n_species, _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(rc_lj),
+            _lib.ptr(mol_id), _lib.ptr(alpha), _lib.ptr(rc_c), _lib.ptr(mol_id), _lib.ptr(tril), 0,
+            _lib.ptr(forces), _lib.ptr(e2), _lib.ptr(vir)))
+        nm = self.nm
+        elr = torch.zeros(n, dtype=torch.float64, device=dev)
+        kmax = torch.as_tensor(self.kmax, dtype=torch.float64, device=dev)
+        _lib.check(lib.kupsmi_ewald_recip(
+            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), total * M, maxn, _lib.ptr(p32), _lib.ptr(q),
+            _lib.ptr(tril), _lib.ptr(alpha), _lib.ptr(kmax), nm, nm, nm, 1, _lib.ptr(forces),
+            _lib.ptr(elr), _lib.ptr(vir), None))
+        dF = (dvec.reshape(-1, 3) * forces.double()).sum(-1)                      # (atoms,)
+        corr = segment_sum(dF, chain_of_mol.repeat_interleave(M), n)
+        tr = torch.diagonal(vir, dim1=-2, dim2=-1).sum(-1)
+        pot = ((tr - corr) / (3.0 * vol)).cpu().numpy()
+        return {"potential": pot, "tail": tail, "ideal": ideal}
 
     _STATE = ("guest", "n_guest", "counts", "sk", "energy", "width", "hist", "hist_idx", "stats")
 

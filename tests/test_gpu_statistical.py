@@ -89,11 +89,14 @@ def test_gcmc_co2_rubtak(gpu):
 @pytest.mark.gpu
 def test_nvt_50co2_30box(gpu):
     """50 CO2 in an empty 30 A box without exchange moves: the energy pins the
-    absolute composition (self, exclusion and intramolecular real-space terms)."""
+    absolute composition (self, exclusion and intramolecular real-space terms);
+    the deck's compute_stress adds the guest molecular-virial, tail and ideal
+    pressures (sampled every 5th cycle)."""
     cfg = McmcConfig.from_yaml(CI / "nvt_50co2_30box.yaml")
+    assert cfg.compute_stress
     r = mcmc_rigid.build(cfg, replicas=16, device=gpu)
     assert int(r.gcmc.loading.min()) == 50
-    mcmc_rigid.run(r)
+    mcmc_rigid.run(r, stress_every=5)
     res = mcmc_rigid.analyze(r)[0]
     assert np.all(r.loading == 50)
     err = np.abs(r.gcmc.energy_error())
