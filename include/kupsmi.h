@@ -261,6 +261,25 @@ int kupsmi_integrate(void* stream, int phase, int integrator, int n_sys,
                      int64_t step, double* kin_energy);
 
 /*
+ * Thermostat and barostat steps of the HBM-resident MD path, one workgroup
+ * per system.  Phase 0: CSVR rescaling p <- alpha p (CSVRStep,
+ * md/integrators.py:470-635) from each system's kinetic energy, with one R1
+ * for the batch as the reference draws it and R2 = 2 Gamma((Ndof - 1) / 2)
+ * per system; tau: coupling time per system.  Phase 1: stochastic cell
+ * rescaling (StochasticCellRescalingStep, md/integrators.py:657-811) from
+ * P = 2K / 3V + tr(virial) / 3V with one R for the batch; baro: n_sys x 4
+ * (target pressure, coupling time, compressibility, minimum scale factor);
+ * tril is scaled in place and the positions with it (momenta kept).  The
+ * random streams and formulas are those of kupsmi_md_lj's integrators 2
+ * and 3, so both paths run the same chains.
+ */
+int kupsmi_thermostat_barostat(void* stream, int phase, int n_sys, const int32_t* sys_off,
+                               float* pos, float* mom, const float* mass, double* tril,
+                               const double* dt, const double* kT, const double* tau,
+                               const double* baro, const double* virial,
+                               const double* kin_energy, uint64_t seed, int64_t step);
+
+/*
  * Rigid-molecule GCMC: n_steps trial moves of every chain, one workgroup per
  * chain with its host framework and guests resident in LDS.  Replaces the
  * MCMC cycle of kups_mcmc_rigid (application/simulations/mcmc_rigid.py:

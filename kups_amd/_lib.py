@@ -71,6 +71,8 @@ _SIGNATURES = {
         [vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64, vp],
         C.c_int,
     ),
+    "kupsmi_thermostat_barostat": (
+        [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64], C.c_int),
 }
 
 

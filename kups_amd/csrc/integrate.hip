@@ -6,6 +6,10 @@
 //            and the fractional wrap of the positions (104-141);
 //   phase 1: B(dt/2) and the per-system kinetic energy (md/observables.py:17-34).
 // The O step uses the same Philox4x32-10 stream as kupsmi_md_lj.
+// kupsmi_thermostat_barostat adds the per-system CSVR rescaling
+// (md/integrators.py:470-635) and the stochastic cell rescaling
+// (657-811) with the random streams and formulas of kupsmi_md_lj's
+// integrators 2 and 3, so the HBM path runs the same NVT / NPT chains.
 #include "common.hpp"
 
 namespace kupsmi {
@@ -79,9 +83,98 @@ __global__ __launch_bounds__(IG_THREADS) void ig_post_kernel(
   }
 }
 
+// CSVR: K of each system (FP64 sum), the Bussi factor alpha (one R1 for the
+// batch, R2 per system) and p <- alpha p.  One workgroup per system.
+__global__ __launch_bounds__(IG_THREADS) void ig_csvr_kernel(
+    const int32_t* sys_off, float* mom, const float* mass, const double* dt, const double* kT,
+    const double* tau, uint64_t seed, int64_t step) {
+  __shared__ double red[IG_THREADS / 64];
+  __shared__ float s_alpha;
+  const int s = blockIdx.x;
+  const int i0 = sys_off[s], n = sys_off[s + 1] - i0;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += IG_THREADS) {
+    const int gi = i0 + i;
+    const float px = mom[3 * gi], py = mom[3 * gi + 1], pz = mom[3 * gi + 2];
+    acc += 0.5 * (double)((px * px + py * py + pz * pz) / mass[gi]);
+  }
+  acc = wave_sum(acc);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double K = 0.0;
+    for (int w = 0; w < IG_THREADS / 64; ++w) K += red[w];
+    s_alpha = (float)csvr_alpha(K, n, kT[s], dt[s], tau[s], seed, (uint32_t)s, step);
+  }
+  __syncthreads();
+  const float alpha = s_alpha;
+  for (int i = threadIdx.x; i < n; i += IG_THREADS) {
+    const int gi = i0 + i;
+    mom[3 * gi] *= alpha; mom[3 * gi + 1] *= alpha; mom[3 * gi + 2] *= alpha;
+  }
+}
+
+// Stochastic cell rescaling (Bernetti and Bussi 2020): P = 2K / 3V + tr W / 3V,
+// d eps = (dt / tau_P) beta (P - P0) + sqrt(2 kT beta dt / (tau_P V)) R with one
+// R ~ N(0, 1) for the batch, mu = exp(d eps / 3) clipped to [mu_min, 1 / mu_min];
+// the cell (FP64, in place) and the positions are scaled by mu, momenta kept.
+__global__ __launch_bounds__(IG_THREADS) void ig_scr_kernel(
+    const int32_t* sys_off, float* pos, double* tril, const double* dt, const double* kT,
+    const double* baro, const double* virial, const double* kin, uint64_t seed, int64_t step) {
+  __shared__ double s_mu;
+  const int s = blockIdx.x;
+  const int i0 = sys_off[s], n = sys_off[s + 1] - i0;
+  if (threadIdx.x == 0) {
+    double* t = tril + 6 * (size_t)s;
+    const double* bp = baro + 4 * (size_t)s;
+    const double* W = virial + 9 * (size_t)s;
+    const double V = fabs(t[0] * t[2] * t[5]);
+    const double P = 2.0 * kin[s] / (3.0 * V) + (W[0] + W[4] + W[8]) / (3.0 * V);
+    uint32_t c4[4];
+    csvr_draw(seed, 0xFFFFFFFEu, step, SCR_TAG, c4);
+    const double R = csvr_bm(c4[0], c4[1]);
+    const double de = dt[s] / bp[1] * bp[2] * (P - bp[0]) +
+                      sqrt(2.0 * kT[s] * bp[2] * dt[s] / (bp[1] * V)) * R;
+    const double mu = fmin(fmax(exp(de / 3.0), bp[3]), 1.0 / bp[3]);
+    for (int c = 0; c < 6; ++c) t[c] *= mu;
+    s_mu = mu;
+  }
+  __syncthreads();
+  const float mu = (float)s_mu;
+  for (int i = threadIdx.x; i < n; i += IG_THREADS) {
+    const int gi = i0 + i;
+    pos[3 * gi] *= mu; pos[3 * gi + 1] *= mu; pos[3 * gi + 2] *= mu;
+  }
+}
+
 }  // namespace kupsmi
 
 using namespace kupsmi;
+
+extern "C" int kupsmi_thermostat_barostat(void* stream, int phase, int n_sys,
+                                          const int32_t* sys_off, float* pos, float* mom,
+                                          const float* mass, double* tril, const double* dt,
+                                          const double* kT, const double* tau,
+                                          const double* baro, const double* virial,
+                                          const double* kin_energy, uint64_t seed,
+                                          int64_t step) {
+  KUPSMI_REQUIRE(phase == 0 || phase == 1, "phase must be 0 (CSVR) or 1 (cell rescaling)");
+  KUPSMI_REQUIRE(n_sys >= 0, "negative size");
+  KUPSMI_REQUIRE(sys_off && dt && kT, "system offsets, time steps and kT required");
+  KUPSMI_REQUIRE(phase != 0 || (mom && mass && tau), "CSVR needs momenta, masses and tau");
+  KUPSMI_REQUIRE(phase != 1 || (pos && tril && baro && virial && kin_energy),
+                 "cell rescaling needs positions, cell, barostat, virial and kinetic energy");
+  if (n_sys == 0) return KUPSMI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (phase == 0)
+    ig_csvr_kernel<<<n_sys, IG_THREADS, 0, st>>>(sys_off, mom, mass, dt, kT, tau, seed, step);
+  else
+    ig_scr_kernel<<<n_sys, IG_THREADS, 0, st>>>(sys_off, pos, tril, dt, kT, baro, virial,
+                                                kin_energy, seed, step);
+  KUPSMI_CHECK_LAUNCH();
+  return KUPSMI_OK;
+}
 
 extern "C" int kupsmi_integrate(void* stream, int phase, int integrator, int n_sys,
                                 const int32_t* sys_off, int max_atoms_per_sys, float* pos,

@@ -170,7 +170,14 @@ class EwaldLJPropagator:
     """MD of charged LJ systems (LJ + Ewald) on the HBM-resident path, one
     force evaluation per step: integrate(phase 0) -> kupsmi_pair_forces (LJ and
     real-space Ewald, one pair loop) -> kupsmi_ewald_recip (k-space, added) ->
-    integrate(phase 1).  Same step order as the fused LJ kernel."""
+    integrate(phase 1).  Same step order as the fused LJ kernel.
+
+    integrator CSVR puts the thermostat rescaling (kupsmi_thermostat_barostat
+    phase 0) before the velocity Verlet step; CSVR_NPT also rescales cell and
+    positions after the closing kick (phase 1, from that step's virial and
+    kinetic energy) and evaluates the forces again at the rescaled state
+    (md/integrators.py:826-888).  The cell changes every step, so NPT runs
+    with a cell list per evaluation (no Verlet reuse)."""
 
     def __init__(self, params: LennardJonesParameters, ewald, integrator: int = BAOAB_LANGEVIN,
                  seed: int = 0, skin: float | None = None, max_neighbors: int | None = None):
@@ -196,7 +203,12 @@ class EwaldLJPropagator:
             self._kmax = torch.as_tensor(self.ewald.kmax, dtype=torch.float64, device=dev).contiguous()
             self._dev = dev
         self._maxn = int(state.sizes.max()) if state.sizes.size else 0
-        if self.skin is not None and self.verlet is None:
+        if self.integrator == CSVR_NPT and (state.barostat is None or
+                                            state.thermostat_time_constant is None):
+            raise ValueError("NPT needs the state's barostat and thermostat coupling")
+        if self.integrator == CSVR and state.thermostat_time_constant is None:
+            raise ValueError("CSVR needs the state's thermostat coupling time")
+        if self.skin is not None and self.verlet is None and self.integrator != CSVR_NPT:
             from ..core.verlet import VerletList, neighbor_capacity
 
             rc = max(float(np.max(self.ewald.cutoff)), float(state.cutoff.max().item()))
@@ -249,20 +261,39 @@ class EwaldLJPropagator:
         kin = torch.zeros((n_steps, state.n_sys), dtype=torch.float64, device=dev)
         kT = state.kT.contiguous()
         seed = int(self.seed) & (2**64 - 1)
+        csvr = self.integrator in (CSVR, CSVR_NPT)
+        npt = self.integrator == CSVR_NPT
+        base = VELOCITY_VERLET if csvr else int(self.integrator)
+        vir = torch.zeros((state.n_sys, 3, 3), dtype=torch.float64, device=dev) if npt else None
+        tau = _lib.ptr(state.thermostat_time_constant) if csvr else None
         for k in range(n_steps):
+            if csvr:
+                _lib.check(lib.kupsmi_thermostat_barostat(
+                    _lib.stream_ptr(), 0, state.n_sys, _lib.ptr(state.sys_off), None,
+                    _lib.ptr(state.momenta), _lib.ptr(state.masses), None,
+                    _lib.ptr(state.time_step), _lib.ptr(kT), tau, None, None, None, seed,
+                    int(state.step)))
             _lib.check(lib.kupsmi_integrate(
-                _lib.stream_ptr(), 0, int(self.integrator), state.n_sys, _lib.ptr(state.sys_off),
+                _lib.stream_ptr(), 0, base, state.n_sys, _lib.ptr(state.sys_off),
                 self._maxn, _lib.ptr(state.positions), _lib.ptr(state.momenta),
                 _lib.ptr(state.forces), _lib.ptr(state.masses), _lib.ptr(state.tril),
                 _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction), seed,
                 int(state.step), None))
-            self.forces(state, e2[k], elr[k])
+            self.forces(state, e2[k], elr[k], vir)
             _lib.check(lib.kupsmi_integrate(
-                _lib.stream_ptr(), 1, int(self.integrator), state.n_sys, _lib.ptr(state.sys_off),
+                _lib.stream_ptr(), 1, base, state.n_sys, _lib.ptr(state.sys_off),
                 self._maxn, _lib.ptr(state.positions), _lib.ptr(state.momenta),
                 _lib.ptr(state.forces), _lib.ptr(state.masses), _lib.ptr(state.tril),
                 _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction), seed,
                 int(state.step), _lib.ptr(kin[k])))
+            if npt:
+                _lib.check(lib.kupsmi_thermostat_barostat(
+                    _lib.stream_ptr(), 1, state.n_sys, _lib.ptr(state.sys_off),
+                    _lib.ptr(state.positions), None, None, _lib.ptr(state.tril),
+                    _lib.ptr(state.time_step), _lib.ptr(kT), None, _lib.ptr(state.barostat),
+                    _lib.ptr(vir), _lib.ptr(kin[k]), seed, int(state.step)))
+                elr[k].zero_()
+                self.forces(state, e2[k], elr[k], vir)
             state.step += 1
         system = torch.repeat_interleave(torch.arange(state.n_sys, device=dev),
                                          torch.as_tensor(state.sizes, device=dev))

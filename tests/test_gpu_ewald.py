@@ -202,3 +202,72 @@ def test_ewald_md_velocity_verlet_vs_oracle(gpu):
         eg = obs.potential_energy[:, k].cpu().numpy()
         np.testing.assert_allclose(eg, e, rtol=1e-5)
     assert _rel_norm(st.forces.cpu().numpy(), frc) < 1e-4
+
+
+def _oracle_forces_w(b, pos, tril, alpha, rc, kv):
+    """FP64 LJ + Ewald energies, forces and virial at (pos, tril)."""
+    rcs = np.full(b.n_sys, rc)
+    e_lj, f_lj, w_lj = oracle.lj_cells(pos, b.species, b.sys_off, tril, rcs, NACL_LJ.sigma,
+                                       NACL_LJ.epsilon)
+    ij, sh = oracle.neighborlist(pos, np.arange(b.n), b.sys_off, tril, rcs)
+    esr, fsr, wsr = oracle.ewald_real_edges(pos, b.charges, b.sys_off, tril,
+                                            np.full(b.n_sys, alpha), rcs, ij, sh)
+    elr, flr, wlr, _ = oracle.ewald_recip(pos, b.charges, b.sys_off, tril, np.full(b.n_sys, alpha),
+                                          kv)
+    es = oracle.ewald_self(b.charges, b.sys_off, np.full(b.n_sys, alpha))
+    return e_lj + esr + elr + es, f_lj + fsr + flr, w_lj + wsr + wlr
+
+
+@pytest.mark.parametrize("npt", [False, True])
+def test_ewald_md_csvr_and_npt_vs_oracle(gpu, npt):
+    """The HBM path's CSVR thermostat and stochastic cell rescaling
+    (kupsmi_thermostat_barostat) around velocity Verlet with LJ + Ewald forces,
+    step by step against the oracle's CSVR rescaling (same Philox draws), FP64
+    velocity Verlet, cell rescaling from the FP64 virial and a fresh FP64
+    force evaluation at the rescaled state."""
+    from kups_amd.md.integrators import CSVR, CSVR_NPT
+
+    b = workloads.nacl_melts(ms=[5, 6], seed=7)
+    p0 = maxwell_boltzmann_momenta(b.masses, b.sizes, 1200.0, np.random.default_rng(1))
+    # a strong barostat coupling so the cells move visibly within a few steps
+    st = make_md_state(b.positions.astype(np.float32), b.masses, b.species, b.sizes, b.tril, 12.0,
+                       time_step_fs=1.0, temperature=1200.0, momenta=p0.astype(np.float32),
+                       charges=b.charges, thermostat_time_constant_fs=20.0,
+                       target_pressure_pa=1.0e8, pressure_coupling_time_fs=20.0,
+                       compressibility_per_pa=2.0e-9)
+    ep = EwaldParameters.box(b.n_sys, alpha=0.3, cutoff=12.0, nmax=6)
+    seed = 23
+    prop = EwaldLJPropagator(NACL_LJ, ep, CSVR_NPT if npt else CSVR, seed=seed)
+    prop.evaluate_forces(st)
+    pos = _f32(st.positions.cpu().numpy())
+    mom = _f32(st.momenta.cpu().numpy())
+    frc = _f32(st.forces.cpu().numpy())
+    tril = np.ascontiguousarray(b.tril.astype(np.float64))
+    kv = np.stack([oracle.kvecs(t, 6) for t in b.tril])
+    m = b.masses.astype(np.float32).astype(np.float64)
+    dt = st.time_step.cpu().numpy()
+    kT = st.kT.cpu().numpy()
+    tau = st.thermostat_time_constant.cpu().numpy()
+    baro = st.barostat.cpu().numpy()
+    step0 = st.step
+    n = 5
+    obs = prop(st, n)
+    mus = []
+    for k in range(n):
+        oracle.csvr_scale(seed, step0 + k, mom, m, b.sys_off, kT, dt, tau)
+        mom += frc * dt[0] / 2
+        pos = np.ascontiguousarray(pos + mom / m[:, None] * dt[0])
+        e, frc, w = _oracle_forces_w(b, pos, tril, 0.3, 12.0, kv)
+        mom += frc * dt[0] / 2
+        if npt:
+            mus.append(oracle.scr_scale(seed, step0 + k, mom, pos, tril, m, b.sys_off, kT, dt, baro,
+                                        w))
+            e, frc, w = _oracle_forces_w(b, pos, tril, 0.3, 12.0, kv)
+        np.testing.assert_allclose(obs.potential_energy[:, k].cpu().numpy(), e, rtol=1e-5)
+    if npt:
+        assert np.abs(np.log(np.array(mus))).max() > 1e-5  # the cells did move
+        np.testing.assert_allclose(st.tril.cpu().numpy(), tril, rtol=1e-6)
+    else:
+        np.testing.assert_array_equal(st.tril.cpu().numpy(), b.tril)
+    assert _rel_norm(st.forces.cpu().numpy(), frc) < 1e-4
+    assert _rel_norm(st.momenta.cpu().numpy(), mom) < 1e-4
