@@ -162,6 +162,7 @@ def ewald_leg(args):
     import torch
 
     from kups_amd import workloads
+    from kups_amd.core.autotune import tune_ewald_skin
     from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
     from kups_amd.md.integrators import BAOAB_LANGEVIN, EwaldLJPropagator
     from kups_amd.potential.ewald import EwaldParameters
@@ -174,7 +175,9 @@ def ewald_leg(args):
                        time_step_fs=1.0, temperature=1200.0, momenta=p0.astype(np.float32),
                        charges=b.charges, friction_per_fs=0.001)
     ep = EwaldParameters.box(b.n_sys, alpha=0.3, cutoff=12.0, nmax=8)
-    prop = EwaldLJPropagator(lj, ep, BAOAB_LANGEVIN, skin=1.0)
+    EwaldLJPropagator(lj, ep, BAOAB_LANGEVIN).evaluate_forces(st)
+    tuned = tune_ewald_skin(lj, ep, st, BAOAB_LANGEVIN)
+    prop = EwaldLJPropagator(lj, ep, BAOAB_LANGEVIN, skin=tuned.best)
     prop.evaluate_forces(st)
     if args.ewald_warmup:
         prop(st, args.ewald_warmup)
@@ -192,8 +195,9 @@ def ewald_leg(args):
         "ms_per_step": wall * 1e3 / args.ewald_steps, "steps": args.ewald_steps,
         "warmup": args.ewald_warmup, "dtype": "fp32 pairs and k sums (fp64 accumulation)",
         "workload": f"LJ+Ewald melts: {b.description}, rc 12 A, alpha 0.3, |n| <= 8, BAOAB 1200 K, "
-                    "dt 1 fs, Verlet skin 1 A",
+                    f"dt 1 fs, Verlet skin {tuned.best:g} A (autotuned)",
         "n_ions": ions,
+        "autotune": {"skin_A": tuned.best, "ms_per_step": tuned.table, "cached": tuned.cached},
         "verlet": {"system_rebuilds_total": rebuilds, "overflowed_lists": overflow},
         "final_potential_energy_eV": float(obs.potential_energy[:, -1].sum().item()),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
@@ -363,6 +367,7 @@ def main():
 
     from kups_amd import workloads
     from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
+    from kups_amd.core.autotune import tune_md_skin
     from kups_amd.md.integrators import make_velocity_verlet_step
     from kups_amd.potential.lennard_jones import LennardJonesParameters
 
@@ -383,6 +388,12 @@ def main():
                        temperature=94.4, momenta=p0.astype(np.float32))
     prop = make_velocity_verlet_step(params)
     prop.evaluate_forces(st)
+    tuned = None
+    if not args.traffic_probe:
+        # skin chosen by measurement (a few hundred steps on this state,
+        # restored afterwards); the traffic probe keeps the fixed 0.3 sigma
+        tuned = tune_md_skin(params, st)
+        prop = make_velocity_verlet_step(params, skin=tuned.best)
     if args.traffic_probe:
         prop(st, args.steps, record=True)
         torch.cuda.synchronize()
@@ -434,7 +445,8 @@ def main():
             "workload": "batched LJ MD: 2048 cubic PBC boxes, N~U{64..4096} (seed 0), "
                         "rho*=0.8, jitter 0.05 sigma, argon LJ rc=2.5 sigma, NVE velocity "
                         "Verlet dt=2 fs, the exact cutoff pair set every step (Verlet lists, "
-                        "skin 0.3 sigma, rebuilt on the skin/2 displacement bound)",
+                        f"skin {tuned.best / workloads.AR_SIGMA:.2f} sigma autotuned, rebuilt "
+                        "on the skin/2 displacement bound)",
             "n_systems": batch.n_sys,
             "n_atoms": atoms,
             "steps_per_launch": spl,
@@ -455,6 +467,10 @@ def main():
         },
         "final_total_energy_eV": etot,
     }
+    line["autotune"] = {"skin_sigma": tuned.best / workloads.AR_SIGMA,
+                        "ms_per_step": {f"{k / workloads.AR_SIGMA:.2f}": v
+                                        for k, v in tuned.table.items()},
+                        "cached": tuned.cached}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(batch, params, args)
     del st, prop

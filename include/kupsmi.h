@@ -253,12 +253,14 @@ int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
  * B(dt/2) A(dt/2) O A(dt/2) (integrator 1) and the fractional wrap; phase 1 =
  * B(dt/2) plus the per-system kinetic energy; phase 2 = kinetic energy only.
  * The O step draws the same Philox4x32-10 normals as kupsmi_md_lj.
+ * step_dev (optional): a device int64 read by the kernels instead of `step`,
+ * so steps recorded in a HIP graph draw fresh noise on every replay.
  */
 int kupsmi_integrate(void* stream, int phase, int integrator, int n_sys,
                      const int32_t* sys_off, int max_atoms_per_sys, float* pos, float* mom,
                      const float* forces, const float* mass, const double* tril,
                      const double* dt, const double* kT, const double* gamma, uint64_t seed,
-                     int64_t step, double* kin_energy);
+                     int64_t step, double* kin_energy, const int64_t* step_dev);
 
 /*
  * Thermostat and barostat steps of the HBM-resident MD path, one workgroup
@@ -277,7 +279,16 @@ int kupsmi_thermostat_barostat(void* stream, int phase, int n_sys, const int32_t
                                float* pos, float* mom, const float* mass, double* tril,
                                const double* dt, const double* kT, const double* tau,
                                const double* baro, const double* virial,
-                               const double* kin_energy, uint64_t seed, int64_t step);
+                               const double* kin_energy, uint64_t seed, int64_t step,
+                               const int64_t* step_dev);
+
+/*
+ * *step_dev += n on the stream: the device-side step counter that
+ * kupsmi_integrate / kupsmi_thermostat_barostat read when they are recorded
+ * in a HIP graph (the host-side step of a graph is fixed at capture).  No
+ * counterpart in the reference (its step counter is a JAX array).
+ */
+int kupsmi_step_advance(void* stream, int64_t* step_dev, int64_t n);
 
 /*
  * Rigid-molecule GCMC: n_steps trial moves of every chain, one workgroup per

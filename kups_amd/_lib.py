@@ -68,11 +68,12 @@ _SIGNATURES = {
     "kupsmi_mlp_round_weights": ([vp, vp, vp, vp], C.c_int),
     "kupsmi_proposal_table": ([vp, i32, i32, vp, C.c_float, C.c_float, vp, vp], C.c_int),
     "kupsmi_integrate": (
-        [vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64, vp],
+        [vp, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64, vp, vp],
         C.c_int,
     ),
     "kupsmi_thermostat_barostat": (
-        [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64], C.c_int),
+        [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u64, i64, vp], C.c_int),
+    "kupsmi_step_advance": ([vp, vp, i64], C.c_int),
 }
 
 

@@ -19,7 +19,7 @@ constexpr int IG_THREADS = 256;
 __global__ __launch_bounds__(IG_THREADS) void ig_pre_kernel(
     int integrator, int n_sys, const int32_t* sys_off, float* pos, float* mom,
     const float* forces, const float* mass, const double* tril, const double* dt,
-    const double* kT, const double* gamma, uint64_t seed, int64_t step) {
+    const double* kT, const double* gamma, uint64_t seed, int64_t step, const int64_t* step_dev) {
   const int s = blockIdx.y;
   const int i0 = sys_off[s], n = sys_off[s + 1] - i0;
   const int i = blockIdx.x * IG_THREADS + threadIdx.x;
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(IG_THREADS) void ig_pre_kernel(
     const double e1 = exp(-gamma[s] * dt[s]);
     const float c1 = (float)e1, c2 = (float)sqrt(kT[s] * (1.0 - e1 * e1));
     float e0, ea, eb;
-    normals3(seed, (uint32_t)gi, step, e0, ea, eb);
+    normals3(seed, (uint32_t)gi, step_dev ? *step_dev : step, e0, ea, eb);
     const float sm = c2 * sqrtf(mass[gi]);
     px = c1 * px + sm * e0; py = c1 * py + sm * ea; pz = c1 * pz + sm * eb;
     rx += px * im * h; ry += py * im * h; rz += pz * im * h;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(IG_THREADS) void ig_post_kernel(
 // batch, R2 per system) and p <- alpha p.  One workgroup per system.
 __global__ __launch_bounds__(IG_THREADS) void ig_csvr_kernel(
     const int32_t* sys_off, float* mom, const float* mass, const double* dt, const double* kT,
-    const double* tau, uint64_t seed, int64_t step) {
+    const double* tau, uint64_t seed, int64_t step, const int64_t* step_dev) {
   __shared__ double red[IG_THREADS / 64];
   __shared__ float s_alpha;
   const int s = blockIdx.x;
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(IG_THREADS) void ig_csvr_kernel(
   if (threadIdx.x == 0) {
     double K = 0.0;
     for (int w = 0; w < IG_THREADS / 64; ++w) K += red[w];
-    s_alpha = (float)csvr_alpha(K, n, kT[s], dt[s], tau[s], seed, (uint32_t)s, step);
+    s_alpha = (float)csvr_alpha(K, n, kT[s], dt[s], tau[s], seed, (uint32_t)s,
+                                step_dev ? *step_dev : step);
   }
   __syncthreads();
   const float alpha = s_alpha;
@@ -121,7 +122,8 @@ __global__ __launch_bounds__(IG_THREADS) void ig_csvr_kernel(
 // the cell (FP64, in place) and the positions are scaled by mu, momenta kept.
 __global__ __launch_bounds__(IG_THREADS) void ig_scr_kernel(
     const int32_t* sys_off, float* pos, double* tril, const double* dt, const double* kT,
-    const double* baro, const double* virial, const double* kin, uint64_t seed, int64_t step) {
+    const double* baro, const double* virial, const double* kin, uint64_t seed, int64_t step,
+    const int64_t* step_dev) {
   __shared__ double s_mu;
   const int s = blockIdx.x;
   const int i0 = sys_off[s], n = sys_off[s + 1] - i0;
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(IG_THREADS) void ig_scr_kernel(
     const double V = fabs(t[0] * t[2] * t[5]);
     const double P = 2.0 * kin[s] / (3.0 * V) + (W[0] + W[4] + W[8]) / (3.0 * V);
     uint32_t c4[4];
-    csvr_draw(seed, 0xFFFFFFFEu, step, SCR_TAG, c4);
+    csvr_draw(seed, 0xFFFFFFFEu, step_dev ? *step_dev : step, SCR_TAG, c4);
     const double R = csvr_bm(c4[0], c4[1]);
     const double de = dt[s] / bp[1] * bp[2] * (P - bp[0]) +
                       sqrt(2.0 * kT[s] * bp[2] * dt[s] / (bp[1] * V)) * R;
@@ -148,6 +150,9 @@ __global__ __launch_bounds__(IG_THREADS) void ig_scr_kernel(
   }
 }
 
+// One thread: *step += n (the device-side step counter of captured steps).
+__global__ void ig_step_kernel(int64_t* step, int64_t n) { *step += n; }
+
 }  // namespace kupsmi
 
 using namespace kupsmi;
@@ -158,7 +163,7 @@ extern "C" int kupsmi_thermostat_barostat(void* stream, int phase, int n_sys,
                                           const double* kT, const double* tau,
                                           const double* baro, const double* virial,
                                           const double* kin_energy, uint64_t seed,
-                                          int64_t step) {
+                                          int64_t step, const int64_t* step_dev) {
   KUPSMI_REQUIRE(phase == 0 || phase == 1, "phase must be 0 (CSVR) or 1 (cell rescaling)");
   KUPSMI_REQUIRE(n_sys >= 0, "negative size");
   KUPSMI_REQUIRE(sys_off && dt && kT, "system offsets, time steps and kT required");
@@ -168,10 +173,18 @@ extern "C" int kupsmi_thermostat_barostat(void* stream, int phase, int n_sys,
   if (n_sys == 0) return KUPSMI_OK;
   hipStream_t st = (hipStream_t)stream;
   if (phase == 0)
-    ig_csvr_kernel<<<n_sys, IG_THREADS, 0, st>>>(sys_off, mom, mass, dt, kT, tau, seed, step);
+    ig_csvr_kernel<<<n_sys, IG_THREADS, 0, st>>>(sys_off, mom, mass, dt, kT, tau, seed, step,
+                                                 step_dev);
   else
     ig_scr_kernel<<<n_sys, IG_THREADS, 0, st>>>(sys_off, pos, tril, dt, kT, baro, virial,
-                                                kin_energy, seed, step);
+                                                kin_energy, seed, step, step_dev);
+  KUPSMI_CHECK_LAUNCH();
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_step_advance(void* stream, int64_t* step_dev, int64_t n) {
+  KUPSMI_REQUIRE(step_dev != nullptr, "null step counter");
+  ig_step_kernel<<<1, 1, 0, (hipStream_t)stream>>>(step_dev, n);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }
@@ -181,7 +194,7 @@ extern "C" int kupsmi_integrate(void* stream, int phase, int integrator, int n_s
                                 float* mom, const float* forces, const float* mass,
                                 const double* tril, const double* dt, const double* kT,
                                 const double* gamma, uint64_t seed, int64_t step,
-                                double* kin_energy) {
+                                double* kin_energy, const int64_t* step_dev) {
   KUPSMI_REQUIRE(phase == 0 || phase == 1 || phase == 2, "phase must be 0, 1 or 2");
   KUPSMI_REQUIRE(integrator == 0 || integrator == 1, "integrator must be 0 (VV) or 1 (BAOAB)");
   KUPSMI_REQUIRE(n_sys >= 0 && max_atoms_per_sys >= 0, "negative size");
@@ -192,7 +205,7 @@ extern "C" int kupsmi_integrate(void* stream, int phase, int integrator, int n_s
     if (max_atoms_per_sys == 0) return KUPSMI_OK;
     dim3 g((max_atoms_per_sys + IG_THREADS - 1) / IG_THREADS, n_sys);
     ig_pre_kernel<<<g, IG_THREADS, 0, st>>>(integrator, n_sys, sys_off, pos, mom, forces, mass,
-                                            tril, dt, kT, gamma, seed, step);
+                                            tril, dt, kT, gamma, seed, step, step_dev);
   } else {
     // phase 1: kick + kinetic energy; phase 2: kinetic energy only
     ig_post_kernel<<<n_sys, IG_THREADS, 0, st>>>(n_sys, sys_off, mom, forces, mass, dt,

@@ -251,9 +251,23 @@ class EwaldLJPropagator:
         return e2, elr, vir
 
     def __call__(self, state: MDState, n_steps: int = 1) -> EwaldObservables:
+        self._prep(state)
+        obs = self._steps(state, n_steps, None, self._self_energy(state))
+        state.step += n_steps
+        return obs
+
+    def _self_energy(self, state: MDState) -> torch.Tensor:
         from ..potential.ewald import self_energy
 
-        self._prep(state)
+        dev = state.positions.device
+        system = torch.repeat_interleave(torch.arange(state.n_sys, device=dev),
+                                         torch.as_tensor(state.sizes, device=dev))
+        return self_energy(state.charges, system, self._alpha, state.n_sys)
+
+    def _steps(self, state: MDState, n_steps: int, step_dev, es) -> EwaldObservables:
+        """n_steps launches sequences; step_dev: device int64 step counter read
+        by the kernels (and advanced after each step) instead of state.step,
+        which this method leaves alone."""
         dev = state.positions.device
         lib = _lib.load()
         e2 = torch.zeros((n_steps, state.n_sys, 2), dtype=torch.float64, device=dev)
@@ -266,36 +280,75 @@ class EwaldLJPropagator:
         base = VELOCITY_VERLET if csvr else int(self.integrator)
         vir = torch.zeros((state.n_sys, 3, 3), dtype=torch.float64, device=dev) if npt else None
         tau = _lib.ptr(state.thermostat_time_constant) if csvr else None
+        sd = _lib.ptr(step_dev) if step_dev is not None else None
         for k in range(n_steps):
+            t = int(state.step) + k
             if csvr:
                 _lib.check(lib.kupsmi_thermostat_barostat(
                     _lib.stream_ptr(), 0, state.n_sys, _lib.ptr(state.sys_off), None,
                     _lib.ptr(state.momenta), _lib.ptr(state.masses), None,
-                    _lib.ptr(state.time_step), _lib.ptr(kT), tau, None, None, None, seed,
-                    int(state.step)))
+                    _lib.ptr(state.time_step), _lib.ptr(kT), tau, None, None, None, seed, t, sd))
             _lib.check(lib.kupsmi_integrate(
                 _lib.stream_ptr(), 0, base, state.n_sys, _lib.ptr(state.sys_off),
                 self._maxn, _lib.ptr(state.positions), _lib.ptr(state.momenta),
                 _lib.ptr(state.forces), _lib.ptr(state.masses), _lib.ptr(state.tril),
                 _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction), seed,
-                int(state.step), None))
+                t, None, sd))
             self.forces(state, e2[k], elr[k], vir)
             _lib.check(lib.kupsmi_integrate(
                 _lib.stream_ptr(), 1, base, state.n_sys, _lib.ptr(state.sys_off),
                 self._maxn, _lib.ptr(state.positions), _lib.ptr(state.momenta),
                 _lib.ptr(state.forces), _lib.ptr(state.masses), _lib.ptr(state.tril),
                 _lib.ptr(state.time_step), _lib.ptr(kT), _lib.ptr(state.friction), seed,
-                int(state.step), _lib.ptr(kin[k])))
+                t, _lib.ptr(kin[k]), sd))
             if npt:
                 _lib.check(lib.kupsmi_thermostat_barostat(
                     _lib.stream_ptr(), 1, state.n_sys, _lib.ptr(state.sys_off),
                     _lib.ptr(state.positions), None, None, _lib.ptr(state.tril),
                     _lib.ptr(state.time_step), _lib.ptr(kT), None, _lib.ptr(state.barostat),
-                    _lib.ptr(vir), _lib.ptr(kin[k]), seed, int(state.step)))
+                    _lib.ptr(vir), _lib.ptr(kin[k]), seed, t, sd))
                 elr[k].zero_()
                 self.forces(state, e2[k], elr[k], vir)
-            state.step += 1
-        system = torch.repeat_interleave(torch.arange(state.n_sys, device=dev),
-                                         torch.as_tensor(state.sizes, device=dev))
-        es = self_energy(state.charges, system, self._alpha, state.n_sys)
+            if step_dev is not None:
+                _lib.check(lib.kupsmi_step_advance(_lib.stream_ptr(), sd, 1))
         return EwaldObservables(e2[:, :, 0].T, e2[:, :, 1].T, elr.T, es, kin.T)
+
+    def capture(self, state: MDState, n_steps: int) -> "GraphedSteps":
+        """Record n_steps of this propagator on ``state`` in a HIP graph
+        (torch.cuda.CUDAGraph); ``replay()`` then runs them with one launch
+        from the host.  The state's tensors are bound by address."""
+        return GraphedSteps(self, state, n_steps)
+
+
+class GraphedSteps:
+    """n_steps of an EwaldLJPropagator recorded in one HIP graph.
+
+    Every kernel of the steps (integrator phases, thermostat and barostat,
+    pair and reciprocal-space forces, Verlet-list maintenance) is recorded;
+    the step counter of the random streams is a device int64 that the
+    recorded kernels read and advance, set from ``state.step`` before each
+    replay, so replays draw the same noise as eager steps would.  Capture
+    first evaluates the forces once eagerly so every scratch buffer and
+    Verlet list is allocated before recording (the recorded launches
+    allocate nothing).  The observables of the last replay are in ``obs``."""
+
+    def __init__(self, prop: EwaldLJPropagator, state: MDState, n_steps: int):
+        self.prop, self.state, self.n = prop, state, int(n_steps)
+        prop._prep(state)
+        f0 = state.forces.clone()
+        prop.evaluate_forces(state)
+        state.forces.copy_(f0)
+        es = prop._self_energy(state)
+        dev = state.positions.device
+        self.step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.obs = prop._steps(state, self.n, self.step_dev, es)
+        torch.cuda.synchronize()
+
+    def replay(self) -> EwaldObservables:
+        self.step_dev.fill_(int(self.state.step))
+        self.graph.replay()
+        self.state.step += self.n
+        return self.obs

@@ -271,3 +271,45 @@ def test_ewald_md_csvr_and_npt_vs_oracle(gpu, npt):
         np.testing.assert_array_equal(st.tril.cpu().numpy(), b.tril)
     assert _rel_norm(st.forces.cpu().numpy(), frc) < 1e-4
     assert _rel_norm(st.momenta.cpu().numpy(), mom) < 1e-4
+
+
+@pytest.mark.parametrize("integ", ["baoab", "npt"])
+def test_ewald_md_graph_replays_match_eager_steps(gpu, integ):
+    """Steps recorded in a HIP graph (GraphedSteps) and replayed three times,
+    with eager work and synchronisations between the replays, give the same
+    bits as the same steps run eagerly: positions, momenta, forces, cells and
+    every energy (the random streams follow the device-side step counter)."""
+    from kups_amd.md.integrators import BAOAB_LANGEVIN, CSVR_NPT
+
+    b = workloads.nacl_melts(ms=[5, 6], seed=9)
+    p0 = maxwell_boltzmann_momenta(b.masses, b.sizes, 1200.0, np.random.default_rng(2))
+
+    def state():
+        return make_md_state(b.positions.astype(np.float32), b.masses, b.species, b.sizes, b.tril,
+                             12.0, time_step_fs=1.0, temperature=1200.0,
+                             momenta=p0.astype(np.float32), charges=b.charges,
+                             friction_per_fs=0.01, thermostat_time_constant_fs=20.0,
+                             target_pressure_pa=1.0e8, pressure_coupling_time_fs=20.0,
+                             compressibility_per_pa=2.0e-9)
+
+    ep = EwaldParameters.box(b.n_sys, alpha=0.3, cutoff=12.0, nmax=6)
+    integrator = BAOAB_LANGEVIN if integ == "baoab" else CSVR_NPT
+    skin = 1.0 if integ == "baoab" else None
+    sa, sb = state(), state()
+    pa = EwaldLJPropagator(NACL_LJ, ep, integrator, seed=5, skin=skin)
+    pb = EwaldLJPropagator(NACL_LJ, ep, integrator, seed=5, skin=skin)
+    pa.evaluate_forces(sa)
+    pb.evaluate_forces(sb)
+    graph = pb.capture(sb, 3)
+    for _ in range(3):
+        oa = pa(sa, 3)
+        ob = graph.replay()
+        torch.cuda.synchronize()
+        assert float(torch.ones(4096, device=gpu).sum().item()) == 4096.0  # eager work
+        for name in ("positions", "momenta", "forces", "tril"):
+            torch.testing.assert_close(getattr(sb, name), getattr(sa, name), rtol=0, atol=0)
+        torch.testing.assert_close(ob.potential_energy, oa.potential_energy, rtol=0, atol=0)
+        torch.testing.assert_close(ob.kinetic_energy, oa.kinetic_energy, rtol=0, atol=0)
+        assert sb.step == sa.step
+    if integ == "npt":
+        assert not torch.equal(sb.tril, torch.as_tensor(b.tril, device=gpu))

@@ -439,3 +439,23 @@ def test_md_full_batch_properties(gpu):
                                   [workloads.AR_CUTOFF], AR.sigma, AR.epsilon)
         assert obs.potential_energy[s, -1].item() == pytest.approx(e[0], rel=1e-5)
         _check_forces(st.forces.cpu().numpy()[a:z], f)
+
+
+def test_autotune_md_skin_restores_state(gpu):
+    """The skin autotuner times each candidate on the caller's state, puts the
+    state back bit for bit, returns a candidate and caches the choice."""
+    from kups_amd.core.autotune import tune_md_skin
+
+    b = _lj_batch([864, 500, 300], seed=3)
+    st = _md_state(b)
+    make_velocity_verlet_step(AR).evaluate_forces(st)
+    before = {k: getattr(st, k).clone() for k in ("positions", "momenta", "forces", "tril")}
+    step = st.step
+    r = tune_md_skin(AR, st, skins_sigma=(0.2, 0.3), steps=30)
+    assert len(r.table) == 2 and r.best in r.table and not r.cached
+    assert all(v > 0 for v in r.table.values())
+    for k, v in before.items():
+        assert torch.equal(getattr(st, k), v)
+    assert st.step == step
+    r2 = tune_md_skin(AR, st, skins_sigma=(0.2, 0.3), steps=30)
+    assert r2.cached and r2.best == r.best
