@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--cpu-sample-every", type=int, default=8,
                     help="CPU baseline runs every k-th system of the batch")
     ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--skin-sigma", type=float, default=None,
+                    help="Verlet skin of the MD leg in sigma (default: autotuned; the traffic "
+                         "probe defaults to 0.3)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="1 GiB device copy (counter calibration) + one MD launch, no JSON line")
     ap.add_argument("--no-ewald", action="store_true")
@@ -346,14 +349,17 @@ def gcmc_leg(args):
     return res
 
 
-def load_traffic():
+def load_traffic(skin_sigma: float):
+    """Measured HBM bytes of the 500-step launch at this skin (the profile
+    holds one entry per autotuner candidate), or None."""
     f = ROOT / "profiles" / "md_lj_kernel_traffic.json"
-    if f.exists():
-        try:
-            return json.loads(f.read_text())
-        except Exception:
-            return None
-    return None
+    if not f.exists():
+        return None
+    try:
+        data = json.loads(f.read_text())
+    except Exception:
+        return None
+    return data.get("by_skin_sigma", {}).get(f"{skin_sigma:.2f}")
 
 
 def main():
@@ -389,15 +395,19 @@ def main():
     prop = make_velocity_verlet_step(params)
     prop.evaluate_forces(st)
     tuned = None
-    if not args.traffic_probe:
+    if args.skin_sigma is not None or args.traffic_probe:
+        skin_sigma = 0.3 if args.skin_sigma is None else args.skin_sigma
+    else:
         # skin chosen by measurement (a few hundred steps on this state,
-        # restored afterwards); the traffic probe keeps the fixed 0.3 sigma
+        # restored afterwards)
         tuned = tune_md_skin(params, st)
-        prop = make_velocity_verlet_step(params, skin=tuned.best)
+        skin_sigma = tuned.best / workloads.AR_SIGMA
+    prop = make_velocity_verlet_step(params, skin=skin_sigma * workloads.AR_SIGMA)
     if args.traffic_probe:
         prop(st, args.steps, record=True)
         torch.cuda.synchronize()
-        print(f"traffic probe: {int(batch.sizes.sum())} atoms, {args.steps} steps in one launch",
+        print(f"traffic probe: {int(batch.sizes.sum())} atoms, {args.steps} steps in one launch, "
+              f"skin {skin_sigma:.2f} sigma",
               flush=True)
         return
     if args.warmup:
@@ -427,7 +437,7 @@ def main():
     steps_per_launch = float(np.mean([k for _, _, k in launches]))
     achieved_tflops = FLOP_PER_ATOM_STEP * atoms * steps_per_launch / (kern_ms * 1e-3) / 1e12
     etot = (obs.potential_energy[:, -1] + obs.kinetic_energy[:, -1]).double().sum().item()
-    traffic = load_traffic()
+    traffic = load_traffic(skin_sigma)
     line = {
         "metric": METRIC,
         "value": value,
@@ -445,7 +455,7 @@ def main():
             "workload": "batched LJ MD: 2048 cubic PBC boxes, N~U{64..4096} (seed 0), "
                         "rho*=0.8, jitter 0.05 sigma, argon LJ rc=2.5 sigma, NVE velocity "
                         "Verlet dt=2 fs, the exact cutoff pair set every step (Verlet lists, "
-                        f"skin {tuned.best / workloads.AR_SIGMA:.2f} sigma autotuned, rebuilt "
+                        f"skin {skin_sigma:.2f} sigma{' autotuned' if tuned else ''}, rebuilt "
                         "on the skin/2 displacement bound)",
             "n_systems": batch.n_sys,
             "n_atoms": atoms,
@@ -467,10 +477,11 @@ def main():
         },
         "final_total_energy_eV": etot,
     }
-    line["autotune"] = {"skin_sigma": tuned.best / workloads.AR_SIGMA,
-                        "ms_per_step": {f"{k / workloads.AR_SIGMA:.2f}": v
-                                        for k, v in tuned.table.items()},
-                        "cached": tuned.cached}
+    if tuned is not None:
+        line["autotune"] = {"skin_sigma": skin_sigma,
+                            "ms_per_step": {f"{k / workloads.AR_SIGMA:.2f}": v
+                                            for k, v in tuned.table.items()},
+                            "cached": tuned.cached}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(batch, params, args)
     del st, prop

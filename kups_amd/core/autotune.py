@@ -5,7 +5,10 @@ entries evaluated every step (more for a thick one).  The balance depends on
 the batch's density, temperature, time step, cutoff and system sizes, so it
 is measured rather than modelled: each candidate runs a few steps on the
 caller's state, timed with HIP events on the current stream, the state is
-put back, and the fastest candidate wins.  Results are cached per workload
+put back, and the fastest candidate wins unless it beats the default (the
+value the kernels were tuned and profiled at) by less than ``margin``, a
+timing-noise guard that keeps the choice stable between runs.  Results are
+cached per workload
 signature in memory and, when the environment variable KUPS_AMD_TUNE_CACHE
 names a JSON file, on disk.
 """
@@ -106,8 +109,9 @@ def _time_steps(run, steps: int) -> float:
     return e0.elapsed_time(e1) / steps
 
 
-def _tune(kind: str, state, candidates, make_runner, steps: int, extra=()) -> TuneResult:
-    key = workload_key(kind, state, tuple(extra) + tuple(candidates) + (steps,))
+def _tune(kind: str, state, candidates, make_runner, steps: int, extra=(), prefer=None,
+          margin: float = 0.03) -> TuneResult:
+    key = workload_key(kind, state, tuple(extra) + tuple(candidates) + (steps, prefer, margin))
     hit = _lookup(key)
     if hit is not None:
         return TuneResult(hit["best"], {float(k): v for k, v in hit["table"].items()}, key, True)
@@ -120,12 +124,16 @@ def _tune(kind: str, state, candidates, make_runner, steps: int, extra=()) -> Tu
         table[float(c)] = _time_steps(run, steps)
         _restore(state, snap)
     best = min(table, key=table.get)
+    if prefer is not None and float(prefer) in table and \
+            table[best] > (1.0 - margin) * table[float(prefer)]:
+        best = float(prefer)
     _store(key, {"best": best, "table": {str(k): v for k, v in table.items()}})
     return TuneResult(best, table, key, False)
 
 
 def tune_md_skin(params, state, integrator: int = 0, seed: int = 0,
-                 skins_sigma=(0.15, 0.2, 0.25, 0.3, 0.4), steps: int = 100) -> TuneResult:
+                 skins_sigma=(0.15, 0.2, 0.25, 0.3, 0.4), steps: int = 100,
+                 default_sigma: float = 0.3, margin: float = 0.03) -> TuneResult:
     """Skin of the LDS MD kernel's Verlet lists (FusedLJPropagator), candidates
     in units of the largest LJ sigma; each runs ``steps`` steps in one launch."""
     from ..md.integrators import FusedLJPropagator
@@ -137,11 +145,13 @@ def tune_md_skin(params, state, integrator: int = 0, seed: int = 0,
         prop = FusedLJPropagator(params, integrator, seed, skin)
         return lambda n: prop(state, n, record=False)
 
-    return _tune("md_lj", state, cands, make, steps, (integrator, sig))
+    prefer = default_sigma * sig if default_sigma in skins_sigma else None
+    return _tune("md_lj", state, cands, make, steps, (integrator, sig), prefer, margin)
 
 
 def tune_ewald_skin(params, ewald, state, integrator: int = 1, seed: int = 0,
-                    skins=(0.5, 1.0, 1.5, 2.0), steps: int = 12) -> TuneResult:
+                    skins=(0.5, 1.0, 1.5, 2.0), steps: int = 12, default: float = 1.0,
+                    margin: float = 0.03) -> TuneResult:
     """Skin (A) of the HBM path's Verlet lists (EwaldLJPropagator)."""
     from ..md.integrators import EwaldLJPropagator
 
@@ -150,4 +160,5 @@ def tune_ewald_skin(params, ewald, state, integrator: int = 1, seed: int = 0,
         return lambda n: prop(state, n)
 
     return _tune("ewald_lj", state, list(skins), make, steps,
-                 (integrator, tuple(np.asarray(ewald.alpha).ravel()[:1].tolist())))
+                 (integrator, tuple(np.asarray(ewald.alpha).ravel()[:1].tolist())),
+                 default if default in skins else None, margin)
