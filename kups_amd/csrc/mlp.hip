@@ -1,6 +1,6 @@
 // Learned GCMC insertion proposal: voxel descriptors, MLP inference (BF16
-// MFMA), MLP training (FP32 forward/backward with BF16-rounded weights, Adam)
-// and the per-chain proposal table.  No counterpart in the reference: the
+// MFMA), MLP training (FP32 MFMA forward/backward with BF16-rounded weights,
+// Adam) and the per-chain proposal table.  No counterpart in the reference: the
 // proposal enters kUPS's MCMC through the ChangesFn log ratio
 // (core/propagator.py:230-235, added at 332-334).
 //
@@ -192,108 +192,337 @@ __global__ __launch_bounds__(64 * ML_WAVES) void ml_infer_kernel(int n_rows, con
 }
 
 // ------------------------------------------------------------ training ----
-// One 128-thread workgroup per slice of a minibatch (thread u = hidden unit
-// u), forward and backward of every sample with BF16-rounded weights in FP32
-// arithmetic, gradients accumulated in LDS (147 KB) and written as one
-// partial per workgroup; loss = 1/2 mean (f(x) - y)^2.
-__global__ __launch_bounds__(ML_H) void ml_train_kernel(int n_samples, int slice,
-                                                        const int32_t* rows,
-                                                        const float* targets,
-                                                        const unsigned short* desc,
-                                                        const float* wq, float* partial,
-                                                        float* loss_partial, float inv_batch) {
-  __shared__ float gW1[ML_IN * ML_H];
-  __shared__ float gW2[ML_H * ML_H];
-  __shared__ float gW3[ML_H * ML_H];
-  __shared__ float x[ML_IN], h1[ML_H], h2[ML_H], h3[ML_H], g1[ML_H], g2[ML_H], g3[ML_H];
-  __shared__ float red[ML_H / 64];
-  const int u = threadIdx.x;
-  for (int e = u; e < ML_IN * ML_H; e += ML_H) gW1[e] = 0.f;
-  for (int e = u; e < ML_H * ML_H; e += ML_H) { gW2[e] = 0.f; gW3[e] = 0.f; }
-  float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, gw4 = 0.f, gb4 = 0.f, lsum = 0.f;
-  const float* W1 = wq + ML_OFF_W1;
-  const float* W2 = wq + ML_OFF_W2;
-  const float* W3 = wq + ML_OFF_W3;
-  const float w4u = wq[ML_OFF_W4 + u];
-  const int s0 = blockIdx.x * slice;
-  const int s1 = s0 + slice < n_samples ? s0 + slice : n_samples;
-  for (int s = s0; s < s1; ++s) {
-    __syncthreads();
-    if (u < ML_IN) x[u] = bf2f(desc[(size_t)rows[s] * ML_IN + u]);
-    __syncthreads();
-    float z1 = wq[ML_OFF_B1 + u];
-    for (int k = 0; k < ML_IN; ++k) z1 += x[k] * W1[k * ML_H + u];
-    const float a1 = silu(z1);
-    h1[u] = a1;
-    __syncthreads();
-    float z2 = wq[ML_OFF_B2 + u];
-    for (int j = 0; j < ML_H; ++j) z2 += h1[j] * W2[j * ML_H + u];
-    const float a2 = silu(z2);
-    h2[u] = a2;
-    __syncthreads();
-    float z3 = wq[ML_OFF_B3 + u];
-    for (int j = 0; j < ML_H; ++j) z3 += h2[j] * W3[j * ML_H + u];
-    const float a3 = silu(z3);
-    h3[u] = a3;
-    // output = b4 + sum_u h3[u] w4[u]
-    float p = a3 * w4u;
-    p = wave_sumf(p);
-    if ((u & 63) == 0) red[u >> 6] = p;
-    __syncthreads();
-    const float out = red[0] + red[1] + wq[ML_OFF_B4];
-    const float diff = out - targets[s];
-    if (u == 0) lsum += 0.5f * diff * diff;
-    const float dout = diff * inv_batch;
-    gb4 += dout;
-    gw4 += dout * a3;
-    auto dsilu = [](float z) {
-      const float sg = 1.0f / (1.0f + __expf(-z));
-      return sg * (1.0f + z * (1.0f - sg));
-    };
-    g3[u] = dout * w4u * dsilu(z3);
-    __syncthreads();
-    const float d3 = g3[u];
-    gb3 += d3;
-    for (int j = 0; j < ML_H; ++j) gW3[j * ML_H + u] += h2[j] * d3;
-    float e2 = 0.f;
-    for (int v = 0; v < ML_H; ++v) e2 += g3[v] * W3[u * ML_H + v];
-    g2[u] = e2 * dsilu(z2);
-    __syncthreads();
-    const float d2 = g2[u];
-    gb2 += d2;
-    for (int j = 0; j < ML_H; ++j) gW2[j * ML_H + u] += h1[j] * d2;
-    float e1 = 0.f;
-    for (int v = 0; v < ML_H; ++v) e1 += g2[v] * W2[u * ML_H + v];
-    g1[u] = e1 * dsilu(z1);
-    __syncthreads();
-    const float d1 = g1[u];
-    gb1 += d1;
-    for (int k = 0; k < ML_IN; ++k) gW1[k * ML_H + u] += x[k] * d1;
-  }
-  __syncthreads();
-  float* P = partial + (size_t)blockIdx.x * ML_NP;
-  for (int e = u; e < ML_IN * ML_H; e += ML_H) P[ML_OFF_W1 + e] = gW1[e];
-  for (int e = u; e < ML_H * ML_H; e += ML_H) { P[ML_OFF_W2 + e] = gW2[e]; P[ML_OFF_W3 + e] = gW3[e]; }
-  P[ML_OFF_B1 + u] = gb1;
-  P[ML_OFF_B2 + u] = gb2;
-  P[ML_OFF_B3 + u] = gb3;
-  P[ML_OFF_W4 + u] = gw4;
-  if (u == 0) {
-    P[ML_OFF_B4] = gb4;
-    loss_partial[blockIdx.x] = lsum;
+// Forward and backward on the matrix cores (v_mfma_f32_16x16x4f32: FP32
+// operands and accumulation) with the BF16-rounded weights; loss = 1/2 mean
+// (f(x) - y)^2.  One 256-thread workgroup per slice of ML_SLICE samples, taken
+// as 16-sample tiles.  Wave w owns hidden columns [32w, 32w + 32) of every
+// layer: it computes those columns of Z1..Z3 and of the back-propagated
+// errors, and the same columns of the weight gradients, which stay in
+// accumulator registers over the slice and are written once as the
+// workgroup's partial.  Weights sit in LDS as BF16 (the training copy is
+// BF16-rounded, so this is exact) in [in][out] order; activations and errors
+// as FP32.  A pass takes ML_RT 16-row tiles of samples, so every weight
+// fragment read from LDS feeds ML_RT independent accumulators.  Per 16
+// samples and wave: 16 + 64 + 64 forward MFMAs, 64 + 64 error and
+// 64 + 64 + 16 weight-gradient MFMAs.
+//
+// 16x16x4 operands: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k; result
+// D[i][j] in lane j + 16 (i / 4), element i % 4.
+constexpr int ML_SLICE = 32;
+constexpr int ML_RT = 2;         // 16-row sample tiles per pass
+constexpr int ML_ROWS = 16 * ML_RT;
+constexpr int ML_TW = 256;       // threads of a training workgroup (4 waves)
+constexpr int ML_LDW = ML_H + 8;    // BF16 weight rows
+constexpr int ML_LDA = ML_H + 20;   // FP32 activation / error rows
+constexpr int ML_LDX = ML_IN + 20;  // FP32 input rows
+
+__device__ inline f32x4 mfma4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ inline float dsilu(float z) {
+  const float sg = 1.0f / (1.0f + __expf(-z));
+  return sg * (1.0f + z * (1.0f - sg));
+}
+
+// z[rt][t] = H (row tile rt, 16 x K) . W[:, columns of tile nt0 + t], W in
+// [in][out] order
+template <int K, int LDH>
+__device__ inline void ml_fwd_cols(const float* H, const unsigned short* W, int nt0, int lane,
+                                   f32x4 (&z)[ML_RT][2]) {
+  const int i = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int rt = 0; rt < ML_RT; ++rt) z[rt][0] = z[rt][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    float a[ML_RT];
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt) a[rt] = H[(rt * 16 + i) * LDH + k0 + q];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const float b = bf2f(W[(k0 + q) * ML_LDW + (nt0 + t) * 16 + i]);
+#pragma unroll
+      for (int rt = 0; rt < ML_RT; ++rt) z[rt][t] = mfma4(a[rt], b, z[rt][t]);
+    }
   }
 }
 
-// Sum of the workgroup partials (fixed order), then Adam on the FP32 master
-// weights and the BF16 copies (rounded weights as FP32 for training, raw
-// BF16 for inference).
-__global__ void ml_adam_kernel(int n_part, const float* partial, float* grad_out, float* master,
-                               float* m1, float* m2, float* wq, unsigned short* wbf, float lr,
-                               float beta1, float beta2, float eps, int t, int apply) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ML_NP) return;
+// e[t][s][j] = sum_u D[s][u] W[j][u] for j in tile nt0 + t (the error of the
+// layer below, before its SiLU derivative)
+__device__ inline void ml_bwd_cols(const float* D, const unsigned short* W, int nt0, int lane,
+                                   f32x4 (&e)[ML_RT][2]) {
+  const int i = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int rt = 0; rt < ML_RT; ++rt) e[rt][0] = e[rt][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int k0 = 0; k0 < ML_H; k0 += 4) {
+    float a[ML_RT];
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt) a[rt] = D[(rt * 16 + i) * ML_LDA + k0 + q];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const float b = bf2f(W[((nt0 + t) * 16 + i) * ML_LDW + k0 + q]);
+#pragma unroll
+      for (int rt = 0; rt < ML_RT; ++rt) e[rt][t] = mfma4(a[rt], b, e[rt][t]);
+    }
+  }
+}
+
+// g[mt][t][j][u] += sum_s H[s][16 mt + j] D[s][16 (nt0 + t) + u] over the pass's samples
+template <int M, int LDH>
+__device__ inline void ml_wgrad(const float* H, const float* D, int nt0, int lane,
+                                f32x4 (&g)[M / 16][2]) {
+  const int i = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < ML_ROWS; k0 += 4) {
+    const float b0 = D[(k0 + q) * ML_LDA + nt0 * 16 + i];
+    const float b1 = D[(k0 + q) * ML_LDA + nt0 * 16 + 16 + i];
+#pragma unroll
+    for (int mt = 0; mt < M / 16; ++mt) {
+      const float a = H[(k0 + q) * LDH + mt * 16 + i];
+      g[mt][0] = mfma4(a, b0, g[mt][0]);
+      g[mt][1] = mfma4(a, b1, g[mt][1]);
+    }
+  }
+}
+
+__device__ inline void ml_load_weights(unsigned short* dst, const unsigned short* src, int rows,
+                                       int tid) {
+  // src rows of 128 BF16 values, eight per 16-byte load (LDS rows are 272 bytes)
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  for (int e = tid; e < rows * (ML_H / 8); e += ML_TW) {
+    const int r = e / (ML_H / 8), c = 8 * (e % (ML_H / 8));
+    *reinterpret_cast<uint4*>(dst + r * ML_LDW + c) = s[e];
+  }
+}
+
+__global__ __launch_bounds__(ML_TW) void ml_train_kernel(int n_samples, const int32_t* rows,
+                                                         const float* targets,
+                                                         const unsigned short* desc,
+                                                         const unsigned short* wbf, const float* wq,
+                                                         float* partial, float* loss_partial,
+                                                         float inv_batch) {
+  __shared__ __attribute__((aligned(16))) unsigned short sW1[ML_IN * ML_LDW];
+  __shared__ __attribute__((aligned(16))) unsigned short sW2[ML_H * ML_LDW], sW3[ML_H * ML_LDW];
+  __shared__ float sX[ML_ROWS * ML_LDX], sH1[ML_ROWS * ML_LDA], sH2[ML_ROWS * ML_LDA],
+      sD[ML_ROWS * ML_LDA];
+  __shared__ float sB1[ML_H], sB2[ML_H], sB3[ML_H], sW4[ML_H], sRed[4][ML_ROWS], sY[ML_ROWS];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, i = lane & 15, q = lane >> 4;
+  const int nt0 = 2 * wave;
+  ml_load_weights(sW1, wbf + ML_OFF_W1, ML_IN, tid);
+  ml_load_weights(sW2, wbf + ML_OFF_W2, ML_H, tid);
+  ml_load_weights(sW3, wbf + ML_OFF_W3, ML_H, tid);
+  if (tid < ML_H) {
+    sB1[tid] = wq[ML_OFF_B1 + tid];
+    sB2[tid] = wq[ML_OFF_B2 + tid];
+    sB3[tid] = wq[ML_OFF_B3 + tid];
+    sW4[tid] = wq[ML_OFF_W4 + tid];
+  }
+  const float b4 = wq[ML_OFF_B4];
+  f32x4 g1[ML_IN / 16][2], g2[ML_H / 16][2], g3[ML_H / 16][2];
+#pragma unroll
+  for (int mt = 0; mt < ML_H / 16; ++mt)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      g2[mt][t] = g3[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (mt < ML_IN / 16) g1[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  float gb1[2] = {0.f, 0.f}, gb2[2] = {0.f, 0.f}, gb3[2] = {0.f, 0.f}, gw4[2] = {0.f, 0.f};
+  float gb4 = 0.f, lsum = 0.f;
+  const int s0 = blockIdx.x * ML_SLICE;
+  const int s1 = s0 + ML_SLICE < n_samples ? s0 + ML_SLICE : n_samples;
+  for (int t0 = s0; t0 < s1; t0 += ML_ROWS) {
+    __syncthreads();  // the previous pass is done with sX, sH1, sH2, sD
+    for (int e = tid; e < ML_ROWS * ML_IN; e += ML_TW) {
+      const int r = e / ML_IN, c = e % ML_IN;
+      sX[r * ML_LDX + c] = t0 + r < s1 ? bf2f(desc[(size_t)rows[t0 + r] * ML_IN + c]) : 0.f;
+    }
+    if (tid < ML_ROWS) sY[tid] = t0 + tid < s1 ? targets[t0 + tid] : 0.f;
+    __syncthreads();
+    // forward; z1, z2, z3 keep the pre-activations of the wave's columns
+    f32x4 z1[ML_RT][2], z2[ML_RT][2], z3[ML_RT][2];
+    ml_fwd_cols<ML_IN, ML_LDX>(sX, sW1, nt0, lane, z1);
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = (nt0 + t) * 16 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          z1[rt][t][r] += sB1[c];
+          sH1[(rt * 16 + 4 * q + r) * ML_LDA + c] = silu(z1[rt][t][r]);
+        }
+      }
+    __syncthreads();
+    ml_fwd_cols<ML_H, ML_LDA>(sH1, sW2, nt0, lane, z2);
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = (nt0 + t) * 16 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          z2[rt][t][r] += sB2[c];
+          sH2[(rt * 16 + 4 * q + r) * ML_LDA + c] = silu(z2[rt][t][r]);
+        }
+      }
+    __syncthreads();
+    ml_fwd_cols<ML_H, ML_LDA>(sH2, sW3, nt0, lane, z3);
+    float h3[ML_RT][2][4], p[ML_RT][4];
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) p[rt][r] = 0.f;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = (nt0 + t) * 16 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          z3[rt][t][r] += sB3[c];
+          h3[rt][t][r] = silu(z3[rt][t][r]);
+          p[rt][r] += h3[rt][t][r] * sW4[c];
+        }
+      }
+      // output: row sums over the 16 lanes of a row group, then over the waves
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        p[rt][r] += __shfl_xor(p[rt][r], 1, 64);
+        p[rt][r] += __shfl_xor(p[rt][r], 2, 64);
+        p[rt][r] += __shfl_xor(p[rt][r], 4, 64);
+        p[rt][r] += __shfl_xor(p[rt][r], 8, 64);
+      }
+      if (i == 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sRed[wave][rt * 16 + 4 * q + r] = p[rt][r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt) {
+      float dout[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rt * 16 + 4 * q + r;
+        const float out = sRed[0][row] + sRed[1][row] + sRed[2][row] + sRed[3][row] + b4;
+        const float diff = t0 + row < s1 ? out - sY[row] : 0.f;
+        dout[r] = diff * inv_batch;
+        if (wave == 0 && i == 0) {
+          lsum += 0.5f * diff * diff;
+          gb4 += dout[r];
+        }
+      }
+      // layer 3 error
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = (nt0 + t) * 16 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          gw4[t] += dout[r] * h3[rt][t][r];
+          const float d = dout[r] * sW4[c] * dsilu(z3[rt][t][r]);
+          gb3[t] += d;
+          sD[(rt * 16 + 4 * q + r) * ML_LDA + c] = d;
+        }
+      }
+    }
+    __syncthreads();
+    f32x4 e[ML_RT][2];
+    ml_wgrad<ML_H, ML_LDA>(sH2, sD, nt0, lane, g3);
+    ml_bwd_cols(sD, sW3, nt0, lane, e);
+    __syncthreads();  // every wave has read the layer-3 error
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = (nt0 + t) * 16 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float d = e[rt][t][r] * dsilu(z2[rt][t][r]);
+          gb2[t] += d;
+          sD[(rt * 16 + 4 * q + r) * ML_LDA + c] = d;
+        }
+      }
+    __syncthreads();
+    ml_wgrad<ML_H, ML_LDA>(sH1, sD, nt0, lane, g2);
+    ml_bwd_cols(sD, sW2, nt0, lane, e);
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < ML_RT; ++rt)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = (nt0 + t) * 16 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float d = e[rt][t][r] * dsilu(z1[rt][t][r]);
+          gb1[t] += d;
+          sD[(rt * 16 + 4 * q + r) * ML_LDA + c] = d;
+        }
+      }
+    __syncthreads();
+    ml_wgrad<ML_IN, ML_LDX>(sX, sD, nt0, lane, g1);
+  }
+  // the workgroup's partial gradient
+  float* P = partial + (size_t)blockIdx.x * ML_NP;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int u = (nt0 + t) * 16 + i;
+#pragma unroll
+    for (int mt = 0; mt < ML_H / 16; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = mt * 16 + 4 * q + r;
+        P[ML_OFF_W2 + j * ML_H + u] = g2[mt][t][r];
+        P[ML_OFF_W3 + j * ML_H + u] = g3[mt][t][r];
+        if (mt < ML_IN / 16) P[ML_OFF_W1 + j * ML_H + u] = g1[mt][t][r];
+      }
+    float v1 = gb1[t], v2 = gb2[t], v3 = gb3[t], v4 = gw4[t];
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      v1 += __shfl_xor(v1, o, 64);
+      v2 += __shfl_xor(v2, o, 64);
+      v3 += __shfl_xor(v3, o, 64);
+      v4 += __shfl_xor(v4, o, 64);
+    }
+    if (q == 0) {
+      P[ML_OFF_B1 + u] = v1;
+      P[ML_OFF_B2 + u] = v2;
+      P[ML_OFF_B3 + u] = v3;
+      P[ML_OFF_W4 + u] = v4;
+    }
+  }
+  if (wave == 0) {
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      gb4 += __shfl_xor(gb4, o, 64);
+      lsum += __shfl_xor(lsum, o, 64);
+    }
+    if (lane == 0) {
+      P[ML_OFF_B4] = gb4;
+      loss_partial[blockIdx.x] = lsum;
+    }
+  }
+}
+
+// Sum of the workgroup partials, then Adam on the FP32 master weights and the
+// BF16 copies (rounded weights as FP32 for training, raw BF16 for inference).
+// A 256-thread block takes 32 parameters; its 8 thread rows each sum every
+// 8th partial and the 8 sums are added in a fixed order, so the result does
+// not depend on scheduling.
+constexpr int ML_RED_P = 32, ML_RED_S = 8;
+
+__global__ __launch_bounds__(ML_RED_P * ML_RED_S) void ml_adam_kernel(
+    int n_part, const float* partial, float* grad_out, float* master, float* m1, float* m2,
+    float* wq, unsigned short* wbf, float lr, float beta1, float beta2, float eps, int t,
+    int apply) {
+  __shared__ float red[ML_RED_S][ML_RED_P];
+  const int lp = threadIdx.x % ML_RED_P, sl = threadIdx.x / ML_RED_P;
+  const int p = blockIdx.x * ML_RED_P + lp;
   float g = 0.f;
-  for (int w = 0; w < n_part; ++w) g += partial[(size_t)w * ML_NP + p];
+  if (p < ML_NP)
+    for (int w = sl; w < n_part; w += ML_RED_S) g += partial[(size_t)w * ML_NP + p];
+  red[sl][lp] = g;
+  __syncthreads();
+  if (sl != 0 || p >= ML_NP) return;
+  g = 0.f;
+#pragma unroll
+  for (int k = 0; k < ML_RED_S; ++k) g += red[k][lp];
   if (grad_out) grad_out[p] = g;
   if (!apply) return;
   const float a = beta1 * m1[p] + (1 - beta1) * g;
@@ -424,20 +653,20 @@ extern "C" int kupsmi_mlp_train_step(kupsmi_ctx* ctx, void* stream, int n_sample
                                      float* loss_out) {
   KUPSMI_REQUIRE(ctx != nullptr, "null context");
   KUPSMI_REQUIRE(n_samples >= 1, "empty minibatch");
-  const int slice = 32;
-  const int n_part = (n_samples + slice - 1) / slice;
+  KUPSMI_REQUIRE(((uintptr_t)weights_bf16 & 15) == 0, "BF16 weights must be 16-byte aligned");
+  const int n_part = (n_samples + ML_SLICE - 1) / ML_SLICE;
   void* buf = nullptr;
   int rc = scratch(ctx, 5, sizeof(float) * ((size_t)n_part * ML_NP + n_part + 64), &buf);
   if (rc) return rc;
   float* partial = static_cast<float*>(buf);
   float* loss_partial = loss_out ? loss_out : partial + (size_t)n_part * ML_NP;
   hipStream_t st = (hipStream_t)stream;
-  ml_train_kernel<<<n_part, ML_H, 0, st>>>(n_samples, slice, rows, targets, desc, weights_f32,
-                                           partial, loss_partial, 1.0f / n_samples);
+  ml_train_kernel<<<n_part, ML_TW, 0, st>>>(n_samples, rows, targets, desc, weights_bf16,
+                                            weights_f32, partial, loss_partial, 1.0f / n_samples);
   KUPSMI_CHECK_LAUNCH();
-  ml_adam_kernel<<<(ML_NP + 255) / 256, 256, 0, st>>>(n_part, partial, grad_out, master, m1, m2,
-                                                      weights_f32, weights_bf16, lr, 0.9f,
-                                                      0.999f, 1e-8f, adam_step, apply);
+  ml_adam_kernel<<<(ML_NP + ML_RED_P - 1) / ML_RED_P, ML_RED_P * ML_RED_S, 0, st>>>(
+      n_part, partial, grad_out, master, m1, m2, weights_f32, weights_bf16, lr, 0.9f, 0.999f,
+      1e-8f, adam_step, apply);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }
