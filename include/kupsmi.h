@@ -318,7 +318,10 @@ int kupsmi_step_advance(void* stream, int64_t* step_dev, int64_t n);
  *                  configuration; a move of an empty chain is accepted but not counted),
  *                  insertions, deletions
  *   trace          optional [n_trace x n_steps x 4] f64 (move, dE, log p, accepted)
- * Random numbers: Philox4x32-10 keyed by seed, counters (chain, step, tag).
+ * Random numbers: Philox4x32-10 keyed by seed, counters (chain, step, tag); the
+ * first trial's step counter is step0, or *step_dev when step_dev is not NULL
+ * (a device int64 that a caller advances between replays of a captured HIP
+ * graph, e.g. with kupsmi_step_advance).
  */
 int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t step0, uint64_t seed,
                     int max_host, const int32_t* host_off, const float* host_xyzq,
@@ -333,7 +336,7 @@ int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t step0, uint6
                     double* energy, float* width, uint32_t* hist, int32_t* hist_idx,
                     long long* stats, int n_trace, double* trace, const float* prop_cdf,
                     const int32_t* prop_guide, int prop_nv, int* sample_count, int sample_cap,
-                    int32_t* sample_row, float* sample_y);
+                    int32_t* sample_row, float* sample_y, const int64_t* step_dev);
 
 /* ---- learned insertion proposal (no counterpart in the reference; it enters
  * kUPS's MCMC through the ChangesFn proposal log ratio,

@@ -47,6 +47,7 @@ constexpr uint32_t GC_TAG_BATCH = 0x42415443u, GC_TAG_A = 0x47434d41u,
 struct GcArgs {
   int n_chain, n_steps, gmax, m, nspec, nk, nm, n_trace, max_host;
   int64_t step0;
+  const int64_t* step_dev;  // when set, the first trial's counter is read here (graph replays)
   uint64_t seed;
   double rc_lj, rc_cl;
   double w_move[4], upper[4];
@@ -417,8 +418,9 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
     double energy = a.energy[ch];
     __syncthreads();
 
+    const int64_t step0 = a.step_dev ? a.step_dev[0] : a.step0;
     for (int st = 0; st < a.n_steps; ++st) {
-      const int64_t t = a.step0 + st;
+      const int64_t t = step0 + st;
       // ---- 1. draws and logs (wave 0), proposal (lane 0)
       double lu = 0.0, ln_np1 = 0.0, ln_n = 0.0;
       if (wv == 0) {
@@ -668,7 +670,7 @@ extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t s
                                long long* stats, int n_trace, double* trace,
                                const float* prop_cdf, const int32_t* prop_guide, int prop_nv,
                                int* sample_count, int sample_cap, int32_t* sample_row,
-                               float* sample_y) {
+                               float* sample_y, const int64_t* step_dev) {
   KUPSMI_REQUIRE(n_chain >= 0 && n_steps >= 0, "negative size");
   KUPSMI_REQUIRE(m >= 1 && m <= GC_MSITE, "molecules must have 1..4 sites");
   KUPSMI_REQUIRE(n_species >= 1 && n_species <= GC_NSPEC, "n_species must be in [1, 16]");
@@ -682,7 +684,7 @@ extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t s
   GcArgs a{};
   a.n_chain = n_chain; a.n_steps = n_steps; a.gmax = gmax; a.m = m; a.nspec = n_species;
   a.nk = nk; a.nm = nm; a.n_trace = n_trace; a.max_host = max_host;
-  a.step0 = step0; a.seed = seed; a.rc_lj = rc_lj; a.rc_cl = rc_coul;
+  a.step0 = step0; a.step_dev = step_dev; a.seed = seed; a.rc_lj = rc_lj; a.rc_cl = rc_coul;
   for (int k = 0; k < 4; ++k) {
     a.w_move[k] = move_weights ? move_weights[k] : 1.0;
     a.upper[k] = width_upper ? width_upper[k] : 1e30;

@@ -456,8 +456,18 @@ class RigidGCMC:
         LearnedProposal (its table biases insertions, with the exact density in
         the acceptance ratio) or None (uniform).  ``samples``: an InsertionSamples
         buffer that records (voxel row, -dE/kT) of every insertion trial."""
+        trace = torch.zeros((n_trace, max(n_steps, 1), 4), dtype=torch.float64,
+                            device=self.dev) if n_trace else None
+        self._launch(n_steps, n_trace, trace, proposal, samples)
+        self.step += int(n_steps)
+        return trace
+
+    def capture(self, n_steps: int, launches: int = 1, proposal=None) -> "GraphedTrials":
+        """``launches`` x ``n_steps`` trial moves recorded in one HIP graph."""
+        return GraphedTrials(self, n_steps, launches, proposal)
+
+    def _launch(self, n_steps, n_trace=0, trace=None, proposal=None, samples=None, step_dev=None):
         d = self.d
-        trace = torch.zeros((max(n_trace, 1), max(n_steps, 1), 4), dtype=torch.float64, device=self.dev)
         cdf = proposal.cdf if proposal is not None else None
         guide = proposal.guide if proposal is not None else None
         nv = proposal.nv if proposal is not None else (samples.nv if samples is not None else 0)
@@ -478,9 +488,7 @@ class RigidGCMC:
             _lib.ptr(cdf), _lib.ptr(guide), int(nv), _lib.ptr(samples.count) if samples is not None else None,
             int(samples.capacity) if samples is not None else 0,
             _lib.ptr(samples.rows) if samples is not None else None,
-            _lib.ptr(samples.y) if samples is not None else None))
-        self.step += int(n_steps)
-        return trace[:n_trace] if n_trace else None
+            _lib.ptr(samples.y) if samples is not None else None, _lib.ptr(step_dev)))
 
     @property
     def loading(self) -> torch.Tensor:
@@ -490,6 +498,37 @@ class RigidGCMC:
     def stats(self) -> dict:
         s = self.d["stats"].cpu().numpy()
         return {"trials": s[:, :4], "accepted": s[:, 4:8], "insertions": s[:, 8], "deletions": s[:, 9]}
+
+
+class GraphedTrials:
+    """``launches`` x ``n_steps`` trial moves of a RigidGCMC batch recorded in
+    one HIP graph (a whole MC cycle replayed with one call).  The trial counter
+    of the random streams is a device int64 that the recorded GCMC launches
+    read and that a recorded kernel advances between them; it is set from
+    ``gcmc.step`` before each replay, so a replay draws the same numbers as the
+    same trials run eagerly.  The learned proposal's table, if given, is read
+    from its buffers at replay time (rebuilding the table in place is seen by
+    later replays)."""
+
+    def __init__(self, gcmc: RigidGCMC, n_steps: int, launches: int = 1, proposal=None):
+        if n_steps < 1 or launches < 1:
+            raise ValueError("a captured cycle needs at least one launch of one trial")
+        self.gcmc, self.n, self.launches = gcmc, int(n_steps), int(launches)
+        self.step_dev = torch.zeros(1, dtype=torch.int64, device=gcmc.dev)
+        lib = _lib.load()
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for _ in range(self.launches):
+                gcmc._launch(self.n, proposal=proposal, step_dev=self.step_dev)
+                _lib.check(lib.kupsmi_step_advance(_lib.stream_ptr(), _lib.ptr(self.step_dev),
+                                                   self.n))
+        torch.cuda.synchronize()
+
+    def replay(self) -> None:
+        self.step_dev.fill_(int(self.gcmc.step))
+        self.graph.replay()
+        self.gcmc.step += self.n * self.launches
 
 
 def make_co2_gcmc(hosts, *, gmax: int = 64, seed: int = 0, weights=(1.0, 1.0, 1.0, 3.0),
