@@ -72,6 +72,29 @@ int kupsmi_neighborlist(kupsmi_ctx* ctx, void* stream, int n_sys,
                         int32_t* edges_ij, int32_t* edges_shift, int64_t* count);
 
 /*
+ * Edges that moving k particles adds and removes.  Replaces
+ * neighborlist_changes (core/neighborlist.py:1424-1515) for the full list
+ * above: moved[k] are distinct particle indices, moved_pos[k x 3] their new
+ * positions.  removed = every directed edge of the current configuration with
+ * an end among the moved particles; added = every directed edge of the
+ * configuration with the moved particles at moved_pos that has an end among
+ * them.  Both use the pair decision of kupsmi_neighborlist and report the
+ * particles' own indices (the reference's remap of appended entries).  Each
+ * output is written up to its capacity, with the required edge count in
+ * *count_added / *count_removed (device int64; > capacity means regrow and
+ * call again, -1 in both means an index in moved is out of range or
+ * repeated).  Order: moved particle, partner, image, edge before its reverse.
+ */
+int kupsmi_neighborlist_changes(kupsmi_ctx* ctx, void* stream, int n_sys,
+                                const int32_t* sys_off, int n_atoms, const float* pos,
+                                const int32_t* exclusion, const double* tril,
+                                const double* cutoff, int k, const int32_t* moved,
+                                const float* moved_pos, int64_t capacity_added,
+                                int32_t* added_ij, int32_t* added_shift, int64_t* count_added,
+                                int64_t capacity_removed, int32_t* removed_ij,
+                                int32_t* removed_shift, int64_t* count_removed);
+
+/*
  * Morse bond energy, forces and virial over an explicit bond list.  Replaces
  * morse_bond_energy (potential/classical/morse.py:142-168) and its gradient:
  * for bond b = (i, j) with image shift s_b, d = r_j - r_i + s_b @ L and

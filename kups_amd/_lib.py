@@ -27,6 +27,10 @@ _SIGNATURES = {
     "kupsmi_ctx_create": ([C.POINTER(vp)], C.c_int),
     "kupsmi_ctx_destroy": ([vp], C.c_int),
     "kupsmi_neighborlist": ([vp, vp, i32, vp, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
+    "kupsmi_neighborlist_changes": (
+        [vp, vp, i32, vp, i32, vp, vp, vp, vp, i32, vp, vp, i64, vp, vp, vp, i64, vp, vp, vp],
+        C.c_int,
+    ),
     "kupsmi_lj_forces": ([vp, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp], C.c_int),
     "kupsmi_md_lj": (
         [vp, vp, i32, i32, i64, u64, i32, vp, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp,

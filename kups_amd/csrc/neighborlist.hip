@@ -104,6 +104,124 @@ __global__ void nl_total_kernel(int n, const int64_t* offsets, const int64_t* co
   if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? offsets[n - 1] + counts[n - 1] : 0;
 }
 
+
+// ------------------------------------------------------------- changes ----
+// Edges that moving k particles removes and adds (neighborlist_changes,
+// core/neighborlist.py:1424-1515).  removed = the directed edges of the
+// current configuration with an end among the moved particles; added = the
+// same for the configuration with the moved particles at their new positions
+// (indices are the particles' own, as the reference remaps them).  Both use
+// the full list's pair decision above (FP64 fractional formula, image grid,
+// minimum-image exclusion).  One thread per (moved particle, configuration)
+// scans its system: an edge to an unmoved partner is written in both
+// directions, an edge between two moved particles (or a moved particle and
+// its own image) only from its first end, so every directed edge appears
+// once.  Order: moved particle, partner, image, then the edge before its
+// reverse.
+struct NcArgs {
+  int n_sys, k;
+  const int32_t* sys_off;
+  const float* pos;
+  const int32_t* excl;
+  const double* tril;
+  const double* cutoff;
+  const int32_t* moved;     // k particle indices
+  const float* moved_pos;   // k x 3 new positions
+  const int32_t* slot;      // per particle: its index in moved, or -1
+  int64_t cap[2];           // removed, added
+  int64_t* counts;          // 2 x k
+  const int64_t* offsets;   // 2 x k
+  int32_t* ij[2];
+  int32_t* shift[2];
+};
+
+__global__ void nc_slot_clear_kernel(int n, int32_t* slot, int32_t* bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) slot[i] = -1;
+  if (i == 0) *bad = 0;
+}
+
+// slot[moved[m]] = m; an index outside [0, n) or named twice sets *bad
+__global__ void nc_slot_set_kernel(int k, int n, const int32_t* moved, int32_t* slot,
+                                   int32_t* bad) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= k) return;
+  const int c = moved[m];
+  if (c < 0 || c >= n) { *bad = 1; return; }
+  if (atomicExch(slot + c, m) != -1) *bad = 1;
+}
+
+#pragma clang fp contract(off)
+template <bool WRITE>
+__global__ __launch_bounds__(NL_THREADS) void nc_kernel(NcArgs a, int n_atoms, const int32_t* bad) {
+  const int m = blockIdx.x * NL_THREADS + threadIdx.x;
+  const int cfg = blockIdx.y;  // 0: current positions (removed), 1: moved (added)
+  if (m >= a.k) return;
+  const int c = a.moved[m];
+  int64_t cnt = 0;
+  if (*bad == 0 && c >= 0 && c < n_atoms) {
+    int lo = 0, hi = a.n_sys - 1;  // the system of c: largest s with sys_off[s] <= c
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.sys_off[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    const int i0 = a.sys_off[lo], i1 = a.sys_off[lo + 1];
+    SysGeom g;
+    make_geom(a.tril + 6 * lo, a.cutoff[lo], 1, g);
+    const int h0 = g.rep ? g.h[0] : 0, h1 = g.rep ? g.h[1] : 0, h2 = g.rep ? g.h[2] : 0;
+    double fc0, fc1, fc2;
+    frac64(g.Li, cfg ? a.moved_pos + 3 * m : a.pos + 3 * c, fc0, fc1, fc2);
+    const int ec = a.excl[c];
+    const int64_t cap = a.cap[cfg];
+    int32_t* ij = a.ij[cfg];
+    int32_t* sh = a.shift[cfg];
+    int64_t out = WRITE ? a.offsets[cfg * a.k + m] : 0;
+    auto emit = [&](int p, int q, int s0, int s1, int s2) {
+      if (WRITE && out < cap) {
+        ij[2 * out] = p; ij[2 * out + 1] = q;
+        sh[3 * out] = s0; sh[3 * out + 1] = s1; sh[3 * out + 2] = s2;
+      }
+      ++out; ++cnt;
+    };
+    for (int j = i0; j < i1; ++j) {
+      const int sj = a.slot[j];
+      const bool moved_j = sj >= 0;
+      double fj0, fj1, fj2;
+      frac64(g.Li, cfg && moved_j ? a.moved_pos + 3 * sj : a.pos + 3 * j, fj0, fj1, fj2);
+      const double d0 = fc0 - fj0, d1 = fc1 - fj1, d2 = fc2 - fj2;
+      const double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
+      const bool same = ec == a.excl[j];
+      if (!g.rep) {
+        if (dist2_64(g.L, d0 - m0, d1 - m1, d2 - m2) < g.rc2 && !same) {
+          emit(c, j, (int)m0, (int)m1, (int)m2);
+          if (!moved_j) emit(j, c, -(int)m0, -(int)m1, -(int)m2);
+        }
+        continue;
+      }
+      for (int o0 = -h0; o0 <= h0; ++o0)
+        for (int o1 = -h1; o1 <= h1; ++o1)
+          for (int o2 = -h2; o2 <= h2; ++o2) {
+            const bool is_min = o0 == m0 && o1 == m1 && o2 == m2;
+            if (dist2_64(g.L, d0 - o0, d1 - o1, d2 - o2) < g.rc2 && !(same && is_min)) {
+              emit(c, j, o0, o1, o2);
+              if (!moved_j) emit(j, c, -o0, -o1, -o2);
+            }
+          }
+    }
+  }
+  if (!WRITE) a.counts[cfg * a.k + m] = cnt;
+}
+#pragma clang fp contract(on)
+
+// totals; -1 in both when an index was out of range or repeated
+__global__ void nc_total_kernel(int k, const int64_t* offsets, const int64_t* counts,
+                                const int32_t* bad, int64_t* removed, int64_t* added) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const bool b = *bad != 0;
+  *removed = b ? -1 : (k > 0 ? offsets[k - 1] + counts[k - 1] : 0);
+  *added = b ? -1 : (k > 0 ? offsets[2 * k - 1] + counts[2 * k - 1] : 0);
+}
+
 }  // namespace kupsmi
 
 using namespace kupsmi;
@@ -144,6 +262,65 @@ extern "C" int kupsmi_neighborlist(kupsmi_ctx* ctx, void* stream, int n_sys,
   nl_dense_kernel<true><<<grid, NL_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   nl_total_kernel<<<1, 1, 0, st>>>(n_atoms, offsets, counts, count);
+  KUPSMI_CHECK_LAUNCH();
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_neighborlist_changes(
+    kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off, int n_atoms,
+    const float* pos, const int32_t* exclusion, const double* tril, const double* cutoff,
+    int k, const int32_t* moved, const float* moved_pos, int64_t capacity_added,
+    int32_t* added_ij, int32_t* added_shift, int64_t* count_added, int64_t capacity_removed,
+    int32_t* removed_ij, int32_t* removed_shift, int64_t* count_removed) {
+  KUPSMI_REQUIRE(ctx != nullptr, "null context");
+  KUPSMI_REQUIRE(n_sys >= 1 && n_atoms >= 1 && k >= 0, "bad sizes");
+  KUPSMI_REQUIRE(capacity_added >= 0 && capacity_removed >= 0, "negative capacity");
+  KUPSMI_REQUIRE(count_added != nullptr && count_removed != nullptr, "count pointers required");
+  KUPSMI_REQUIRE(k == 0 || (moved != nullptr && moved_pos != nullptr), "moved particles required");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t kk = (size_t)(k > 0 ? k : 1);
+  size_t temp_bytes = 0;
+  KUPSMI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, (int64_t*)nullptr,
+                                              (int64_t*)nullptr, (int)kk, st));
+  // scratch: counts | offsets | slot | bad | scan temp, each 256-byte aligned
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_off = al(2 * kk * sizeof(int64_t));
+  const size_t o_slot = o_off + o_off;
+  const size_t o_bad = o_slot + al((size_t)n_atoms * sizeof(int32_t));
+  const size_t o_temp = o_bad + 256;
+  void* buf = nullptr;
+  int rc = scratch(ctx, 7, o_temp + temp_bytes, &buf);
+  if (rc) return rc;
+  char* base = static_cast<char*>(buf);
+  int64_t* counts = reinterpret_cast<int64_t*>(base);
+  int64_t* offsets = reinterpret_cast<int64_t*>(base + o_off);
+  int32_t* slot = reinterpret_cast<int32_t*>(base + o_slot);
+  int32_t* bad = reinterpret_cast<int32_t*>(base + o_bad);
+  void* temp = base + o_temp;
+  nc_slot_clear_kernel<<<(n_atoms + 255) / 256, 256, 0, st>>>(n_atoms, slot, bad);
+  KUPSMI_CHECK_LAUNCH();
+  if (k == 0) {
+    nc_total_kernel<<<1, 1, 0, st>>>(0, offsets, counts, bad, count_removed, count_added);
+    KUPSMI_CHECK_LAUNCH();
+    return KUPSMI_OK;
+  }
+  nc_slot_set_kernel<<<(k + 255) / 256, 256, 0, st>>>(k, n_atoms, moved, slot, bad);
+  KUPSMI_CHECK_LAUNCH();
+  NcArgs a{};
+  a.n_sys = n_sys; a.k = k; a.sys_off = sys_off; a.pos = pos; a.excl = exclusion;
+  a.tril = tril; a.cutoff = cutoff; a.moved = moved; a.moved_pos = moved_pos; a.slot = slot;
+  a.cap[0] = capacity_removed; a.cap[1] = capacity_added;
+  a.counts = counts; a.offsets = offsets;
+  a.ij[0] = removed_ij; a.ij[1] = added_ij;
+  a.shift[0] = removed_shift; a.shift[1] = added_shift;
+  const dim3 grid((k + NL_THREADS - 1) / NL_THREADS, 2);
+  nc_kernel<false><<<grid, NL_THREADS, 0, st>>>(a, n_atoms, bad);
+  KUPSMI_CHECK_LAUNCH();
+  KUPSMI_HIP(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, offsets, k, st));
+  KUPSMI_HIP(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts + k, offsets + k, k, st));
+  nc_kernel<true><<<grid, NL_THREADS, 0, st>>>(a, n_atoms, bad);
+  KUPSMI_CHECK_LAUNCH();
+  nc_total_kernel<<<1, 1, 0, st>>>(k, offsets, counts, bad, count_removed, count_added);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }

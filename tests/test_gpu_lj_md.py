@@ -14,7 +14,7 @@ import torch
 import oracle
 from kups_amd import workloads
 from kups_amd.core.batch import ParticleBatch, SystemBatch
-from kups_amd.core.neighborlist import CellListNeighborList
+from kups_amd.core.neighborlist import CellListNeighborList, neighborlist_changes
 from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
 from kups_amd.md.integrators import (VELOCITY_VERLET, FusedLJPropagator,
                                      make_baoab_langevin_step, make_csvr_npt_step,
@@ -99,6 +99,98 @@ def test_neighborlist_triclinic_and_capacity_regrow(gpu):
 def test_neighborlist_empty_and_single(gpu):
     gi, gs = gpu_nl(np.zeros((1, 3)), [1], [[20, 0, 20, 0, 0, 20]], 5.0)
     assert len(gi) == 0
+
+
+def _rows(ij, sh):
+    a = np.concatenate([np.asarray(ij, np.int64).reshape(-1, 2),
+                        np.asarray(sh, np.int64).reshape(-1, 3)], axis=1)
+    return a[np.lexsort(a.T[::-1])] if len(a) else a.reshape(0, 5)
+
+
+def gpu_changes(pos, sizes, tril, cutoff, moved, new_pos, excl=None, compaction=0.5,
+                avg_edges=1, dev="cuda"):
+    pos = np.asarray(pos, np.float32)
+    system = np.repeat(np.arange(len(sizes)), sizes).astype(np.int32)
+    pb = ParticleBatch(torch.as_tensor(pos, device=dev), torch.as_tensor(system, device=dev),
+                       exclusion=None if excl is None else torch.as_tensor(np.asarray(excl, np.int32), device=dev))
+    sb = SystemBatch(torch.as_tensor(np.asarray(tril, np.float64).reshape(-1, 6), device=dev))
+    cut = torch.as_tensor(np.broadcast_to(np.asarray(cutoff, np.float64), (len(sizes),)).copy(), device=dev)
+    r = neighborlist_changes(CellListNeighborList(avg_edges=avg_edges), pb,
+                             torch.as_tensor(np.asarray(moved, np.int32), device=dev),
+                             torch.as_tensor(np.asarray(new_pos, np.float32).reshape(-1, 3), device=dev),
+                             sb, cut, compaction)
+    return (_rows(r.added.indices.cpu().numpy(), r.added.shifts[:, 0].cpu().numpy()),
+            _rows(r.removed.indices.cpu().numpy(), r.removed.shifts[:, 0].cpu().numpy()))
+
+
+def ref_changes(pos, sizes, tril, cutoff, moved, new_pos, excl=None):
+    """The full lists before and after the move, restricted to edges with an
+    end among the moved particles (what neighborlist_changes reports, as the
+    reference's test_neighborlist.py:1551-1607 checks with separate calls)."""
+    pos = np.asarray(pos, np.float32)
+    after = pos.copy()
+    after[np.asarray(moved)] = np.asarray(new_pos, np.float32).reshape(-1, 3)
+    out = []
+    for p in (after, pos):
+        ij, sh = ref_nl(p, sizes, tril, cutoff, excl)
+        keep = np.isin(ij[:, 0], moved) | np.isin(ij[:, 1], moved)
+        out.append(_rows(ij[keep], sh[keep]))
+    return out
+
+
+def test_neighborlist_changes_reference_cases(gpu):
+    # test_neighborlist.py:1609-1689: one particle moves next to another; two systems
+    box = [10, 0, 10, 0, 0, 10]
+    for compaction in (0.5, 0.75, 1.0):
+        added, removed = gpu_changes([[0, 0, 0], [1, 0, 0], [5, 0, 0]], [3], [box], 1.5, [1],
+                                     [[4.5, 0, 0]], compaction=compaction)
+        assert {tuple(r[:2]) for r in removed} == {(0, 1), (1, 0)}
+        assert {tuple(r[:2]) for r in added} == {(1, 2), (2, 1)}
+    added, removed = gpu_changes([[0, 0, 0], [5, 0, 0], [0, 0, 0], [5, 0, 0]], [2, 2],
+                                 [box, box], 1.5, [1], [[0.5, 0, 0]])
+    assert {tuple(r[:2]) for r in added} == {(0, 1), (1, 0)}
+    assert len(removed) == 0
+
+
+def test_neighborlist_changes_match_full_lists(gpu):
+    """Batched boxes (image grid for the 64/90-atom ones), molecules as
+    exclusion groups, moved sets with neighbouring moved particles, a whole
+    small system moved; bit-exact edge arrays, edges each exactly once."""
+    b = workloads.batched_lj_boxes(sizes=[64, 90, 333, 1000], seed=5)
+    rng = np.random.default_rng(9)
+    n = int(np.sum(b.sizes))
+    off = np.concatenate([[0], np.cumsum(b.sizes)])
+    excl = np.arange(n) // 3
+    moved = np.concatenate([np.arange(off[0], off[1]),                  # all of box 0
+                            rng.choice(np.arange(off[1], off[2]), 7, replace=False),
+                            [off[2] + 10, off[2] + 11, off[2] + 12],      # one molecule
+                            rng.choice(np.arange(off[3], off[4]), 20, replace=False)])
+    new_pos = np.asarray(b.positions, np.float64)[moved] + rng.normal(0, 0.8, (len(moved), 3))
+    for ex in (None, excl):
+        added, removed = gpu_changes(b.positions, b.sizes, b.tril, workloads.AR_CUTOFF, moved,
+                                     new_pos, excl=ex)
+        ra, rr = ref_changes(b.positions, b.sizes, b.tril, workloads.AR_CUTOFF, moved, new_pos,
+                             excl=ex)
+        assert len(ra) > 1000 and len(rr) > 1000
+        np.testing.assert_array_equal(added, ra)
+        np.testing.assert_array_equal(removed, rr)
+
+
+def test_neighborlist_changes_triclinic_and_edge_cases(gpu):
+    rng = np.random.default_rng(4)
+    lat = np.array([[9.0, 0, 0], [2.0, 8.5, 0], [1.5, -1.0, 9.5]])
+    pos = rng.uniform(-0.5, 0.5, (120, 3)) @ lat
+    tril = lat[[0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]][None]
+    moved = [3, 50, 51, 119]
+    new_pos = rng.uniform(-0.5, 0.5, (4, 3)) @ lat
+    added, removed = gpu_changes(pos, [120], tril, 4.2, moved, new_pos)
+    ra, rr = ref_changes(pos, [120], tril, 4.2, moved, new_pos)
+    np.testing.assert_array_equal(added, ra)
+    np.testing.assert_array_equal(removed, rr)
+    added, removed = gpu_changes(pos, [120], tril, 4.2, [], np.zeros((0, 3)))
+    assert len(added) == 0 and len(removed) == 0
+    with pytest.raises(ValueError):
+        gpu_changes(pos, [120], tril, 4.2, [5, 5], new_pos[:2])
 
 
 def _lj_batch(sizes, seed):
