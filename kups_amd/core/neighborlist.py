@@ -55,7 +55,8 @@ class DenseNearestNeighborList:
                  cutoffs: torch.Tensor, rh_index_remap=None) -> Edges:
         if rh is not None or rh_index_remap is not None:
             raise NotImplementedError(
-                "rh queries are served inside the GCMC kernels; the export takes rh=None")
+                "rh queries are served by neighborlist_changes (the edges a move removes and "
+                "adds) and inside the GCMC kernels; the full list takes rh=None")
         lib = _lib.load()
         n_sys = systems.n_sys
         sys_off, _ = system_offsets(lh.system, n_sys)
