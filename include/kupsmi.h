@@ -210,6 +210,12 @@ int kupsmi_normals(void* stream, uint64_t seed, int64_t step, int n_atoms,
  * (application/simulations/mcmc_rigid.py:156-165).
  * energy: n_sys x 2 (LJ, real-space) FP64; virial n_sys x 9.  accumulate != 0
  * adds into forces/energy/virial instead of overwriting.
+ * lj_truncation (n_sys FP64 or NULL): the smooth pairwise switch of
+ * pair_tail_corrected_lennard_jones_energy (lennard_jones.py:200-222): pair
+ * energies between r_tr and the LJ cutoff are multiplied by
+ * S = (rc^2 - r^2)^2 (2 r^2 + rc^2 - 3 r_tr^2) / (rc^2 - r_tr^2)^3 and the
+ * forces and virial are those of the switched energy; a system with
+ * r_tr < 0 or r_tr >= rc is not switched.
  */
 int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
                        int n_atoms, int max_atoms_per_sys, const float* pos,
@@ -218,7 +224,7 @@ int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
                        const int32_t* lj_exclusion, const double* alpha,
                        const double* coul_cutoff, const int32_t* coul_exclusion,
                        const double* tril, int accumulate, float* forces, double* energy,
-                       double* virial);
+                       double* virial, const double* lj_truncation);
 
 /*
  * The same evaluation over a Verlet list of radius rc + skin kept in a

@@ -40,7 +40,7 @@ _SIGNATURES = {
     "kupsmi_normals": ([vp, u64, i64, i32, vp], C.c_int),
     "kupsmi_pair_forces": (
         [vp, vp, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp,
-         vp, vp],
+         vp, vp, vp],
         C.c_int,
     ),
     "kupsmi_ewald_recip": (

@@ -153,13 +153,30 @@ struct PairParams {
   int has_lj, has_coul, has_lj_excl, has_coul_excl, ns;
   double lj_rc2, lj_lo, lj_hi, cl_rc2, cl_lo, cl_hi, all_hi;
   float alpha, two_a_sqrtpi, alpha2;
+  int has_sw;                                 // LJ smooth switch from sw_rtr2 to the cutoff
+  float sw_rtr2, sw_rc2, sw_3rtr2, sw_inv_div;
   const float *sig2, *e4, *e24;  // LDS tables
 };
+
+// Smooth switch of pair_tail_corrected_lennard_jones_energy
+// (potential/classical/lennard_jones.py:200-222): for r_tr < r < rc the pair
+// energy u becomes u S with S = (rc^2 - r^2)^2 (2 r^2 + rc^2 - 3 r_tr^2) /
+// (rc^2 - r_tr^2)^3 (S = 1, S' = 0 at r_tr; S = S' = 0 at rc).  gl holds
+// 2 du/d(r^2) and becomes 2 d(uS)/d(r^2), with dS/d(r^2) =
+// 6 (rc^2 - r^2)(r_tr^2 - r^2) / (rc^2 - r_tr^2)^3.
+__device__ inline void lj_switch(const PairParams& pp, float r2, float& e, float& gl) {
+  const float x = pp.sw_rc2 - r2;
+  const float S = x * x * (2.0f * r2 + pp.sw_rc2 - pp.sw_3rtr2) * pp.sw_inv_div;
+  const float dS = 6.0f * x * (pp.sw_rtr2 - r2) * pp.sw_inv_div;
+  gl = gl * S + 2.0f * e * dS;
+  e *= S;
+}
 
 __device__ inline void pc_pair_params(PairParams& pp, int has_lj, int has_coul, int has_lj_excl,
                                       int has_coul_excl, const double* lj_rc, const double* cl_rc,
                                       const double* alpha, int s, int ns, const float* sig2,
-                                      const float* e4, const float* e24) {
+                                      const float* e4, const float* e24,
+                                      const double* lj_rtr = nullptr) {
   pp.has_lj = has_lj; pp.has_coul = has_coul;
   pp.has_lj_excl = has_lj_excl; pp.has_coul_excl = has_coul_excl; pp.ns = ns;
   pp.lj_rc2 = has_lj ? lj_rc[s] * lj_rc[s] : 0.0;
@@ -174,6 +191,13 @@ __device__ inline void pc_pair_params(PairParams& pp, int has_lj, int has_coul, 
   pp.two_a_sqrtpi = 2.0f * pp.alpha * 0.5641895835477563f;
   pp.alpha2 = pp.alpha * pp.alpha;
   pp.sig2 = sig2; pp.e4 = e4; pp.e24 = e24;
+  const double rtr = has_lj && lj_rtr ? lj_rtr[s] : -1.0;
+  pp.has_sw = rtr >= 0.0 && rtr * rtr < pp.lj_rc2;
+  pp.sw_rtr2 = (float)(rtr * rtr);
+  pp.sw_rc2 = (float)pp.lj_rc2;
+  pp.sw_3rtr2 = (float)(3.0 * rtr * rtr);
+  const double span = pp.lj_rc2 - rtr * rtr;
+  pp.sw_inv_div = pp.has_sw ? (float)(1.0 / (span * span * span)) : 0.0f;
 }
 
 // LJ and real-space Ewald terms of the directed pair (i -> j + s): adds the
@@ -197,8 +221,11 @@ __device__ inline float pc_pair_terms(const PairParams& pp, const SysGeom& g, fl
       const float ir2 = 1.0f / r2;
       float c6 = pp.sig2[p] * ir2;
       c6 = c6 * c6 * c6;
-      elj += pp.e4[p] * c6 * (c6 - 1.0f);
-      gsum += pp.e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;
+      float e = pp.e4[p] * c6 * (c6 - 1.0f);
+      float gl = pp.e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;
+      if (pp.has_sw && r2 > pp.sw_rtr2) lj_switch(pp, r2, e, gl);
+      elj += e;
+      gsum += gl;
     }
   }
   if (pp.has_coul && r2d < pp.cl_hi && !self) {

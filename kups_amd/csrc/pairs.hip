@@ -34,6 +34,7 @@ struct PairArgs {
   const double* sigma;
   const double* epsilon;
   const double* lj_rc;
+  const double* lj_rtr;  // per-system LJ switch start (NULL: no switch)
   const double* alpha;
   const double* cl_rc;
   const double* tril;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
   __syncthreads();
   PairParams pp;
   pc_pair_params(pp, a.has_lj, a.has_coul, a.has_lj_excl, a.has_coul_excl, a.lj_rc, a.cl_rc,
-                 a.alpha, s, ns, s_sig2, s_e4, s_e24);
+                 a.alpha, s, ns, s_sig2, s_e4, s_e24, a.lj_rtr);
   const double all_hi = pp.all_hi;
   const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
   const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
@@ -187,7 +188,8 @@ extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, cons
                                   const double* lj_cutoff, const int32_t* lj_exclusion,
                                   const double* alpha, const double* coul_cutoff,
                                   const int32_t* coul_exclusion, const double* tril,
-                                  int accumulate, float* forces, double* energy, double* virial) {
+                                  int accumulate, float* forces, double* energy, double* virial,
+                                  const double* lj_truncation) {
   KUPSMI_REQUIRE(ctx != nullptr, "null context");
   KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0, "negative size");
   KUPSMI_REQUIRE(max_atoms_per_sys <= PC_NMAX, "systems of the pair path hold at most 16384 atoms");
@@ -206,6 +208,7 @@ extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   if (a.tiles < 1) a.tiles = 1;
   a.sys_off = sys_off; a.pos = pos; a.q = charges; a.species = species;
   a.lj_excl = lj_exclusion; a.cl_excl = coul_exclusion; a.sigma = sigma; a.epsilon = epsilon;
+  a.lj_rtr = lj_truncation;
   a.lj_rc = lj_cutoff; a.alpha = alpha; a.cl_rc = coul_cutoff; a.tril = tril;
   a.forces = forces; a.energy = energy; a.virial = virial;
   // scratch layout (slot 3)

@@ -261,7 +261,7 @@ class RigidGCMC:
             _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), len(pos), int(sizes.max()),
             _lib.ptr(pb.positions), None, _lib.ptr(pb.species), self.n_species, _lib.ptr(sig),
             _lib.ptr(eps), _lib.ptr(rc), _lib.ptr(excl_t), None, None, None, _lib.ptr(sb.tril), 0,
-            _lib.ptr(f), _lib.ptr(e2), None))
+            _lib.ptr(f), _lib.ptr(e2), None, None))
         ng = self.d["n_guest"].cpu().numpy()
         counts = self.d["counts"].cpu().numpy().astype(float)
         e_tail = np.einsum("sab,sa,sb->s", self.tail, counts, counts)
@@ -422,7 +422,7 @@ class RigidGCMC:
             _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), total * M, maxn, _lib.ptr(p32),
             _lib.ptr(q), _lib.ptr(sp), self.n_species, _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(rc_lj),
             _lib.ptr(mol_id), _lib.ptr(alpha), _lib.ptr(rc_c), _lib.ptr(mol_id), _lib.ptr(tril), 0,
-            _lib.ptr(forces), _lib.ptr(e2), _lib.ptr(vir)))
+            _lib.ptr(forces), _lib.ptr(e2), _lib.ptr(vir), None))
         nm = self.nm
         elr = torch.zeros(n, dtype=torch.float64, device=dev)
         kmax = torch.as_tensor(self.kmax, dtype=torch.float64, device=dev)

@@ -233,7 +233,7 @@ class EwaldLJPropagator:
                 _lib.ptr(state.species), self.params.n_species, _lib.ptr(self._sig),
                 _lib.ptr(self._eps), _lib.ptr(state.cutoff), None, _lib.ptr(self._alpha),
                 _lib.ptr(self._rc), None, _lib.ptr(state.tril), 0, _lib.ptr(state.forces),
-                _lib.ptr(e2), _lib.ptr(vir)))
+                _lib.ptr(e2), _lib.ptr(vir), None))
         mx, my, mz = self.ewald.grid
         _lib.check(lib.kupsmi_ewald_recip(
             _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,

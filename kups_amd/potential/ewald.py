@@ -209,7 +209,7 @@ class EwaldPotential:
             _lib.ctx(), _lib.stream_ptr(), n_sys, _lib.ptr(sys_off), particles.n, maxn,
             _lib.ptr(pos), _lib.ptr(q), _lib.ptr(species), 1, None, None, None, None,
             _lib.ptr(alpha), _lib.ptr(rc), None, _lib.ptr(systems.tril), 0, _lib.ptr(forces),
-            _lib.ptr(e2), _lib.ptr(vir)))
+            _lib.ptr(e2), _lib.ptr(vir), None))
         mx, my, mz = self.params.grid
         nk = (mx + 1) * (2 * my + 1) * (2 * mz + 1)
         elr = torch.zeros(n_sys, dtype=torch.float64, device=dev)

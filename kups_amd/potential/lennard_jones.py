@@ -119,3 +119,45 @@ class LennardJonesPotential:
             self.params.n_species, _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(systems.tril),
             _lib.ptr(cutoffs), _lib.ptr(forces), _lib.ptr(energy), _lib.ptr(virial)))
         return PotentialOut(energy, -forces, virial)
+
+
+class PairTailCorrectedLennardJonesPotential(LennardJonesPotential):
+    """LJ with the smooth pairwise switch of
+    ``make_pair_tail_corrected_lennard_jones_potential``
+    (potential/classical/lennard_jones.py:182-222, 516-540): pair energies
+    between ``truncation_radius`` and the cutoff are multiplied by
+    S = (rc^2 - r^2)^2 (2 r^2 + rc^2 - 3 r_tr^2) / (rc^2 - r_tr^2)^3, so energy
+    and force go to zero smoothly at the cutoff.  Evaluated by the HBM pair
+    path (``kupsmi_pair_forces`` with ``lj_truncation``); systems up to 16384
+    atoms.  ``truncation_radius``: a scalar or one value per system."""
+
+    def __init__(self, params: LennardJonesParameters, truncation_radius):
+        super().__init__(params)
+        self.truncation_radius = truncation_radius
+
+    def __call__(self, particles: ParticleBatch, systems: SystemBatch, cutoffs=None) -> PotentialOut:
+        dev = particles.positions.device
+        n_sys = systems.n_sys
+        sys_off, sizes = system_offsets(particles.system, n_sys)
+        sig, eps = self.tables(dev)
+        if cutoffs is None:
+            cutoffs = self.params.cutoff
+        cut = torch.as_tensor(cutoffs, dtype=torch.float64, device=dev)
+        cut = (cut.expand(n_sys) if cut.dim() == 0 or cut.numel() == 1 else cut).contiguous()
+        rtr = torch.as_tensor(self.truncation_radius, dtype=torch.float64, device=dev)
+        rtr = (rtr.expand(n_sys) if rtr.dim() == 0 or rtr.numel() == 1 else rtr).contiguous()
+        species = particles.species
+        if species is None:
+            species = torch.zeros(particles.n, dtype=torch.int32, device=dev)
+        pos = particles.positions.to(torch.float32).contiguous()
+        species = species.to(torch.int32).contiguous()
+        forces = torch.empty((particles.n, 3), dtype=torch.float32, device=dev)
+        e2 = torch.empty((n_sys, 2), dtype=torch.float64, device=dev)
+        virial = torch.empty((n_sys, 3, 3), dtype=torch.float64, device=dev)
+        _lib.check(_lib.load().kupsmi_pair_forces(
+            _lib.ctx(), _lib.stream_ptr(), n_sys, _lib.ptr(sys_off), particles.n,
+            int(sizes.max()) if sizes.size else 0, _lib.ptr(pos), None, _lib.ptr(species),
+            self.params.n_species, _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(cut), None, None, None,
+            None, _lib.ptr(systems.tril), 0, _lib.ptr(forces), _lib.ptr(e2), _lib.ptr(virial),
+            _lib.ptr(rtr)))
+        return PotentialOut(e2[:, 0].clone(), -forces, virial)

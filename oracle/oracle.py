@@ -96,6 +96,56 @@ def lj_edges(pos, species, sys_off, tril, cutoff, sigma, eps, ij, shift):
     return e, f, w.reshape(nsys, 3, 3)
 
 
+def lj_switch_factor(r2, rc, rtr):
+    """S(r^2) and dS/d(r^2) of the pairwise switch (lennard_jones.py:210-220):
+    S = 1 for r <= r_tr, (rc^2 - r^2)^2 (2 r^2 + rc^2 - 3 r_tr^2) / (rc^2 - r_tr^2)^3
+    for r_tr < r < rc, 0 for r >= rc."""
+    r2 = np.asarray(r2, np.float64)
+    rc2, t2 = rc * rc, rtr * rtr
+    div = (rc2 - t2) ** 3
+    x = rc2 - r2
+    band = (r2 > t2) & (r2 < rc2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        S = np.where(band, x * x * (2 * r2 + rc2 - 3 * t2) / div, np.where(r2 >= rc2, 0.0, 1.0))
+        dS = np.where(band, 6 * x * (t2 - r2) / div, 0.0)
+    return S, dS
+
+
+def lj_switched_edges(pos, species, sys_off, tril, cutoff, rtr, sigma, eps, ij, shift):
+    """LJ with the smooth pairwise switch of pair_tail_corrected_lennard_jones_energy
+    (lennard_jones.py:200-222) over a directed edge list, numpy FP64: energy
+    1/2 u S per directed edge, forces and virial of the switched energy, with
+    lj_edges' conventions (d = r_j - r_i + shift @ L, virial -= d(uS)/d(r^2) d d)."""
+    pos, species, sys_off = _d(pos), _i(species), _i(sys_off)
+    tril, cutoff, rtr = _d(tril).reshape(-1, 6), _d(cutoff), _d(rtr)
+    sigma, eps, ij, shift = _d(sigma), _d(eps), _i(ij).reshape(-1, 2), _d(shift).reshape(-1, 3)
+    nsys = len(sys_off) - 1
+    sys_of = np.repeat(np.arange(nsys), np.diff(sys_off))
+    lat = np.zeros((nsys, 3, 3))
+    lat[:, 0, 0] = tril[:, 0]
+    lat[:, 1, :2] = tril[:, 1:3]
+    lat[:, 2, :] = tril[:, 3:6]
+    i, j = ij[:, 0], ij[:, 1]
+    s = sys_of[i]
+    d = pos[j] - pos[i] + np.einsum("ek,ekl->el", shift, lat[s])
+    r2 = (d * d).sum(1)
+    keep = r2 < cutoff[s] ** 2
+    sg, ep = sigma[species[i], species[j]], eps[species[i], species[j]]
+    c6 = (sg * sg / r2) ** 3
+    u = 4 * ep * (c6 * c6 - c6)
+    du = 12 * ep * c6 * (1 - 2 * c6) / r2
+    S, dS = lj_switch_factor(r2, cutoff[s], rtr[s])
+    us = np.where(keep, u * S, 0.0)
+    dus = np.where(keep, du * S + u * dS, 0.0)
+    e = np.zeros(nsys)
+    np.add.at(e, s, 0.5 * us)
+    f = np.zeros_like(pos)
+    np.add.at(f, i, 2 * dus[:, None] * d)
+    w = np.zeros((nsys, 3, 3))
+    np.add.at(w, s, -dus[:, None, None] * d[:, :, None] * d[:, None, :])
+    return e, f, w
+
+
 def lj_cells(pos, species, sys_off, tril, cutoff, sigma, eps):
     """LJ energy/forces/virial by cell list, same pair set (particle exclusion)."""
     pos, species, sys_off, tril, cutoff = _d(pos), _i(species), _i(sys_off), _d(tril), _d(cutoff)

@@ -52,7 +52,7 @@ class Case:
             _lib.ptr(self.pos), _lib.ptr(self.q), _lib.ptr(self.sp), self.lj.n_species,
             _lib.ptr(self.sig), _lib.ptr(self.eps), _lib.ptr(self.rc_lj), _lib.ptr(self.lj_excl),
             _lib.ptr(self.alpha), _lib.ptr(self.rc_cl), None, _lib.ptr(self.tril), 0, _lib.ptr(f),
-            _lib.ptr(e), _lib.ptr(w)))
+            _lib.ptr(e), _lib.ptr(w), None))
         return f.cpu().numpy(), e.cpu().numpy(), w.cpu().numpy()
 
     def verlet(self, vl: VerletList):

@@ -193,6 +193,68 @@ def test_lj_virial_is_strain_derivative():
     assert w[0, 0, 0] == pytest.approx(-(es[0] - es[1]) / (2 * h), rel=1e-5)
 
 
+def test_lj_switch_polynomial_properties():
+    # S = 1 and S' = 0 at r_tr, S = S' = 0 at rc, dS/d(r^2) = FD of S, S decreasing between
+    rc, rtr = 8.5, 6.0
+    S, dS = oracle.lj_switch_factor(np.array([rtr**2, rc**2]), rc, rtr)
+    assert S[0] == 1.0 and S[1] == 0.0
+    r2 = np.linspace(rtr**2 + 1e-3, rc**2 - 1e-3, 57)
+    S, dS = oracle.lj_switch_factor(r2, rc, rtr)
+    h = 1e-6
+    fd = (oracle.lj_switch_factor(r2 + h, rc, rtr)[0] - oracle.lj_switch_factor(r2 - h, rc, rtr)[0]) / (2 * h)
+    np.testing.assert_allclose(dS, fd, rtol=1e-6, atol=1e-9)
+    assert np.all(np.diff(S) < 0) and abs(dS[0]) < 1e-3 and abs(dS[-1]) < 1e-3
+    assert oracle.lj_switch_factor(rtr**2 + 1e-12, rc, rtr)[0] == pytest.approx(1.0, abs=1e-12)
+
+
+def test_lj_switched_reference_regimes():
+    # test_lennard_jones.py:316-387: particles A, B one apart, a single directed edge
+    sig = np.array([[1.0, 1.2], [1.2, 1.0]])
+    eps = np.array([[1.0, 0.8], [0.8, 1.0]])
+    pos = np.array([[0.0, 0, 0], [1.0, 0, 0]])
+    args = (pos, [0, 1], [0, 2], [10, 0, 10, 0, 0, 10])
+    ij, sh = np.array([[0, 1]]), np.zeros((1, 3))
+    target = oracle.lj_edges(*args, [12.0], sig, eps, ij, sh)[0][0]
+    def sw(rc, rtr):
+        return oracle.lj_switched_edges(*args, [rc], [rtr], sig, eps, ij, sh)[0][0]
+    assert sw(2.0, 1.5) == pytest.approx(target, rel=1e-15)   # r < r_tr
+    assert sw(1.2, 1.0) == pytest.approx(target, rel=1e-15)   # r = r_tr
+    assert 0.0 < sw(1.2, 0.8) < target                         # switched
+    assert sw(1.0, 0.8) == 0.0                                 # r = rc
+
+
+def test_lj_switched_forces_and_virial_are_derivatives():
+    b = workloads.batched_lj_boxes(sizes=[90, 300], seed=11)
+    sig, eps = np.array([[workloads.AR_SIGMA]]), np.array([[workloads.AR_EPS]])
+    cut = np.full(2, workloads.AR_CUTOFF)
+    rtr = np.array([0.7, 0.85]) * workloads.AR_CUTOFF
+    def run(pos, tril):
+        ij, sh = oracle.neighborlist(pos, np.arange(b.n), b.sys_off, tril, cut)
+        return oracle.lj_switched_edges(pos, b.species, b.sys_off, tril, cut, rtr, sig, eps, ij, sh)
+    e, f, w = run(b.positions, b.tril)
+    np.testing.assert_allclose(f[:90].sum(0), 0, atol=1e-10)
+    h = 1e-5
+    for atom in [3, 100, 250]:
+        for k in range(3):
+            p = b.positions.copy(); p[atom, k] += h
+            ep = run(p, b.tril)[0].sum()
+            p[atom, k] -= 2 * h
+            em = run(p, b.tril)[0].sum()
+            assert -(ep - em) / (2 * h) == pytest.approx(f[atom, k], rel=2e-5, abs=1e-8)
+    hs = 1e-6
+    es = []
+    for sgn in (1, -1):
+        m = np.diag([1 + sgn * hs, 1, 1])
+        es.append(run(b.positions @ m, b.tril * [1 + sgn * hs, 1, 1, 1, 1, 1])[0][1])
+    assert w[1, 0, 0] == pytest.approx(-(es[0] - es[1]) / (2 * hs), rel=1e-5)
+    # no switch when r_tr >= rc
+    ij, sh = oracle.neighborlist(b.positions, np.arange(b.n), b.sys_off, b.tril, cut)
+    e0 = oracle.lj_edges(b.positions, b.species, b.sys_off, b.tril, cut, sig, eps, ij, sh)[0]
+    e1 = oracle.lj_switched_edges(b.positions, b.species, b.sys_off, b.tril, cut, cut, sig, eps, ij, sh)[0]
+    np.testing.assert_allclose(e1, e0, rtol=1e-12)
+    assert np.all(e > e0)  # the switch removes attractive tail energy
+
+
 # ---- Morse ----
 
 def test_morse_closed_form():
