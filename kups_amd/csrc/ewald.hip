@@ -372,6 +372,9 @@ __global__ __launch_bounds__(EK_THREADS) void ek_energy_kernel(EkArgs a) {
   }
 }
 
+// MZ > 0: the k grid's z half width as a constant (inner loop fully
+// unrolled, (n2) factors folded); MZ = 0: read from the arguments.
+template <int MZ>
 __global__ __launch_bounds__(EK_THREADS) void ek_force_kernel(EkArgs a) {
   extern __shared__ float2 skp[];
   const int s = blockIdx.y;
@@ -386,12 +389,13 @@ __global__ __launch_bounds__(EK_THREADS) void ek_force_kernel(EkArgs a) {
   const double* t = a.tril + 6 * s;
   float2 ux, uy, uz;
   unit_phases(t, a.pos + 3 * (size_t)(i0 + i), ux, uy, uz);
-  const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
+  const int mz = MZ > 0 ? MZ : a.mz;
+  const int wy = 2 * a.my + 1, wz = 2 * mz + 1;
   // u^-m starting points
   float2 uy0 = make_float2(1.f, 0.f), uz0 = make_float2(1.f, 0.f);
   const float2 cy = make_float2(uy.x, -uy.y), cz = make_float2(uz.x, -uz.y);
   for (int m = 0; m < a.my; ++m) uy0 = cmul(uy0, cy);
-  for (int m = 0; m < a.mz; ++m) uz0 = cmul(uz0, cz);
+  for (int m = 0; m < mz; ++m) uz0 = cmul(uz0, cz);
   float t0 = 0.f, t1 = 0.f, t2 = 0.f;
   float2 ex = make_float2(1.f, 0.f);
   int k = 0;
@@ -401,14 +405,16 @@ __global__ __launch_bounds__(EK_THREADS) void ek_force_kernel(EkArgs a) {
       const float n1 = (float)(j1 - a.my);
       float2 e = cmul(exy, uz0);
       float a0 = 0.f, a2 = 0.f;
-      for (int j2 = 0; j2 < wz; ++j2, ++k) {
-        const float2 sp = skp[k];
+#pragma unroll
+      for (int j2 = 0; j2 < (MZ > 0 ? 2 * MZ + 1 : wz); ++j2) {
+        const float2 sp = skp[k + j2];
         // Im(S' conj(e))
         const float v = sp.y * e.x - sp.x * e.y;
         a0 += v;
-        a2 += (float)(j2 - a.mz) * v;
+        a2 += (float)(j2 - mz) * v;
         e = cmul(e, uz);
       }
+      k += wz;
       t0 += (float)n0 * a0;
       t1 += n1 * a0;
       t2 += a2;
@@ -475,7 +481,9 @@ extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   KUPSMI_CHECK_LAUNCH();
   if (forces && max_atoms_per_sys > 0) {
     dim3 g2((max_atoms_per_sys + EK_THREADS - 1) / EK_THREADS, n_sys);
-    ek_force_kernel<<<g2, EK_THREADS, sizeof(float2) * a.nk, st>>>(a);
+    const size_t lds = sizeof(float2) * a.nk;
+    if (a.mz == 8) ek_force_kernel<8><<<g2, EK_THREADS, lds, st>>>(a);
+    else ek_force_kernel<0><<<g2, EK_THREADS, lds, st>>>(a);
     KUPSMI_CHECK_LAUNCH();
   }
   return KUPSMI_OK;
