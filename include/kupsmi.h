@@ -72,6 +72,79 @@ int kupsmi_neighborlist(kupsmi_ctx* ctx, void* stream, int n_sys,
                         int32_t* edges_ij, int32_t* edges_shift, int64_t* count);
 
 /*
+ * The same directed pair set from a cell list.  Replaces CellListNeighborList
+ * (core/neighborlist.py:985-1077: _cell_hash / _cell_stencil / _num_cells,
+ * 258-292, and _cell_list_subselect, 295-364, feeding basic_neighborlist,
+ * 743-786): the atoms of each system are counting-sorted into cells at
+ * least rc wide (at most 16 per axis), and each atom tests the atoms of the
+ * distinct cells among its 27 neighbours (the reference's jnp.unique over
+ * the stencil hashes) with the minimum-image shift round(f_i - f_j) and the
+ * FP64 decision of kupsmi_neighborlist; systems narrower than 2 rc use the
+ * image grid over all their atoms.  The edge SET equals kupsmi_neighborlist's
+ * (the reference's contract: its order comes from jnp.unique/argsort);
+ * the order here is i-major, then stencil cell, then the cell's atoms in
+ * index order.  Systems hold at most 16384 atoms (max_atoms_per_sys).
+ * Capacity and *count as for kupsmi_neighborlist.
+ */
+int kupsmi_neighborlist_cells(kupsmi_ctx* ctx, void* stream, int n_sys,
+                              const int32_t* sys_off, int n_atoms, int max_atoms_per_sys,
+                              const float* pos, const int32_t* exclusion, const double* tril,
+                              const double* cutoff, int64_t capacity, int32_t* edges_ij,
+                              int32_t* edges_shift, int64_t* count);
+
+/*
+ * Edges between the full particle table (lh: pos, exclusion, systems by
+ * sys_off) and k further particles (rh: rh_pos [k x 3], rh_system [k],
+ * rh_exclusion [k]).  Replaces NearestNeighborList.__call__(lh, rh, systems,
+ * cutoffs, rh_index_remap) (core/neighborlist.py:233-255), the form that
+ * RadiusGraphConstructor's patch queries (potential/common/graph.py:252-267)
+ * and neighborlist_changes (1467) make, with the rules of
+ * _build_segmentation_mask (505-539) and _compact_edges (651-689):
+ *   - a pair (i, r) needs i and r in the same system (rh_system outside
+ *     [0, n_sys) drops r); shift round(f_i - f_r), or every offset of the
+ *     image grid of a system narrower than 2 rc; kept when
+ *     |(f_i - f_r - s) @ L|^2 < rc^2 (FP64, strict) and not (equal exclusion
+ *     ids on the minimum image);
+ *   - rh_index_remap (optional): rh particle r is lh particle remap[r]; an lh
+ *     particle named by any remap entry pairs only with entries whose remap
+ *     is <= its index (536-537), and every edge is output in both directions:
+ *     all edges (i, remap[r], s) first, then (remap[r], i, -s) in the same
+ *     order (680-685).  A remap outside [0, n_atoms) names no lh particle
+ *     and is written as n_atoms (the reference's out-of-bounds index).
+ *     Without a remap the edges are (i, r, s), r indexing the rh particles.
+ * Forward edges are ordered by r, then i, then image offset.  Every rh
+ * particle scans its system densely (O(k x system size)): the form suits
+ * the small patches of MC moves.  Capacity and *count as for
+ * kupsmi_neighborlist (the count includes both directions).
+ */
+int kupsmi_neighborlist_query(kupsmi_ctx* ctx, void* stream, int n_sys,
+                              const int32_t* sys_off, int n_atoms, const float* pos,
+                              const int32_t* exclusion, int k, const float* rh_pos,
+                              const int32_t* rh_system, const int32_t* rh_exclusion,
+                              const int32_t* rh_index_remap, const double* tril,
+                              const double* cutoff, int64_t capacity, int32_t* edges_ij,
+                              int32_t* edges_shift, int64_t* count);
+
+/*
+ * Parity output of the force kernels (no counterpart in the reference: it
+ * exposes the pair set a force evaluation used, for comparison with the
+ * neighbor list above).  Arms the next pair-evaluating call made with the
+ * context (kupsmi_md_lj, kupsmi_lj_forces, kupsmi_pair_forces) or with the
+ * Verlet handle (kupsmi_pair_forces_verlet) to write every directed pair its
+ * force loops kept: for kupsmi_md_lj the pairs of the last step's final
+ * force pass, at the positions the call writes back; for the others the
+ * pairs at the caller's positions.  ij[e] = (i, j) global particle indices,
+ * shift[e] = s with d = r_j - r_i + s @ L in the caller's frame, terms[e]
+ * (optional) = 1 LJ | 2 real-space Coulomb; entries in no particular order;
+ * *count (device uint64) receives the number of pairs (entries past capacity
+ * are not written).  The armed call runs the recording instantiation of its
+ * kernels: the same source and pair decisions plus the writes.  One call
+ * consumes the arming.
+ */
+int kupsmi_ctx_record_pairs(kupsmi_ctx* ctx, int64_t capacity, int32_t* ij, int32_t* shift,
+                            uint8_t* terms, unsigned long long* count);
+
+/*
  * Edges that moving k particles adds and removes.  Replaces
  * neighborlist_changes (core/neighborlist.py:1424-1515) for the full list
  * above: moved[k] are distinct particle indices, moved_pos[k x 3] their new
@@ -124,6 +197,9 @@ int kupsmi_morse_bonds(void* stream, int n_sys, const int32_t* sys_off, int n_at
  * accumulated in FP64.  sigma/eps: (n_species x n_species) mixed tables in
  * FP64 (Lorentz-Berthelot, lennard_jones.py:125-141).
  * Exclusion is by particle (i != j on the minimum image), as in md_lj.
+ * Positions are wrapped into the cell by whole lattice vectors (in FP64,
+ * rounded once to FP32) before the pairs are decided; a position already
+ * inside the cell is used bit for bit.
  * Systems must hold <= KUPSMI_LDS_MAX_ATOMS atoms.
  * Outputs: forces (n_atoms x 3, f32), energy[n_sys], virial[n_sys x 9] (f64).
  */
@@ -250,6 +326,10 @@ int kupsmi_verlet_invalidate(kupsmi_verlet* v, void* stream);
 /* out[0]: system rebuilds, out[1]: atom lists that overflowed (host array,
  * synchronises the stream). */
 int kupsmi_verlet_counters(kupsmi_verlet* v, void* stream, unsigned long long* out);
+/* Arms the handle's next kupsmi_pair_forces_verlet to record its pair set,
+ * as kupsmi_ctx_record_pairs does for the context's calls. */
+int kupsmi_verlet_record_pairs(kupsmi_verlet* v, int64_t capacity, int32_t* ij, int32_t* shift,
+                               uint8_t* terms, unsigned long long* count);
 int kupsmi_pair_forces_verlet(kupsmi_verlet* v, void* stream, int n_sys, const int32_t* sys_off,
                               int n_atoms, int max_atoms_per_sys, const float* pos,
                               const float* charges, const int32_t* species, int n_species,

@@ -27,6 +27,12 @@ _SIGNATURES = {
     "kupsmi_ctx_create": ([C.POINTER(vp)], C.c_int),
     "kupsmi_ctx_destroy": ([vp], C.c_int),
     "kupsmi_neighborlist": ([vp, vp, i32, vp, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
+    "kupsmi_neighborlist_cells": (
+        [vp, vp, i32, vp, i32, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
+    "kupsmi_neighborlist_query": (
+        [vp, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
+    "kupsmi_ctx_record_pairs": ([vp, i64, vp, vp, vp, vp], C.c_int),
+    "kupsmi_verlet_record_pairs": ([vp, i64, vp, vp, vp, vp], C.c_int),
     "kupsmi_neighborlist_changes": (
         [vp, vp, i32, vp, i32, vp, vp, vp, vp, i32, vp, vp, i64, vp, vp, vp, i64, vp, vp, vp],
         C.c_int,

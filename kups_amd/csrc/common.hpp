@@ -33,11 +33,23 @@ struct Scratch {
   void* ptr = nullptr;
   size_t bytes = 0;
 };
+
+// Parity output of the force kernels (see record_pair below).
+struct PairRecord {
+  int64_t cap = 0;
+  int32_t* ij = nullptr;
+  int32_t* shift = nullptr;
+  uint8_t* terms = nullptr;
+  unsigned long long* count = nullptr;
+};
 }  // namespace kupsmi
 
 struct kupsmi_ctx {
   int device = 0;
-  kupsmi::Scratch slots[8];
+  kupsmi::Scratch slots[16];
+  // armed by kupsmi_ctx_record_pairs, taken by the next pair-evaluating call
+  kupsmi::PairRecord rec;
+  int rec_armed = 0;
 };
 
 namespace kupsmi {
@@ -62,6 +74,8 @@ struct SysGeom {
 };
 
 __host__ __device__ inline void tril_inverse(const double* t, double* li) {
+  // unfused, so the inverse has the same bits as a plain C evaluation
+#pragma clang fp contract(off)
   const double a = t[0], b = t[1], c = t[2], d = t[3], e = t[4], f = t[5];
   li[0] = 1.0 / a;
   li[1] = -b / (a * c);
@@ -118,7 +132,9 @@ __host__ __device__ inline void make_geom(const double* tril, double rc, int max
       g.nc[a] = 1;
       g.h[a] = (img[a] - 1) / 2;
     } else {
-      int n = isfinite(rc) ? (int)(sub * perp[a] / rc) : 1;
+      // cells at least rc (1 + 1e-6) / sub wide: an FP32 cell assignment off
+      // by one at a cell face then still finds every pair within rc
+      int n = isfinite(rc) ? (int)(sub * perp[a] / (rc * (1.0 + 1e-6))) : 1;
       if (n < 1) n = 1;
       if (n > max_nc_axis) n = max_nc_axis;
       g.nc[a] = n;
@@ -154,6 +170,78 @@ __device__ inline void wrap_shift(const double* L, const float* Lif, float& x, f
     z = (float)((double)z - s2);
   }
 }
+
+// ------------------------------------------- FP64 pair decisions ----
+// The reference's pair test (core/neighborlist.py:542-564, 589): fractional
+// coordinates f = r @ L^-1, d = (f_i - f_j - s) @ L, kept when |d|^2 < rc^2
+// (strict).  Evaluated in FP64 without multiply-add fusion, operation by
+// operation in the order of a plain C evaluation, so every kernel that
+// decides a pair here decides it exactly as a CPU restatement does.
+__device__ inline void frac64(const double* li, double x, double y, double z, double& f0,
+                              double& f1, double& f2) {
+#pragma clang fp contract(off)
+  f0 = x * li[0] + y * li[1] + z * li[3];
+  f1 = y * li[2] + z * li[4];
+  f2 = z * li[5];
+}
+
+__device__ inline double dist2_64(const double* L, double d0, double d1, double d2) {
+#pragma clang fp contract(off)
+  const double x = d0 * L[0] + d1 * L[1] + d2 * L[3];
+  const double y = d1 * L[2] + d2 * L[4];
+  const double z = d2 * L[5];
+  return x * x + y * y + z * z;
+}
+
+// |(f_i - f_j - s) @ L|^2 for positions pi, pj (FP32 as stored, widened).
+__device__ inline double pair_dist2_64(const SysGeom& g, const float* pi, const float* pj, int s0,
+                                       int s1, int s2) {
+#pragma clang fp contract(off)
+  double fi0, fi1, fi2, fj0, fj1, fj2;
+  frac64(g.Li, pi[0], pi[1], pi[2], fi0, fi1, fi2);
+  frac64(g.Li, pj[0], pj[1], pj[2], fj0, fj1, fj2);
+  const double d0 = fi0 - fj0, d1 = fi1 - fj1, d2 = fi2 - fj2;
+  return dist2_64(g.L, d0 - s0, d1 - s1, d2 - s2);
+}
+
+// Minimum image of the pair: s == rint(f_i - f_j) per axis (jnp.round rounds
+// half to even, as rint does in the default rounding mode).
+__device__ inline bool pair_is_min_image64(const SysGeom& g, const float* pi, const float* pj,
+                                           int s0, int s1, int s2) {
+#pragma clang fp contract(off)
+  double fi0, fi1, fi2, fj0, fj1, fj2;
+  frac64(g.Li, pi[0], pi[1], pi[2], fi0, fi1, fi2);
+  frac64(g.Li, pj[0], pj[1], pj[2], fj0, fj1, fj2);
+  return rint(fi0 - fj0) == s0 && rint(fi1 - fj1) == s1 && rint(fi2 - fj2) == s2;
+}
+
+// Whole lattice vectors n with caller = copy + n @ L, for a working copy of a
+// caller's position that was moved by lattice vectors (and rounded once).
+__device__ inline void lattice_offset(const SysGeom& g, const float* caller, float cx, float cy,
+                                      float cz, int& n0, int& n1, int& n2) {
+  double e0, e1, e2;
+  frac64(g.Li, (double)caller[0] - cx, (double)caller[1] - cy, (double)caller[2] - cz, e0, e1,
+         e2);
+  n0 = (int)rint(e0); n1 = (int)rint(e1); n2 = (int)rint(e2);
+}
+
+// -------------------------------------------------- pair-set records ----
+// Parity output of the force kernels: the directed pairs (i, j, s) a kernel
+// evaluated, in global particle indices and the caller's frame, with a bit
+// mask of the terms that kept the pair (bit 0 LJ, bit 1 real-space
+// Coulomb).  Appended in no particular order; entries past cap are counted,
+// not written.
+__device__ inline void record_pair(const PairRecord& r, int gi, int gj, int s0, int s1, int s2,
+                                   int terms) {
+  const unsigned long long e = atomicAdd(r.count, 1ull);
+  if (e < (unsigned long long)r.cap) {
+    r.ij[2 * e] = gi; r.ij[2 * e + 1] = gj;
+    r.shift[3 * e] = s0; r.shift[3 * e + 1] = s1; r.shift[3 * e + 2] = s2;
+    if (r.terms) r.terms[e] = (uint8_t)terms;
+  }
+}
+
+static __global__ void record_clear_kernel(unsigned long long* count) { *count = 0ull; }
 
 __device__ inline int cell_index(float fw, int n) {
   int b = (int)((fw + 0.5f) * (float)n);

@@ -10,7 +10,7 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 int scratch(kupsmi_ctx* ctx, int slot, size_t bytes, void** out) {
   KUPSMI_REQUIRE(ctx != nullptr, "null context");
-  KUPSMI_REQUIRE(slot >= 0 && slot < 8, "bad scratch slot");
+  KUPSMI_REQUIRE(slot >= 0 && slot < 16, "bad scratch slot");
   Scratch& s = ctx->slots[slot];
   if (s.bytes < bytes) {
     // Grow geometrically so a slowly growing workload reallocates rarely.
@@ -40,6 +40,16 @@ extern "C" int kupsmi_ctx_create(kupsmi_ctx** out) {
   KUPSMI_REQUIRE(c != nullptr, "out of host memory");
   KUPSMI_HIP(hipGetDevice(&c->device));
   *out = c;
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_ctx_record_pairs(kupsmi_ctx* ctx, int64_t capacity, int32_t* ij,
+                                      int32_t* shift, uint8_t* terms, unsigned long long* count) {
+  KUPSMI_REQUIRE(ctx != nullptr, "null context");
+  KUPSMI_REQUIRE(capacity >= 0 && count != nullptr, "count pointer and capacity >= 0 required");
+  KUPSMI_REQUIRE(capacity == 0 || (ij && shift), "ij and shift buffers required");
+  ctx->rec = PairRecord{capacity, ij, shift, terms, count};
+  ctx->rec_armed = 1;
   return KUPSMI_OK;
 }
 

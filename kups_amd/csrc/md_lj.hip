@@ -64,6 +64,8 @@ struct MdArgs {
   double* pot;
   double* kin;
   double* virial;
+  // parity output: the pair set of the last step's final force pass
+  PairRecord rec;
 };
 
 // Largest-first order of the systems (counting sort of the sizes, clamped to
@@ -178,11 +180,8 @@ __device__ inline bool band_keep(const SysGeom& g, float4 ri, float4 rj, int sx,
                                  int sz) {
   // keep the FP64 cell matrices out of the loop's live registers: reload them
   asm volatile("" ::: "memory");
-  double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
-  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
-  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
-  tmul(g.L, fi0 - fj0 - sx, fi1 - fj1 - sy, fi2 - fj2 - sz, d0, d1, d2);
-  return d0 * d0 + d1 * d1 + d2 * d2 < g.rc2;
+  const float pi[3] = {ri.x, ri.y, ri.z}, pj[3] = {rj.x, rj.y, rj.z};
+  return pair_dist2_64(g, pi, pj, sx, sy, sz) < g.rc2;
 }
 
 // floor(x / n) for a small integer x and 1 <= n <= MD_MAX_NC_AXIS: the
@@ -218,14 +217,51 @@ __device__ inline void md_lj_terms(float ex, float ey, float ez, float r2, bool 
 // rare call claims no registers in the pair loops.
 // Positions are passed as x, y, z only, so the pair loop's loads of the
 // neighbour need not fetch the w word.
+// The FP64 nearest image s = rint(f_i - f_j) of a pair.
+__device__ inline void md_nearest64(const SysGeom& g, float xi, float yi, float zi, float xj,
+                                    float yj, float zj, int& s0, int& s1, int& s2) {
+#pragma clang fp contract(off)
+  double fi0, fi1, fi2, fj0, fj1, fj2;
+  frac64(g.Li, xi, yi, zi, fi0, fi1, fi2);
+  frac64(g.Li, xj, yj, zj, fj0, fj1, fj2);
+  s0 = (int)rint(fi0 - fj0); s1 = (int)rint(fi1 - fj1); s2 = (int)rint(fi2 - fj2);
+}
+
 __device__ __attribute__((noinline)) bool md_band_nearest(const SysGeom& g, float xi, float yi,
                                                           float zi, float xj, float yj, float zj) {
-  double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
-  tmul(g.Li, (double)xi, (double)yi, (double)zi, fi0, fi1, fi2);
-  tmul(g.Li, (double)xj, (double)yj, (double)zj, fj0, fj1, fj2);
+#pragma clang fp contract(off)
+  double fi0, fi1, fi2, fj0, fj1, fj2;
+  frac64(g.Li, xi, yi, zi, fi0, fi1, fi2);
+  frac64(g.Li, xj, yj, zj, fj0, fj1, fj2);
   const double t0 = fi0 - fj0, t1 = fi1 - fj1, t2 = fi2 - fj2;
-  tmul(g.L, t0 - rint(t0), t1 - rint(t1), t2 - rint(t2), d0, d1, d2);
-  return d0 * d0 + d1 * d1 + d2 * d2 < g.rc2;
+  return dist2_64(g.L, t0 - rint(t0), t1 - rint(t1), t2 - rint(t2)) < g.rc2;
+}
+
+// Where a record goes: the pair set of a force walk in global indices
+// (local ids in the w word of the momenta buffer).
+// caller_pos (a force evaluation without steps, whose positions are not
+// written back): shifts are converted to the frame of the caller's positions,
+// of which the working copies may differ by the lattice vectors of the wrap.
+struct MdRec {
+  const PairRecord* rec;
+  const float4* pb;
+  const float4* sp;
+  const SysGeom* g;
+  const float* caller_pos;
+  int i0;
+};
+
+__device__ __attribute__((noinline)) void md_record(const MdRec& r, int slot, int t, int s0, int s1,
+                                                    int s2) {
+  const int gi = r.i0 + __float_as_int(r.pb[slot].w), gj = r.i0 + __float_as_int(r.pb[t].w);
+  if (r.caller_pos) {
+    int a0, a1, a2, b0, b1, b2;
+    const float4 ri = r.sp[slot], rj = r.sp[t];
+    lattice_offset(*r.g, r.caller_pos + 3 * (size_t)gi, ri.x, ri.y, ri.z, a0, a1, a2);
+    lattice_offset(*r.g, r.caller_pos + 3 * (size_t)gj, rj.x, rj.y, rj.z, b0, b1, b2);
+    s0 += a0 - b0; s1 += a1 - b1; s2 += a2 - b2;
+  }
+  record_pair(*r.rec, gi, gj, s0, s1, s2, 1);
 }
 
 __device__ __attribute__((noinline)) bool md_band_shift(const SysGeom& g, float4 ri, float4 rj,
@@ -237,11 +273,11 @@ __device__ __attribute__((noinline)) bool md_band_shift(const SysGeom& g, float4
 // stencil.  Atoms are sorted by cell index (z ny + y) nx + x, so for a
 // stencil row (dz, dy) the cells cx-h .. cx+h of one image shift sx are one
 // contiguous run of sp[]: the row is walked as one run per image.
-template <bool MULTI, bool VIR>
+template <bool MULTI, bool VIR, bool REC = false>
 __device__ inline void md_stencil_forces(const SysGeom& g, const float4* sp, const int* cstart,
                                          const float4* s_par, int ns, int slot, float rc2_lo,
                                          float rc2_hi, float& ax, float& ay, float& az, float& ei,
-                                         float (&w)[6]) {
+                                         float (&w)[6], const MdRec* rec = nullptr) {
   const float4 ri = sp[slot];
   const int wi = __float_as_int(ri.w);
   const int c = (wi >> 8) & 0xfff;
@@ -275,8 +311,12 @@ __device__ inline void md_stencil_forces(const SysGeom& g, const float4* sp, con
           const bool lo = r2 < rc2_lo;
           bool keep = lo && t != self;
           if (__builtin_expect(!lo && r2 < rc2_hi, 0)) keep = md_band_shift(g, ri, rj, sx, sy, sz);
-          const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj.w) & 0xff)] : p0;
-          md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
+          if constexpr (REC) {
+            if (keep) md_record(*rec, slot, t, sx, sy, sz);
+          } else {
+            const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj.w) & 0xff)] : p0;
+            md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
+          }
         }
       }
     }
@@ -369,11 +409,11 @@ __device__ inline int md_build_list(const SysGeom& g, const float4* sp, const in
 // on its per-axis nearest image (the only image within the cutoff when every
 // perpendicular width is at least 2 rc).  Four entries per trip, their list
 // words and positions loaded before any is used.
-template <bool MULTI, bool VIR, bool ORTHO>
+template <bool MULTI, bool VIR, bool ORTHO, bool REC = false>
 __device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const unsigned short* nbl,
                                       int cnt, int slot, const float4* s_par, int ns,
                                       float rc2_lo, float rc2_hi, float& ax, float& ay, float& az,
-                                      float& ei, float (&w)[6]) {
+                                      float& ei, float (&w)[6], const MdRec* rec = nullptr) {
   const float4 ri = sp[slot];
   const int si = __float_as_int(ri.w) & 0xff;
   const float4 p0 = s_par[0];
@@ -392,25 +432,38 @@ __device__ inline void md_list_forces(const SysGeom& g, const float4* sp, const 
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       float ex = rj[u].x - ri.x, ey = rj[u].y - ri.y, ez = rj[u].z - ri.z;
+      // nearest image n (fractional units); the edge shift is -n
+      float n0, n1, n2;
       if (ORTHO) {
-        ex -= rintf(ex * g.Lif[0]) * g.Lf[0];
-        ey -= rintf(ey * g.Lif[2]) * g.Lf[2];
-        ez -= rintf(ez * g.Lif[5]) * g.Lf[5];
+        n0 = rintf(ex * g.Lif[0]); n1 = rintf(ey * g.Lif[2]); n2 = rintf(ez * g.Lif[5]);
+        ex -= n0 * g.Lf[0];
+        ey -= n1 * g.Lf[2];
+        ez -= n2 * g.Lf[5];
       } else {
         float f0, f1, f2;
         tmul(g.Lif, ex, ey, ez, f0, f1, f2);
+        n0 = rintf(f0); n1 = rintf(f1); n2 = rintf(f2);
         float sx, sy, sz;
-        tmul(g.Lf, rintf(f0), rintf(f1), rintf(f2), sx, sy, sz);
+        tmul(g.Lf, n0, n1, n2, sx, sy, sz);
         ex -= sx; ey -= sy; ez -= sz;
       }
       const float r2 = ex * ex + ey * ey + ez * ez;
       const bool valid = 4 * q + u < cnt;
       const bool lo = r2 < rc2_lo;
       bool keep = valid && lo;
-      if (__builtin_expect(valid && !lo && r2 < rc2_hi, 0))
+      const bool band = valid && !lo && r2 < rc2_hi;
+      if (__builtin_expect(band, 0))
         keep = md_band_nearest(g, ri.x, ri.y, ri.z, rj[u].x, rj[u].y, rj[u].z);
-      const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj[u].w) & 0xff)] : p0;
-      md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
+      if constexpr (REC) {
+        if (keep) {
+          int s0 = -(int)n0, s1 = -(int)n1, s2 = -(int)n2;
+          if (band) md_nearest64(g, ri.x, ri.y, ri.z, rj[u].x, rj[u].y, rj[u].z, s0, s1, s2);
+          md_record(*rec, slot, j[u], s0, s1, s2);
+        }
+      } else {
+        const float4 pp = MULTI ? s_par[si * ns + (__float_as_int(rj[u].w) & 0xff)] : p0;
+        md_lj_terms<VIR>(ex, ey, ez, r2, keep, pp, ax, ay, az, ei, w);
+      }
     }
     cur = nxt;
   }
@@ -456,6 +509,30 @@ __device__ inline void md_force_phase(const MdArgs& a, const SysGeom& g, const f
   }
 }
 
+// The pair set of the current state, walked exactly as the force phase walks
+// it (same lists or stencil, same decisions), appended to a.rec.
+template <bool ORTHO>
+__device__ inline void md_record_phase(const MdArgs& a, const SysGeom& g, const float4* sp,
+                                       const float4* pb, const int* cstart, const short* nbcount,
+                                       const unsigned short* nbl, bool use_list, const float4* s_par,
+                                       int ns, int n, int i0, float rc2_lo, float rc2_hi) {
+  const int tid = threadIdx.x;
+  const MdRec rec{&a.rec, pb, sp, &g, a.n_steps == 0 ? a.pos : nullptr, i0};
+  float ax = 0.f, ay = 0.f, az = 0.f, ei = 0.f;
+  float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int k = 0; k < MD_SLOTS; ++k) {
+    const int slot = tid + k * MD_THREADS;
+    if (slot >= n) continue;
+    if (use_list)
+      md_list_forces<false, false, ORTHO, true>(g, sp, nbl, nbcount[slot], slot, s_par, ns, rc2_lo,
+                                                rc2_hi, ax, ay, az, ei, w, &rec);
+    else
+      md_stencil_forces<false, false, true>(g, sp, cstart, s_par, ns, slot, rc2_lo, rc2_hi, ax, ay,
+                                            az, ei, w, &rec);
+  }
+}
+
 // Maximum of v over the workgroup, returned in every thread.
 __device__ inline float block_max_all(float v, double* red) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -489,7 +566,9 @@ __device__ inline void permute_slots(float4* buf, const int (&dst)[MD_SLOTS], in
 // THERMO: 0 for velocity Verlet / BAOAB, 1 for CSVR, 2 for CSVR + cell
 // rescaling; each is its own instance, so the plain integrators carry none of
 // the thermostat's and barostat's state.
-template <int THERMO>
+// REC: the instantiation that also writes the pair set of the last step's
+// final force pass to a.rec (parity output); the MD itself is the same code.
+template <int THERMO, bool REC>
 __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
   __shared__ float4 sp[MD_NMAX];   // sorted x,y,z ; w = bits(species | cell << 8)
   __shared__ float4 pb[MD_NMAX];   // sorted px,py,pz ; w = bits(local id)
@@ -816,6 +895,15 @@ __global__ __launch_bounds__(MD_THREADS) void md_lj_kernel(MdArgs a) {
         double e = block_sum((double)epot, red);
         if (tid == 0) a.pot[(size_t)s * total + st] = e;
       }
+      if (REC && last) {
+        __syncthreads();
+        if (ortho)
+          md_record_phase<true>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
+                                rc2_lo, rc2_hi);
+        else
+          md_record_phase<false>(a, g, sp, pb, cstart, keys, nbl, use_list, s_par, ns, n, i0,
+                                 rc2_lo, rc2_hi);
+      }
     }
     if (npt && tid == 0)
       for (int c = 0; c < 6; ++c) a.tril_out[6 * (size_t)s + c] = s_tril[c];
@@ -860,6 +948,13 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
   // the barostat rebuilds the cell list every step: no Verlet lists
   if (a.integrator == 3) a.skin = 0.0f;
   KUPSMI_REQUIRE(a.n_steps >= 0, "n_steps < 0");
+  // a pending pair record is taken by this call
+  if (ctx->rec_armed) {
+    a.rec = ctx->rec;
+    ctx->rec_armed = 0;
+    record_clear_kernel<<<1, 1, 0, stream>>>(a.rec.count);
+    KUPSMI_CHECK_LAUNCH();
+  }
   if (a.n_sys == 0) return KUPSMI_OK;
   void* buf = nullptr;
   const size_t geom_bytes = sizeof(SysGeom) * (size_t)a.n_sys;
@@ -889,12 +984,17 @@ static int md_launch(kupsmi_ctx* ctx, hipStream_t stream, MdArgs& a) {
     if (rc) return rc;
     a.nblist = static_cast<unsigned short*>(lb);
   }
-  if (a.integrator == 3)
-    md_lj_kernel<2><<<grid, MD_THREADS, 0, stream>>>(a);
-  else if (a.integrator == 2)
-    md_lj_kernel<1><<<grid, MD_THREADS, 0, stream>>>(a);
-  else
-    md_lj_kernel<0><<<grid, MD_THREADS, 0, stream>>>(a);
+  const bool rec = a.rec.count != nullptr;
+  if (a.integrator == 3) {
+    if (rec) md_lj_kernel<2, true><<<grid, MD_THREADS, 0, stream>>>(a);
+    else md_lj_kernel<2, false><<<grid, MD_THREADS, 0, stream>>>(a);
+  } else if (a.integrator == 2) {
+    if (rec) md_lj_kernel<1, true><<<grid, MD_THREADS, 0, stream>>>(a);
+    else md_lj_kernel<1, false><<<grid, MD_THREADS, 0, stream>>>(a);
+  } else {
+    if (rec) md_lj_kernel<0, true><<<grid, MD_THREADS, 0, stream>>>(a);
+    else md_lj_kernel<0, false><<<grid, MD_THREADS, 0, stream>>>(a);
+  }
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }

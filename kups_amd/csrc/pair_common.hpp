@@ -13,7 +13,11 @@ constexpr int PC_MAX_NC_AXIS = 16;
 constexpr int PC_NCMAX = PC_MAX_NC_AXIS * PC_MAX_NC_AXIS * PC_MAX_NC_AXIS;
 constexpr int PC_TILE = 256;
 constexpr int PC_NSPEC = 8;
-constexpr double PC_BAND64 = 1e-12;
+// relative half width of the band around each cutoff whose pairs are
+// re-decided on the caller's positions: it covers the rounding of a wrapped
+// copy (half an FP32 ulp per coordinate, ~6e-7 of rc^2 at rc = 12 A in a
+// 30 A cell) with a wide margin
+constexpr double PC_BAND64 = 4e-6;
 constexpr int PC_NPART = 8;  // e_lj, e_real, w00 w01 w02 w11 w12 w22
 
 // Exclusive scan of cnt[0..n) into start[0..n], n <= 4 * blockDim.
@@ -128,25 +132,101 @@ __device__ inline void pc_bin_system(const SysGeom& g, int i0, int n, const floa
   for (int c = tid; c <= ncell; c += PC_BIN_THREADS) o.cs[c] = L.start[c];
 }
 
-// FP64 re-decision |(f_i - f_j - s) @ L|^2 < rc2 (core/neighborlist.py:542-564).
-__device__ inline bool keep64(const SysGeom& g, float4 ri, float4 rj, int sx, int sy, int sz,
-                              double rc2) {
+// Band decisions on the caller's positions.  The pair kernels work on
+// copies of the caller's positions that may have been moved by whole lattice
+// vectors (wrapped, or kept next to a list's build positions) and rounded
+// once; a pair within the band of a cutoff is re-decided with the
+// reference's FP64 formula |(f_i - f_j - s) @ L|^2 < rc2
+// (core/neighborlist.py:542-564, 589) on the caller's own positions, where
+// r_j - r_i + s @ L on the copies is r_j - r_i + (s - n_j + n_i) @ L with
+// caller = copy + n @ L.
+//
+// The out-of-line helpers take only the partner's sorted index and the
+// shift: the copies, the caller's positions and the sorted -> caller map of
+// the workgroup's system are in band_ctx (set by the kernel), and the left
+// atom is sorted atom blockIdx.x * PC_TILE + threadIdx.x, so the rare call
+// costs the pair loops no registers.
+struct BandCtx {
+  const float4* spos;  // copies of the system (sorted)
+  const float* cpos;   // caller's positions of the system
+  const int* csidx;    // sorted -> caller index within the system
+};
+static __shared__ BandCtx band_ctx;
+
+// The caller's positions of the pair and its caller-frame shift.  Written
+// as a sequence of short phases (compiler barriers between them), so the
+// helpers hold few values live at once and claim few registers.
+__device__ inline void band_pair(const SysGeom& g, int tj, int sx, int sy, int sz,
+                                 const float*& pi, const float*& pj, int& c0, int& c1, int& c2) {
+  const BandCtx& c = band_ctx;
+  const int ti = blockIdx.x * PC_TILE + threadIdx.x;
+  pi = c.cpos + 3 * (size_t)c.csidx[ti];
+  pj = c.cpos + 3 * (size_t)c.csidx[tj];
+  int a0, a1, a2, b0, b1, b2;
+  {
+    const float4 r = c.spos[ti];
+    lattice_offset(g, pi, r.x, r.y, r.z, a0, a1, a2);
+  }
   asm volatile("" ::: "memory");
-  double fi0, fi1, fi2, fj0, fj1, fj2, d0, d1, d2;
-  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
-  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
-  tmul(g.L, fi0 - fj0 - sx, fi1 - fj1 - sy, fi2 - fj2 - sz, d0, d1, d2);
-  return d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+  {
+    const float4 r = c.spos[tj];
+    lattice_offset(g, pj, r.x, r.y, r.z, b0, b1, b2);
+  }
+  asm volatile("" ::: "memory");
+  c0 = sx - b0 + a0; c1 = sy - b1 + a1; c2 = sz - b2 + a2;
 }
 
-// Minimum-image test of a replicated system's pair: s == round(f_i - f_j).
-__device__ inline bool is_min_image(const SysGeom& g, float4 ri, float4 rj, int sx, int sy,
-                                    int sz) {
-  double fi0, fi1, fi2, fj0, fj1, fj2;
-  tmul(g.Li, (double)ri.x, (double)ri.y, (double)ri.z, fi0, fi1, fi2);
-  tmul(g.Li, (double)rj.x, (double)rj.y, (double)rj.z, fj0, fj1, fj2);
-  return rint(fi0 - fj0) == sx && rint(fi1 - fj1) == sy && rint(fi2 - fj2) == sz;
+__device__ __attribute__((noinline)) bool band_keep(const SysGeom& g, int tj, int sx, int sy,
+                                                    int sz, double rc2) {
+  const float *pi, *pj;
+  int c0, c1, c2;
+  band_pair(g, tj, sx, sy, sz, pi, pj, c0, c1, c2);
+  return pair_dist2_64(g, pi, pj, c0, c1, c2) < rc2;
 }
+
+// Minimum-image test of a replicated system's pair on the caller's
+// positions: s == round(f_i - f_j) (the exclusion rule, 731-735).
+__device__ __attribute__((noinline)) bool band_min_image(const SysGeom& g, int tj, int sx, int sy,
+                                                         int sz) {
+  const float *pi, *pj;
+  int c0, c1, c2;
+  band_pair(g, tj, sx, sy, sz, pi, pj, c0, c1, c2);
+  return pair_is_min_image64(g, pi, pj, c0, c1, c2);
+}
+
+// Appends the pair (sorted left atom of this lane, tj, s) to a record in the
+// caller's indices and frame; i0 is the system's first particle.
+__device__ __attribute__((noinline)) void band_record(const PairRecord& r, const SysGeom& g, int i0,
+                                                      int tj, int sx, int sy, int sz, int terms) {
+  const float *pi, *pj;
+  int c0, c1, c2;
+  band_pair(g, tj, sx, sy, sz, pi, pj, c0, c1, c2);
+  const BandCtx& c = band_ctx;
+  record_pair(r, i0 + (int)((pi - c.cpos) / 3), i0 + (int)((pj - c.cpos) / 3), c0, c1, c2, terms);
+}
+
+// Pair vectors with every fused operation spelled out: the main pass and the
+// band pass of a pair kernel form bitwise the same vector and r^2, so they
+// agree on which pairs lie inside a cutoff band.
+__device__ inline void shift_vec64(const double* L, int sx, int sy, int sz, double& x, double& y,
+                                   double& z) {
+  x = fma((double)sz, L[3], fma((double)sy, L[1], (double)sx * L[0]));
+  y = fma((double)sz, L[4], (double)sy * L[2]);
+  z = (double)sz * L[5];
+}
+
+__device__ inline double norm2_64(double x, double y, double z) {
+  return fma(z, z, fma(y, y, x * x));
+}
+
+__device__ inline void shift_vec32(const float* L, int sx, int sy, int sz, float& x, float& y,
+                                   float& z) {
+  x = fmaf((float)sz, L[3], fmaf((float)sy, L[1], (float)sx * L[0]));
+  y = fmaf((float)sz, L[4], (float)sy * L[2]);
+  z = (float)sz * L[5];
+}
+
+__device__ inline float norm2_32(float x, float y, float z) { return fmaf(z, z, fmaf(y, y, x * x)); }
 
 // Per-system constants of the pair terms (cutoffs, FP64 bands, Ewald alpha).
 struct PairParams {
@@ -200,23 +280,34 @@ __device__ inline void pc_pair_params(PairParams& pp, int has_lj, int has_coul, 
   pp.sw_inv_div = pp.has_sw ? (float)(1.0 / (span * span * span)) : 0.0f;
 }
 
-// LJ and real-space Ewald terms of the directed pair (i -> j + s): adds the
-// pair energies to elj / ecl and returns 2 d(E_directed)/d(r^2).  r2d is the
-// FP64 squared distance, r2 its FP32 rounding; exclusion ids are compared on
-// the minimum image only (core/neighborlist.py:731-735).
-template <bool EXCL>
-__device__ inline float pc_pair_terms(const PairParams& pp, const SysGeom& g, float4 ri, float4 rj,
-                                      int sx, int sy, int sz, double r2d, float r2, bool self,
-                                      int si, int sj, int xlj_i, int xlj_j, int xcl_i, int xcl_j,
-                                      float& elj, float& ecl) {
+// LJ and real-space Ewald terms of the directed pair (i -> j + s) of working
+// copies ri, rj (i the lane's sorted atom, j sorted atom tj): adds the pair
+// energies to elj / ecl, sets kept (bit 0 LJ, bit 1 Coulomb) and returns
+// 2 d(E_directed)/d(r^2).  r2d is the FP64 squared distance of the copies,
+// r2 its FP32 rounding; exclusion ids are compared on the minimum image only
+// (core/neighborlist.py:731-735).  Two passes: PASS 0 keeps the pairs below
+// the band of a cutoff and sets inband for a pair inside it; PASS 1 (run
+// only by lanes that met such a pair) keeps exactly the band pairs the
+// caller-frame FP64 decision keeps.  The pair loops of PASS 0 then make no
+// calls, which would pin registers across every trip.
+template <bool EXCL, int PASS>
+__device__ inline float pc_pair_terms(const PairParams& pp, const SysGeom& g, int tj, float4 ri,
+                                      float4 rj, int sx, int sy, int sz, double r2d, float r2,
+                                      bool self, int si, int sj, int xlj_i, int xlj_j, int xcl_i,
+                                      int xcl_j, float& elj, float& ecl, int& kept, bool& inband) {
   const float K = 14.399645351950548f;  // HARTREE * BOHR, eV A
   bool min_img = true;
-  if (EXCL && g.rep) min_img = is_min_image(g, ri, rj, sx, sy, sz);
+  if (EXCL && g.rep) min_img = band_min_image(g, tj, sx, sy, sz);
   float gsum = 0.f;
+  kept = 0;
   if (pp.has_lj && r2d < pp.lj_hi && !self) {
-    bool ok = !(r2d > pp.lj_lo) || keep64(g, ri, rj, sx, sy, sz, pp.lj_rc2);
+    const bool mid = r2d > pp.lj_lo;
+    bool ok;
+    if (PASS == 0) { ok = !mid; inband |= mid; }
+    else ok = mid && band_keep(g, tj, sx, sy, sz, pp.lj_rc2);
     if (EXCL && pp.has_lj_excl && min_img && xlj_j == xlj_i) ok = false;
     if (ok) {
+      kept |= 1;
       const int p = si * pp.ns + sj;
       const float ir2 = 1.0f / r2;
       float c6 = pp.sig2[p] * ir2;
@@ -229,9 +320,13 @@ __device__ inline float pc_pair_terms(const PairParams& pp, const SysGeom& g, fl
     }
   }
   if (pp.has_coul && r2d < pp.cl_hi && !self) {
-    bool ok = !(r2d > pp.cl_lo) || keep64(g, ri, rj, sx, sy, sz, pp.cl_rc2);
+    const bool mid = r2d > pp.cl_lo;
+    bool ok;
+    if (PASS == 0) { ok = !mid; inband |= mid; }
+    else ok = mid && band_keep(g, tj, sx, sy, sz, pp.cl_rc2);
     if (EXCL && pp.has_coul_excl && min_img && xcl_j == xcl_i) ok = false;
     if (ok) {
+      kept |= 2;
       const float r = sqrtf(r2);
       const float ir = 1.0f / r;
       const float qq = ri.w * rj.w * K;
@@ -263,13 +358,19 @@ __device__ inline void pc_block_partials(const double (&v)[PC_NPART], double (*r
   }
 }
 
-// Fixed-order sum of the tile partials of system s into energy[2 s..] (LJ,
-// real space) and the symmetric virial[9 s..].
+// Fixed-order sum of the tile partials of system s (then those of the band
+// pass, if any) into energy[2 s..] (LJ, real space) and the symmetric
+// virial[9 s..].
 __device__ inline void pc_reduce_system(int s, int tiles, const double* partial, int accumulate,
-                                        double* energy, double* virial) {
+                                        double* energy, double* virial,
+                                        const double* partial2 = nullptr) {
   double acc[PC_NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int t = 0; t < tiles; ++t)
     for (int k = 0; k < PC_NPART; ++k) acc[k] += partial[((size_t)s * tiles + t) * PC_NPART + k];
+  // the band pass's partials, after the main pass's
+  if (partial2)
+    for (int t = 0; t < tiles; ++t)
+      for (int k = 0; k < PC_NPART; ++k) acc[k] += partial2[((size_t)s * tiles + t) * PC_NPART + k];
   if (energy) {
     double* e = energy + 2 * (size_t)s;
     if (accumulate) { e[0] += acc[0]; e[1] += acc[1]; } else { e[0] = acc[0]; e[1] = acc[1]; }

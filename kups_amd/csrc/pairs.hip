@@ -18,6 +18,7 @@
 // K/2 sum q_i q_j erfc(alpha r)/r [r < rc]), neighbor semantics of
 // core/neighborlist.py:692-786 (strict cutoff, image grid for narrow cells,
 // exclusion on the minimum image only, 731-735).
+#include <type_traits>
 #include "pair_common.hpp"
 
 namespace kupsmi {
@@ -47,10 +48,12 @@ struct PairArgs {
   int* slj;
   int* scl;
   int* cstart;
-  double* partial;
+  double* partial;      // main pass, then the band pass (n_sys x tiles x PC_NPART each)
+  uint8_t* bandflag;    // per sorted atom: the main pass met a pair in a cutoff band
   float* forces;
   double* energy;
   double* virial;
+  PairRecord rec;  // parity output (REC instantiation only)
 };
 
 __global__ void pc_geom_kernel(PairArgs a) {
@@ -77,7 +80,12 @@ __global__ __launch_bounds__(PC_BIN_THREADS) void pc_bin_kernel(PairArgs a) {
   }
 }
 
-template <bool EXCL>
+// PASS 0: the pair terms of every candidate below the band of a cutoff, one
+// lane per sorted atom; PASS 1 (the band pass): lanes whose PASS 0 met a band
+// pair add the band pairs the FP64 decision on the caller's positions keeps
+// (a launch of its own, so the hot loop makes no calls).  Partials of PASS 1
+// go to the second half of a.partial.
+template <bool EXCL, bool REC, int PASS>
 __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
   __shared__ SysGeom g;
   __shared__ float s_sig2[PC_NSPEC * PC_NSPEC], s_e4[PC_NSPEC * PC_NSPEC],
@@ -87,12 +95,22 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
   const int start = blockIdx.x * PC_TILE;
   const int tid = threadIdx.x;
-  double* part = a.partial + ((size_t)s * a.tiles + blockIdx.x) * PC_NPART;
+  double* part = a.partial + ((size_t)(PASS ? a.n_sys : 0) * a.tiles + (size_t)s * a.tiles +
+                              blockIdx.x) * PC_NPART;
   if (start >= n || n > PC_NMAX) {
     if (tid < PC_NPART) part[tid] = 0.0;
     return;
   }
-  if (tid == 0) g = a.geom[s];
+  if (PASS == 1) {
+    if (!__syncthreads_or(start + tid < n && a.bandflag[i0 + start + tid])) {
+      if (tid < PC_NPART) part[tid] = 0.0;
+      return;
+    }
+  }
+  if (tid == 0) {
+    g = a.geom[s];
+    band_ctx = BandCtx{a.spos + i0, a.pos + 3 * (size_t)i0, a.sidx + i0};
+  }
   const int ns = a.n_species;
   for (int k = tid; k < ns * ns; k += PC_TILE) {
     if (a.has_lj) {
@@ -113,60 +131,68 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
   const int slot = start + tid;
   float ax = 0.f, ay = 0.f, az = 0.f, elj = 0.f, ecl = 0.f;
   float w00 = 0.f, w01 = 0.f, w02 = 0.f, w11 = 0.f, w12 = 0.f, w22 = 0.f;
-  if (slot < n) {
+  bool inband = false;
+  if (slot < n && (PASS == 0 || a.bandflag[i0 + slot])) {
     const float4 ri = a.spos[i0 + slot];
     const int c = a.scell[i0 + slot];
     const int si = a.sspec[i0 + slot];
     const int xlj = EXCL && a.has_lj_excl ? a.slj[i0 + slot] : 0;
     const int xcl = EXCL && a.has_coul_excl ? a.scl[i0 + slot] : 0;
     const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-    for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
-      int zz = cz + dz;
-      const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
-      zz -= sz * ncz;
-      for (int dy = -g.h[1]; dy <= g.h[1]; ++dy) {
-        int yy = cy + dy;
-        const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
-        yy -= sy * ncy;
-        for (int dx = -g.h[0]; dx <= g.h[0]; ++dx) {
-          int xx = cx + dx;
-          const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
-          xx -= sx * ncx;
-          // pair vectors are formed in FP64 (same VALU rate as unpacked FP32 on
-          // gfx950): |r| up to half a box would otherwise cost ~1e-5 relative
-          double Sx, Sy, Sz;
-          tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
-          const double qx = (double)ri.x - Sx, qy = (double)ri.y - Sy, qz = (double)ri.z - Sz;
-          const bool center = dx == 0 && dy == 0 && dz == 0;
-          const int nb = (zz * ncy + yy) * ncx + xx;
-          const int tb = cs[nb], te = cs[nb + 1];
-          for (int t = tb; t < te; ++t) {
-            const float4 rj = a.spos[i0 + t];
-            const double exd = (double)rj.x - qx, eyd = (double)rj.y - qy, ezd = (double)rj.z - qz;
-            const double r2d = exd * exd + eyd * eyd + ezd * ezd;
-            if (!(r2d < all_hi)) continue;
-            const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
-            // particle exclusion: the zero-shift self pair
-            const bool self = center && t == slot;
-            const float gsum = pc_pair_terms<EXCL>(
-                pp, g, ri, rj, sx, sy, sz, r2d, r2, self, si, a.sspec[i0 + t], xlj,
-                EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
-                EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
-            ax += gsum * ex; ay += gsum * ey; az += gsum * ez;
-            const float hg = -0.5f * gsum;
-            w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
-            w11 += hg * ey * ey; w12 += hg * ey * ez; w22 += hg * ez * ez;
+    {
+      for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
+        int zz = cz + dz;
+        const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
+        zz -= sz * ncz;
+        for (int dy = -g.h[1]; dy <= g.h[1]; ++dy) {
+          int yy = cy + dy;
+          const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
+          yy -= sy * ncy;
+          for (int dx = -g.h[0]; dx <= g.h[0]; ++dx) {
+            int xx = cx + dx;
+            const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
+            xx -= sx * ncx;
+            // pair vectors are formed in FP64 (same VALU rate as unpacked FP32
+            // on gfx950): |r| up to half a box would otherwise cost ~1e-5
+            // relative
+            double Sx, Sy, Sz;
+            shift_vec64(g.L, sx, sy, sz, Sx, Sy, Sz);
+            const double qx = (double)ri.x - Sx, qy = (double)ri.y - Sy, qz = (double)ri.z - Sz;
+            const bool center = dx == 0 && dy == 0 && dz == 0;
+            const int nb = (zz * ncy + yy) * ncx + xx;
+            const int tb = cs[nb], te = cs[nb + 1];
+            for (int t = tb; t < te; ++t) {
+              const float4 rj = a.spos[i0 + t];
+              const double exd = (double)rj.x - qx, eyd = (double)rj.y - qy,
+                           ezd = (double)rj.z - qz;
+              const double r2d = norm2_64(exd, eyd, ezd);
+              if (!(r2d < all_hi)) continue;
+              const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
+              // particle exclusion: the zero-shift self pair
+              const bool self = center && t == slot;
+              int kept;
+              const float gsum = pc_pair_terms<EXCL, PASS>(
+                  pp, g, t, ri, rj, sx, sy, sz, r2d, r2, self, si, a.sspec[i0 + t], xlj,
+                  EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
+                  EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl, kept, inband);
+              if (REC && kept) band_record(a.rec, g, i0, t, sx, sy, sz, kept);
+              ax += gsum * ex; ay += gsum * ey; az += gsum * ez;
+              const float hg = -0.5f * gsum;
+              w00 += hg * ex * ex; w01 += hg * ex * ey; w02 += hg * ex * ez;
+              w11 += hg * ey * ey; w12 += hg * ey * ez; w22 += hg * ez * ez;
+            }
           }
         }
       }
     }
     const int gi = i0 + a.sidx[i0 + slot];
-    if (a.accumulate) {
+    if (PASS == 1 || a.accumulate) {
       a.forces[3 * gi] += ax; a.forces[3 * gi + 1] += ay; a.forces[3 * gi + 2] += az;
     } else {
       a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
     }
   }
+  if (PASS == 0 && slot < n) a.bandflag[i0 + slot] = inband;
   const double v[PC_NPART] = {0.5 * elj, 0.5 * ecl, w00, w01, w02, w11, w12, w22};
   pc_block_partials<PC_TILE>(v, red, part);
 }
@@ -174,7 +200,8 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
 __global__ void pc_reduce_kernel(PairArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n_sys) return;
-  pc_reduce_system(s, a.tiles, a.partial, a.accumulate, a.energy, a.virial);
+  pc_reduce_system(s, a.tiles, a.partial, a.accumulate, a.energy, a.virial,
+                   a.partial + (size_t)a.n_sys * a.tiles * PC_NPART);
 }
 
 }  // namespace kupsmi
@@ -191,6 +218,9 @@ extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, cons
                                   int accumulate, float* forces, double* energy, double* virial,
                                   const double* lj_truncation) {
   KUPSMI_REQUIRE(ctx != nullptr, "null context");
+  // a pending pair record is taken by this call
+  const PairRecord rec = ctx->rec_armed ? ctx->rec : PairRecord{};
+  ctx->rec_armed = 0;
   KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0, "negative size");
   KUPSMI_REQUIRE(max_atoms_per_sys <= PC_NMAX, "systems of the pair path hold at most 16384 atoms");
   KUPSMI_REQUIRE(n_species >= 1 && n_species <= PC_NSPEC, "n_species must be in [1, 8]");
@@ -223,7 +253,8 @@ extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   const size_t o_slj = take(sizeof(int) * na);
   const size_t o_scl = take(sizeof(int) * na);
   const size_t o_cs = take(sizeof(int) * (size_t)n_sys * (PC_NCMAX + 1));
-  const size_t o_part = take(sizeof(double) * (size_t)n_sys * a.tiles * PC_NPART);
+  const size_t o_part = take(2 * sizeof(double) * (size_t)n_sys * a.tiles * PC_NPART);
+  const size_t o_flag = take(na);
   void* buf = nullptr;
   int rc = scratch(ctx, 3, off, &buf);
   if (rc) return rc;
@@ -237,6 +268,7 @@ extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   a.scl = reinterpret_cast<int*>(b + o_scl);
   a.cstart = reinterpret_cast<int*>(b + o_cs);
   a.partial = reinterpret_cast<double*>(b + o_part);
+  a.bandflag = reinterpret_cast<uint8_t*>(b + o_flag);
   pc_geom_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   int dev = 0, ncu = 256;
@@ -245,11 +277,26 @@ extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   pc_bin_kernel<<<n_sys < ncu ? n_sys : ncu, PC_BIN_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   dim3 grid(a.tiles, n_sys);
-  if (a.has_lj_excl || a.has_coul_excl)
-    pc_pair_kernel<true><<<grid, PC_TILE, 0, st>>>(a);
-  else
-    pc_pair_kernel<false><<<grid, PC_TILE, 0, st>>>(a);
+  const bool excl = a.has_lj_excl || a.has_coul_excl;
+  if (rec.count) {
+    a.rec = rec;
+    record_clear_kernel<<<1, 1, 0, st>>>(rec.count);
+    KUPSMI_CHECK_LAUNCH();
+  }
+  // main pass, then the band pass
+#define PC_PASS(P)                                                               \
+  if (rec.count) {                                                               \
+    if (excl) pc_pair_kernel<true, true, P><<<grid, PC_TILE, 0, st>>>(a);        \
+    else pc_pair_kernel<false, true, P><<<grid, PC_TILE, 0, st>>>(a);            \
+  } else if (excl) {                                                             \
+    pc_pair_kernel<true, false, P><<<grid, PC_TILE, 0, st>>>(a);                 \
+  } else {                                                                       \
+    pc_pair_kernel<false, false, P><<<grid, PC_TILE, 0, st>>>(a);                \
+  }                                                                              \
   KUPSMI_CHECK_LAUNCH();
+  PC_PASS(0)
+  PC_PASS(1)
+#undef PC_PASS
   pc_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;

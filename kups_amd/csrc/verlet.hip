@@ -33,11 +33,15 @@ struct kupsmi_verlet {
   int n_sys = 0, n_atoms = 0, cap = 0, tiles = 0;
   void* mem = nullptr;
   size_t bytes = 0;
+  // armed by kupsmi_verlet_record_pairs, taken by the next evaluation
+  kupsmi::PairRecord rec;
+  int rec_armed = 0;
 };
 
 namespace kupsmi {
 
 constexpr int VL_TILE = 256;
+static_assert(VL_TILE == PC_TILE, "band_ctx locates the left atom by the pair tile");
 constexpr int VL_SHIFT_MAX = 7;  // |image shift| per axis in a packed entry
 
 struct VlArgs {
@@ -70,12 +74,19 @@ struct VlArgs {
   uint32_t* list;       // cap x n_atoms
   int* valid;           // n_sys
   int* need;            // n_sys: rebuild in this call
+  int* reprc;           // n_sys: the cutoff itself needs the image grid
   unsigned* maxd2;      // n_sys: float bits of the max squared displacement
   unsigned long long* counters;  // [0] system rebuilds, [1] overflowed atom lists
-  double* partial;
+  double* partial;      // main pass, then the band pass (n_sys x tiles x PC_NPART each)
+  // per sorted atom: the number of list entries inside a cutoff band met by
+  // the main pass (VL_NBAND_FULL: more than VL_NBAND, or met on the stencil
+  // path), and up to VL_NBAND of those entries
+  uint8_t* bandflag;
+  uint32_t* bandbuf;
   float* forces;
   double* energy;
   double* virial;
+  PairRecord rec;  // parity output (REC instantiation only)
 };
 
 // List entry: partner slot (15 bits), image shift + 8 per axis (4 bits each),
@@ -117,6 +128,10 @@ __global__ __launch_bounds__(VL_TILE) void vl_gather_kernel(VlArgs a) {
     const double ux = (double)r[0] - s0, uy = (double)r[1] - s1, uz = (double)r[2] - s2;
     const double dx = ux - rb.x, dy = uy - rb.y, dz = uz - rb.z;
     d2 = (float)(dx * dx + dy * dy + dz * dz);
+    // a system whose cutoff needs the image grid takes the reference's
+    // centred grid of shifts in the caller's frame (core/neighborlist.py:
+    // 595-624), which moves with every wrap: rebuild once an atom was wrapped
+    if (a.reprc[s] && (rint(f0) != 0.0 || rint(f1) != 0.0 || rint(f2) != 0.0)) d2 = INFINITY;
     a.spos[d] = make_float4((float)ux, (float)uy, (float)uz, rb.w);
   }
   for (int o = 32; o > 0; o >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
@@ -151,6 +166,9 @@ __global__ __launch_bounds__(PC_BIN_THREADS) void vl_bin_kernel(VlArgs a) {
         double rc = 0.0;
         if (a.has_lj) rc = a.lj_rc[s];
         if (a.has_coul && a.cl_rc[s] > rc) rc = a.cl_rc[s];
+        SysGeom g0;
+        make_geom(a.tril + 6 * s, rc, PC_MAX_NC_AXIS, g0, 2);
+        a.reprc[s] = g0.rep;
         make_geom(a.tril + 6 * s, rc + a.skin, PC_MAX_NC_AXIS, L.g, 2);
         a.geom[s] = L.g;
         a.valid[s] = 1;
@@ -225,45 +243,53 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
   if (cnt > a.cap) atomicAdd(a.counters + 1, 1ull);
 }
 
-// FP64 re-decision of a pair within VL_BAND32 of a cutoff (the reference's
-// fractional formula).  Out of line: the rare call claims no registers in
-// the list loop.
-__device__ __attribute__((noinline)) bool vl_keep64(const SysGeom& g, float4 ri, float4 rj, int sx,
-                                                    int sy, int sz, double rc2) {
-  return keep64(g, ri, rj, sx, sy, sz, rc2);
-}
-
-__device__ __attribute__((noinline)) bool vl_min_image(const SysGeom& g, float4 ri, float4 rj,
-                                                       int sx, int sy, int sz) {
-  return is_min_image(g, ri, rj, sx, sy, sz);
-}
-
 // relative half width of the FP32 cutoff band re-decided in FP64: the FP32
 // pair vector of positions within 1.5 box lengths of the origin is good to
 // ~4e-6 A per component, i.e. ~2e-4 A^2 in r^2 at rc = 12 A, well inside
 // 4e-6 rc^2 = 6e-4 A^2
 constexpr float VL_BAND32 = 4e-6f;
+// band entries kept per atom for the band pass (about 0.4 % of the atoms of
+// a 12 A melt meet one at all), and the flag of a lane the band pass walks
+// in full
+constexpr int VL_NBAND = 4;
+constexpr int VL_NBAND_FULL = 255;
+// band-entry word: the list entry's slot and shifts, plus which terms had
+// the pair inside their band
+constexpr uint32_t VL_MID_LJ = 1u << 27, VL_MID_CL = 1u << 28;
 
 // LJ and real-space Ewald terms of one list entry, d = r_j + s@L - r_i in
 // FP32.  Each cutoff test is folded into its 1/r factor (zero for a rejected
-// pair), so the loop has no data-dependent branch but the rare band case;
-// exclusion ids compare on the minimum image only (core/neighborlist.py:
-// 731-735).  Adds the pair energies to elj / ecl and returns u'(r) / r.
-template <bool EXCL>
+// pair), so the loop has no data-dependent branch; exclusion ids compare on
+// the minimum image only (core/neighborlist.py:731-735).  Adds the pair
+// energies to elj / ecl, sets kept (bit 0 LJ, bit 1 Coulomb) and returns
+// u'(r) / r.  PASS 0 keeps the pairs below the band of a cutoff and sets
+// mid (bit 0 LJ, bit 1 Coulomb) for a pair inside a band; PASS 1 keeps
+// exactly the band pairs that the FP64 decision on the caller's positions
+// keeps, taking the band bits from given_mid when the main pass stored them
+// (the band pass is a launch of its own, so the hot loop makes no calls).
+template <bool EXCL, int PASS>
 __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)[4],
-                                       const SysGeom& g, float4 ri, float4 rj, int sx, int sy,
-                                       int sz, float r2, int si, int sj, int xlj_i, int xlj_j,
-                                       int xcl_i, int xcl_j, float& elj, float& ecl) {
+                                       const SysGeom& g, int tj, float4 ri, float4 rj, int sx,
+                                       int sy, int sz, float r2, int si, int sj, int xlj_i,
+                                       int xlj_j, int xcl_i, int xcl_j, float& elj, float& ecl,
+                                       int& kept, int& mid, int given_mid = -1) {
   const float K = 14.399645351950548f;  // HARTREE * BOHR, eV A
   float gsum = 0.f;
   bool min_img = true;
+  kept = 0;
+  mid = 0;
   if (EXCL && g.rep && ((pp.has_lj_excl && xlj_j == xlj_i) || (pp.has_coul_excl && xcl_j == xcl_i)))
-    min_img = vl_min_image(g, ri, rj, sx, sy, sz);
+    min_img = band_min_image(g, tj, sx, sy, sz);
   if (pp.has_lj) {
     const bool lo = r2 < band[0];
-    bool keep = lo;
-    if (__builtin_expect(!lo && r2 < band[1], 0)) keep = vl_keep64(g, ri, rj, sx, sy, sz, pp.lj_rc2);
+    bool m = !lo && r2 < band[1];
+    if (PASS == 1 && given_mid >= 0) m = (given_mid & 1) != 0;
+    bool keep;
+    if (PASS == 0) keep = lo;
+    else keep = m && band_keep(g, tj, sx, sy, sz, pp.lj_rc2);
     if (EXCL && pp.has_lj_excl && min_img && xlj_j == xlj_i) keep = false;
+    kept |= keep ? 1 : 0;
+    mid |= m ? 1 : 0;
     const int p = si * pp.ns + sj;
     const float ir2 = keep ? __builtin_amdgcn_rcpf(r2) : 0.0f;
     float c6 = pp.sig2[p] * ir2;
@@ -273,9 +299,14 @@ __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)
   }
   if (pp.has_coul) {
     const bool lo = r2 < band[2];
-    bool keep = lo;
-    if (__builtin_expect(!lo && r2 < band[3], 0)) keep = vl_keep64(g, ri, rj, sx, sy, sz, pp.cl_rc2);
+    bool m = !lo && r2 < band[3];
+    if (PASS == 1 && given_mid >= 0) m = (given_mid & 2) != 0;
+    bool keep;
+    if (PASS == 0) keep = lo;
+    else keep = m && band_keep(g, tj, sx, sy, sz, pp.cl_rc2);
     if (EXCL && pp.has_coul_excl && min_img && xcl_j == xcl_i) keep = false;
+    kept |= keep ? 2 : 0;
+    mid |= m ? 2 : 0;
     const float ir = keep ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     const float x = pp.alpha * (r2 * ir);
     const float ec = erfc_cheb(x);
@@ -287,7 +318,13 @@ __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)
   return gsum;
 }
 
-template <bool EXCL, bool VIR>
+// PASS 0: the pair terms of every list entry below the band of a cutoff, one
+// lane per sorted atom; the lane keeps up to VL_NBAND entries it met inside a
+// band.  PASS 1 (the band pass): each lane that met any adds the band pairs
+// the FP64 decision keeps, from its stored entries, or, past VL_NBAND, by
+// walking its list again.  Partials of PASS 1 go to the second half of
+// a.partial.
+template <bool EXCL, bool VIR, bool REC, int PASS>
 __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   __shared__ SysGeom g;
   __shared__ float s_sig2[PC_NSPEC * PC_NSPEC], s_e4[PC_NSPEC * PC_NSPEC],
@@ -297,12 +334,23 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
   const int start = blockIdx.x * VL_TILE;
   const int tid = threadIdx.x;
-  double* part = a.partial + ((size_t)s * a.tiles + blockIdx.x) * PC_NPART;
+  double* part = a.partial + ((size_t)(PASS ? a.n_sys : 0) * a.tiles + (size_t)s * a.tiles +
+                              blockIdx.x) * PC_NPART;
   if (start >= n || n > PC_NMAX) {
     if (tid < PC_NPART) part[tid] = 0.0;
     return;
   }
-  if (tid == 0) g = a.geom[s];
+  if (PASS == 1) {
+    const int d = i0 + start + tid;
+    if (!__syncthreads_or(start + tid < n && a.bandflag[d])) {
+      if (tid < PC_NPART) part[tid] = 0.0;
+      return;
+    }
+  }
+  if (tid == 0) {
+    g = a.geom[s];
+    band_ctx = BandCtx{a.spos + i0, a.pos + 3 * (size_t)i0, a.sidx + i0};
+  }
   const int ns = a.n_species;
   for (int k = tid; k < ns * ns; k += VL_TILE) {
     if (a.has_lj) {
@@ -321,7 +369,10 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   const int slot = start + tid;
   float ax = 0.f, ay = 0.f, az = 0.f, elj = 0.f, ecl = 0.f;
   float w00 = 0.f, w01 = 0.f, w02 = 0.f, w11 = 0.f, w12 = 0.f, w22 = 0.f;
-  if (slot < n) {
+  int nband = 0;  // band entries met (PASS 0)
+  bool inband = false;  // met a band pair on the stencil path
+  const int flag = PASS == 1 && slot < n ? a.bandflag[i0 + slot] : 0;
+  if (slot < n && (PASS == 0 || flag)) {
     const int d = i0 + slot;
     const float4 ri = a.spos[d];
     const int si = a.sspec[d];
@@ -336,7 +387,28 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
       }
     };
     const int cnt = a.nlist[d];
-    if (cnt <= a.cap) {
+    if (PASS == 1 && flag <= VL_NBAND) {
+      // the band pass over the entries the main pass stored
+      for (int k = 0; k < flag; ++k) {
+        const uint32_t e = a.bandbuf[(size_t)d * VL_NBAND + k];
+        const int t = (int)(e & 0x7fffu);
+        const int sx = (int)((e >> 15) & 15u) - 8, sy = (int)((e >> 19) & 15u) - 8,
+                  sz = (int)((e >> 23) & 15u) - 8;
+        const float4 rj = a.spos[i0 + t];
+        float Sx, Sy, Sz;
+        shift_vec32(g.Lf, sx, sy, sz, Sx, Sy, Sz);
+        const float ex = (rj.x + Sx) - ri.x, ey = (rj.y + Sy) - ri.y, ez = (rj.z + Sz) - ri.z;
+        const float r2 = norm2_32(ex, ey, ez);
+        int kept, mid;
+        const float gsum = vl_entry_terms<EXCL, PASS>(
+            pp, band, g, t, ri, rj, sx, sy, sz, r2, si, a.sspec[i0 + t], xlj,
+            EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
+            EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl, kept, mid,
+            (e & VL_MID_LJ ? 1 : 0) | (e & VL_MID_CL ? 2 : 0));
+        if (REC && kept) band_record(a.rec, g, i0, t, sx, sy, sz, kept);
+        accumulate(gsum, ex, ey, ez);
+      }
+    } else if (cnt <= a.cap) {
       // four entries per trip; while group q is evaluated, the positions of
       // group q + 1 and the list words of group q + 2 are in flight (indices
       // past the end are clamped to the last entry and masked off)
@@ -371,15 +443,23 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
           const int sx = (int)((e >> 15) & 15u) - 8, sy = (int)((e >> 19) & 15u) - 8,
                     sz = (int)((e >> 23) & 15u) - 8;
           float Sx, Sy, Sz;
-          tmul(g.Lf, (float)sx, (float)sy, (float)sz, Sx, Sy, Sz);
+          shift_vec32(g.Lf, sx, sy, sz, Sx, Sy, Sz);
           const float ex = (rj.x + Sx) - ri.x, ey = (rj.y + Sy) - ri.y, ez = (rj.z + Sz) - ri.z;
           // a masked-off entry gets a huge finite r^2 (above every cutoff
           // band; times the zero 1/r of a rejected pair it stays 0)
-          const float r2 = k0 + u < cnt ? ex * ex + ey * ey + ez * ez : 3.0e38f;
-          const float gsum = vl_entry_terms<EXCL>(
-              pp, band, g, ri, rj, sx, sy, sz, r2, si, (int)(e >> 27),
+          const float r2 = k0 + u < cnt ? norm2_32(ex, ey, ez) : 3.0e38f;
+          int kept, mid;
+          const float gsum = vl_entry_terms<EXCL, PASS>(
+              pp, band, g, t, ri, rj, sx, sy, sz, r2, si, (int)(e >> 27),
               xlj, EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
-              EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
+              EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl, kept, mid);
+          if (REC && kept) band_record(a.rec, g, i0, t, sx, sy, sz, kept);
+          if (PASS == 0 && __builtin_expect(mid != 0, 0)) {
+            if (nband < VL_NBAND)
+              a.bandbuf[(size_t)d * VL_NBAND + nband] =
+                  (e & 0x7ffffffu) | (mid & 1 ? VL_MID_LJ : 0u) | (mid & 2 ? VL_MID_CL : 0u);
+            ++nband;
+          }
           accumulate(gsum, ex, ey, ez);
         }
 #pragma unroll
@@ -394,17 +474,19 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
       auto visit = [&](int t, float4 rj, int sj, int sx, int sy, int sz, bool self) {
         // FP64 pair vector d = r_j - r_i + s @ L
         double Sx, Sy, Sz;
-        tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
+        shift_vec64(g.L, sx, sy, sz, Sx, Sy, Sz);
         const double exd = (double)rj.x - ((double)ri.x - Sx);
         const double eyd = (double)rj.y - ((double)ri.y - Sy);
         const double ezd = (double)rj.z - ((double)ri.z - Sz);
-        const double r2d = exd * exd + eyd * eyd + ezd * ezd;
+        const double r2d = norm2_64(exd, eyd, ezd);
         if (!(r2d < pp.all_hi)) return;
         const float ex = (float)exd, ey = (float)eyd, ez = (float)ezd, r2 = (float)r2d;
-        const float gsum = pc_pair_terms<EXCL>(
-            pp, g, ri, rj, sx, sy, sz, r2d, r2, self, si, sj, xlj,
+        int kept;
+        const float gsum = pc_pair_terms<EXCL, PASS>(
+            pp, g, t, ri, rj, sx, sy, sz, r2d, r2, self, si, sj, xlj,
             EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
-            EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl);
+            EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl, kept, inband);
+        if (REC && kept) band_record(a.rec, g, i0, t, sx, sy, sz, kept);
         accumulate(gsum, ex, ey, ez);
       };
       const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
@@ -432,12 +514,14 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
       }
     }
     const int gi = i0 + a.sidx[d];
-    if (a.accumulate) {
+    if (PASS == 1 || a.accumulate) {
       a.forces[3 * gi] += ax; a.forces[3 * gi + 1] += ay; a.forces[3 * gi + 2] += az;
     } else {
       a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
     }
   }
+  if (PASS == 0 && slot < n)
+    a.bandflag[i0 + slot] = inband || nband > VL_NBAND ? VL_NBAND_FULL : nband;
   const double v[PC_NPART] = {0.5 * elj, 0.5 * ecl, w00, w01, w02, w11, w12, w22};
   pc_block_partials<VL_TILE>(v, red, part);
 }
@@ -445,13 +529,15 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
 __global__ void vl_reduce_kernel(VlArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n_sys) return;
-  pc_reduce_system(s, a.tiles, a.partial, a.accumulate, a.energy, a.virial);
+  pc_reduce_system(s, a.tiles, a.partial, a.accumulate, a.energy, a.virial,
+                   a.partial + (size_t)a.n_sys * a.tiles * PC_NPART);
 }
 
 // Byte offsets of the handle's arrays.
 struct VlLayout {
-  size_t geom, spos, sref, sidx, scell, sspec, slj, scl, cstart, nlist, list, valid, need, maxd2,
-      counters, partial, total;
+  size_t geom, spos, sref, sidx, scell, sspec, slj, scl, cstart, nlist, list, valid, need, reprc,
+      maxd2,
+      counters, partial, bandflag, bandbuf, total;
   VlLayout(int n_sys, int n_atoms, int cap, int tiles) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
@@ -469,9 +555,12 @@ struct VlLayout {
     list = take(sizeof(uint32_t) * na * (size_t)cap);
     valid = take(sizeof(int) * ns);
     need = take(sizeof(int) * ns);
+    reprc = take(sizeof(int) * ns);
     maxd2 = take(sizeof(unsigned) * ns);
     counters = take(sizeof(unsigned long long) * 2);
-    partial = take(sizeof(double) * ns * (size_t)tiles * PC_NPART);
+    partial = take(2 * sizeof(double) * ns * (size_t)tiles * PC_NPART);
+    bandflag = take(na);
+    bandbuf = take(sizeof(uint32_t) * na * VL_NBAND);
     total = off;
   }
 };
@@ -500,6 +589,17 @@ extern "C" int kupsmi_verlet_destroy(kupsmi_verlet* v) {
     KUPSMI_HIP(e);
   }
   delete v;
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_verlet_record_pairs(kupsmi_verlet* v, int64_t capacity, int32_t* ij,
+                                         int32_t* shift, uint8_t* terms,
+                                         unsigned long long* count) {
+  KUPSMI_REQUIRE(v != nullptr, "null handle");
+  KUPSMI_REQUIRE(capacity >= 0 && count != nullptr, "count pointer and capacity >= 0 required");
+  KUPSMI_REQUIRE(capacity == 0 || (ij && shift), "ij and shift buffers required");
+  v->rec = PairRecord{capacity, ij, shift, terms, count};
+  v->rec_armed = 1;
   return KUPSMI_OK;
 }
 
@@ -591,9 +691,12 @@ extern "C" int kupsmi_pair_forces_verlet(
   a.list = reinterpret_cast<uint32_t*>(b + lay.list);
   a.valid = reinterpret_cast<int*>(b + lay.valid);
   a.need = reinterpret_cast<int*>(b + lay.need);
+  a.reprc = reinterpret_cast<int*>(b + lay.reprc);
   a.maxd2 = reinterpret_cast<unsigned*>(b + lay.maxd2);
   a.counters = reinterpret_cast<unsigned long long*>(b + lay.counters);
   a.partial = reinterpret_cast<double*>(b + lay.partial);
+  a.bandflag = reinterpret_cast<uint8_t*>(b + lay.bandflag);
+  a.bandbuf = reinterpret_cast<uint32_t*>(b + lay.bandbuf);
   a.forces = forces; a.energy = energy; a.virial = virial;
   int ncu = 256;
   KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -607,14 +710,30 @@ extern "C" int kupsmi_pair_forces_verlet(
   vl_list_kernel<<<grid, VL_TILE, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   const bool excl = a.has_lj_excl || a.has_coul_excl, vir = a.virial != nullptr;
-  if (excl) {
-    if (vir) vl_pair_kernel<true, true><<<grid, VL_TILE, 0, st>>>(a);
-    else vl_pair_kernel<true, false><<<grid, VL_TILE, 0, st>>>(a);
-  } else {
-    if (vir) vl_pair_kernel<false, true><<<grid, VL_TILE, 0, st>>>(a);
-    else vl_pair_kernel<false, false><<<grid, VL_TILE, 0, st>>>(a);
+  // main pass, then the band pass; with a pending pair record both run as the
+  // recording instantiation (the same pair decisions)
+  const bool rec = v->rec_armed != 0;
+  if (rec) {
+    a.rec = v->rec;
+    v->rec_armed = 0;
+    record_clear_kernel<<<1, 1, 0, st>>>(a.rec.count);
+    KUPSMI_CHECK_LAUNCH();
   }
+#define VL_PASS(P)                                                                 \
+  if (rec) {                                                                       \
+    if (excl) vl_pair_kernel<true, true, true, P><<<grid, VL_TILE, 0, st>>>(a);    \
+    else vl_pair_kernel<false, true, true, P><<<grid, VL_TILE, 0, st>>>(a);        \
+  } else if (excl) {                                                               \
+    if (vir) vl_pair_kernel<true, true, false, P><<<grid, VL_TILE, 0, st>>>(a);    \
+    else vl_pair_kernel<true, false, false, P><<<grid, VL_TILE, 0, st>>>(a);       \
+  } else {                                                                         \
+    if (vir) vl_pair_kernel<false, true, false, P><<<grid, VL_TILE, 0, st>>>(a);   \
+    else vl_pair_kernel<false, false, false, P><<<grid, VL_TILE, 0, st>>>(a);      \
+  }                                                                                \
   KUPSMI_CHECK_LAUNCH();
+  VL_PASS(0)
+  VL_PASS(1)
+#undef VL_PASS
   vl_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;

@@ -171,6 +171,84 @@ long orc_neighborlist(const double* pos, const int32_t* excl, int nsys,
   return count;
 }
 
+/*
+ * rh-subset query (NearestNeighborList.__call__(lh, rh, systems, cutoffs,
+ * rh_index_remap), core/neighborlist.py:233-255, through basic_neighborlist,
+ * 743-786): candidates (i in lh, r in rh) of the same system
+ * (_build_segmentation_mask, 505-539; an rh system outside [0, nsys) drops
+ * r); with a remap, an lh particle that any remap entry names is paired only
+ * with entries whose remapped index is <= i (536-537: lh_idx >= rh_idx_out);
+ * shift = round(f_i - f_r) or the centred image grid of a replicated system
+ * (542-564, 595-624); kept when |(f_i - f_r - s) @ L|^2 < rc^2 and not
+ * (equal exclusion ids on the minimum image) (731-735); output (i, out_r, s)
+ * with out_r = remap[r] (n_lh for a remap outside [0, n_lh)) or r, and with
+ * a remap every edge again reversed, (out_r, i, -s), after all forward edges
+ * (_compact_edges, 680-685).  Forward edges in (r, i, offset) order.  Returns
+ * the total count (both directions); writes at most cap.
+ */
+long orc_neighborlist_query(const double* pos, const int32_t* excl, int nsys,
+                            const int32_t* sys_off, int k, const double* rh_pos,
+                            const int32_t* rh_sys, const int32_t* rh_excl,
+                            const int32_t* remap, const double* tril, const double* cutoff,
+                            long cap, int32_t* out_ij, int32_t* out_shift) {
+  const int n = sys_off[nsys];
+  char* named = (char*)calloc(n > 0 ? n : 1, 1);
+  if (remap)
+    for (int r = 0; r < k; ++r)
+      if (remap[r] >= 0 && remap[r] < n) named[remap[r]] = 1;
+  /* forward edges first (count), then written with their reverses */
+  long fwd = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    long e = 0;
+    for (int r = 0; r < k; ++r) {
+      const int s = rh_sys[r];
+      if (s < 0 || s >= nsys) continue;
+      cell_t c; cell_from_tril(tril + 6 * s, &c);
+      const double rc = cutoff[s];
+      int img[3]; image_counts(&c, rc, img);
+      const int replicated = img[0] * img[1] * img[2] > 1;
+      const int h[3] = {(img[0]-1)/2, (img[1]-1)/2, (img[2]-1)/2};
+      int out_r = r;
+      if (remap) out_r = (remap[r] >= 0 && remap[r] < n) ? remap[r] : n;
+      double fr[3]; rmul(c.Li, rh_pos + 3*r, fr);
+      for (int i = sys_off[s]; i < sys_off[s+1]; ++i) {
+        if (remap && named[i] && !(i >= out_r)) continue;
+        double fi[3]; rmul(c.Li, pos + 3*i, fi);
+        double df[3] = {fi[0]-fr[0], fi[1]-fr[1], fi[2]-fr[2]};
+        double mins[3] = {rint(df[0]), rint(df[1]), rint(df[2])};
+        const int same = excl[i] == rh_excl[r];
+        for (int o0 = -h[0]; o0 <= h[0]; ++o0)
+          for (int o1 = -h[1]; o1 <= h[1]; ++o1)
+            for (int o2 = -h[2]; o2 <= h[2]; ++o2) {
+              double sh[3] = {o0, o1, o2};
+              if (!replicated) { sh[0] = mins[0]; sh[1] = mins[1]; sh[2] = mins[2]; }
+              double dd[3] = {df[0]-sh[0], df[1]-sh[1], df[2]-sh[2]}, d[3];
+              rmul(c.L, dd, d);
+              const double r2 = d[0]*d[0] + d[1]*d[1] + d[2]*d[2];
+              const int is_min = sh[0] == mins[0] && sh[1] == mins[1] && sh[2] == mins[2];
+              if (r2 < rc*rc && !(same && is_min)) {
+                if (pass == 1) {
+                  if (e < cap) {
+                    out_ij[2*e] = i; out_ij[2*e+1] = out_r;
+                    for (int a = 0; a < 3; ++a) out_shift[3*e+a] = (int32_t)sh[a];
+                  }
+                  const long rv = fwd + e;
+                  if (remap && rv < cap) {
+                    out_ij[2*rv] = out_r; out_ij[2*rv+1] = i;
+                    for (int a = 0; a < 3; ++a) out_shift[3*rv+a] = -(int32_t)sh[a];
+                  }
+                }
+                ++e;
+              }
+            }
+      }
+    }
+    fwd = e;
+  }
+  free(named);
+  return remap ? 2 * fwd : fwd;
+}
+
 /* ------------------------------------------------------ pair potentials -- */
 /* Edge difference vector d = r_j - r_i + s @ L (core/neighborlist.py:164-184). */
 static inline void edge_vec(const double* pos, const cell_t* c, int i, int j,

@@ -32,6 +32,7 @@ def lib():
             build()
         _lib = C.CDLL(str(LIB_PATH))
         _lib.orc_neighborlist.restype = C.c_long
+        _lib.orc_neighborlist_query.restype = C.c_long
         _lib.orc_kvecs.restype = C.c_long
         _lib.orc_coulomb_k.restype = C.c_double
         _lib.orc_md_vv_f32.restype = C.c_int
@@ -78,6 +79,25 @@ def neighborlist(pos, excl, sys_off, tril, cutoff):
     sh = np.zeros((max(n, 1), 3), np.int32)
     n2 = L.orc_neighborlist(_p(pos), _p(excl), nsys, _p(sys_off), _p(tril), _p(cutoff),
                             C.c_long(n), _p(ij), _p(sh))
+    assert n2 == n
+    return ij[:n], sh[:n]
+
+
+def neighborlist_query(pos, excl, sys_off, tril, cutoff, rh_pos, rh_sys, rh_excl, remap=None):
+    """rh-subset query (i, out_r, shift) of orc_neighborlist_query: forward
+    edges in (r, i, image) order, then (with a remap) their reverses."""
+    pos, excl, sys_off, tril, cutoff = _d(pos), _i(excl), _i(sys_off), _d(tril), _d(cutoff)
+    rh_pos, rh_sys, rh_excl = _d(rh_pos).reshape(-1, 3), _i(rh_sys), _i(rh_excl)
+    rm = None if remap is None else _i(remap)
+    nsys, k = len(sys_off) - 1, len(rh_sys)
+    L = lib()
+    args = (_p(pos), _p(excl), nsys, _p(sys_off), k, _p(rh_pos), _p(rh_sys), _p(rh_excl),
+            None if rm is None else _p(rm), _p(tril), _p(cutoff))
+    dummy_ij, dummy_s = np.zeros(2, np.int32), np.zeros(3, np.int32)
+    n = L.orc_neighborlist_query(*args, C.c_long(0), _p(dummy_ij), _p(dummy_s))
+    ij = np.zeros((max(n, 1), 2), np.int32)
+    sh = np.zeros((max(n, 1), 3), np.int32)
+    n2 = L.orc_neighborlist_query(*args, C.c_long(n), _p(ij), _p(sh))
     assert n2 == n
     return ij[:n], sh[:n]
 

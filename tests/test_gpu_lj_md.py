@@ -14,7 +14,8 @@ import torch
 import oracle
 from kups_amd import workloads
 from kups_amd.core.batch import ParticleBatch, SystemBatch
-from kups_amd.core.neighborlist import CellListNeighborList, neighborlist_changes
+from kups_amd.core.neighborlist import (CellListNeighborList, DenseNearestNeighborList,
+                                        neighborlist_changes)
 from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
 from kups_amd.md.integrators import (VELOCITY_VERLET, FusedLJPropagator,
                                      make_baoab_langevin_step, make_csvr_npt_step,
@@ -39,7 +40,9 @@ def gpu_nl(pos, sizes, tril, cutoff, excl=None, dev="cuda"):
                        exclusion=None if excl is None else torch.as_tensor(np.asarray(excl, np.int32), device=dev))
     sb = SystemBatch(torch.as_tensor(np.asarray(tril, np.float64).reshape(-1, 6), device=dev))
     cut = torch.as_tensor(np.broadcast_to(np.asarray(cutoff, np.float64), (len(sizes),)).copy(), device=dev)
-    e = CellListNeighborList(avg_edges=1)(pb, None, sb, cut)
+    # the dense export: same arrays and order as the oracle (the cell-list
+    # export's edge set is compared in test_gpu_pairsets.py)
+    e = DenseNearestNeighborList(avg_edges=1)(pb, None, sb, cut)
     return e.indices.cpu().numpy(), e.shifts[:, 0].cpu().numpy()
 
 
