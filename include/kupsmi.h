@@ -5,7 +5,10 @@
  * Conventions (all entry points):
  *   - Every pointer argument except `ctx` and host scalars is a DEVICE pointer
  *     owned by the caller.  Nothing here allocates on the hot path: scratch
- *     lives in the context and only grows (kupsmi_ctx_reserve / first use).
+ *     lives in the context and grows on first use by a larger call.  Growing
+ *     a scratch slot frees the old buffer, so work recorded with a context
+ *     (a captured HIP graph) must keep that context to itself and freeze it
+ *     (kupsmi_ctx_freeze), which makes any further growth an error.
  *   - `stream` is a hipStream_t passed as void*; all work is enqueued on it
  *     and no entry point synchronises the host.  Capacity overflows are
  *     reported through device counters that the caller reads at its own sync
@@ -51,6 +54,10 @@ const char* kupsmi_last_error(void);
  * adapter would keep per device. */
 int kupsmi_ctx_create(kupsmi_ctx** out);
 int kupsmi_ctx_destroy(kupsmi_ctx* ctx);
+/* frozen != 0: a call that needs more scratch than the context holds fails
+ * (KUPSMI_ERR_ARG) instead of reallocating, so addresses held by recorded
+ * work stay valid; frozen == 0 lifts it.  No counterpart in the reference. */
+int kupsmi_ctx_freeze(kupsmi_ctx* ctx, int frozen);
 
 /*
  * Full directed neighbor list of every system (lh = rh, no remap).
@@ -326,6 +333,10 @@ int kupsmi_verlet_invalidate(kupsmi_verlet* v, void* stream);
 /* out[0]: system rebuilds, out[1]: atom lists that overflowed (host array,
  * synchronises the stream). */
 int kupsmi_verlet_counters(kupsmi_verlet* v, void* stream, unsigned long long* out);
+/* frozen != 0: a call that would resize the handle's buffers (more systems,
+ * atoms, list entries or a larger system) fails instead of reallocating
+ * them; for handles used by recorded work. */
+int kupsmi_verlet_freeze(kupsmi_verlet* v, int frozen);
 /* Arms the handle's next kupsmi_pair_forces_verlet to record its pair set,
  * as kupsmi_ctx_record_pairs does for the context's calls. */
 int kupsmi_verlet_record_pairs(kupsmi_verlet* v, int64_t capacity, int32_t* ij, int32_t* shift,

@@ -26,6 +26,8 @@ _SIGNATURES = {
     "kupsmi_last_error": ([], C.c_char_p),
     "kupsmi_ctx_create": ([C.POINTER(vp)], C.c_int),
     "kupsmi_ctx_destroy": ([vp], C.c_int),
+    "kupsmi_ctx_freeze": ([vp, i32], C.c_int),
+    "kupsmi_verlet_freeze": ([vp, i32], C.c_int),
     "kupsmi_neighborlist": ([vp, vp, i32, vp, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
     "kupsmi_neighborlist_cells": (
         [vp, vp, i32, vp, i32, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
@@ -133,6 +135,34 @@ def ctx() -> C.c_void_p:
         check(load().kupsmi_ctx_create(C.byref(h)))
         _ctx[dev] = h
     return _ctx[dev]
+
+
+class Context:
+    """A library context of its own (scratch that no other caller can grow or
+    free), e.g. for work recorded in a HIP graph; ``freeze()`` then makes any
+    call that would have to grow its scratch fail instead of reallocating."""
+
+    def __init__(self):
+        require_gpu()
+        h = vp()
+        check(load().kupsmi_ctx_create(C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def freeze(self, frozen: bool = True) -> None:
+        check(load().kupsmi_ctx_freeze(self._h, int(frozen)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().kupsmi_ctx_destroy(h)
+            except Exception:
+                pass
+            self._h = None
 
 
 def stream_ptr(stream: torch.cuda.Stream | None = None) -> int:

@@ -43,6 +43,11 @@ class VerletList:
     def handle(self) -> C.c_void_p:
         return self._h
 
+    def freeze(self, frozen: bool = True) -> None:
+        """While frozen, a call that would resize the handle's buffers fails
+        (for handles used by recorded work)."""
+        _lib.check(_lib.load().kupsmi_verlet_freeze(self._h, int(frozen)))
+
     def invalidate(self):
         _lib.check(_lib.load().kupsmi_verlet_invalidate(self._h, _lib.stream_ptr()))
 

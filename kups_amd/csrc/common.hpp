@@ -28,7 +28,10 @@ void set_error(const std::string& msg);
 
 // ------------------------------------------------------------ context ----
 // Grow-only device scratch.  Allocation happens only when a call needs more
-// than any earlier call; steady-state calls allocate nothing.
+// than any earlier call; steady-state calls allocate nothing.  Growing a slot
+// frees its old buffer, so work recorded with the old address (a captured
+// HIP graph) must not outlive it: kupsmi_ctx_freeze turns growth into an
+// error.
 struct Scratch {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -50,6 +53,9 @@ struct kupsmi_ctx {
   // armed by kupsmi_ctx_record_pairs, taken by the next pair-evaluating call
   kupsmi::PairRecord rec;
   int rec_armed = 0;
+  // set by kupsmi_ctx_freeze: a slot that would have to grow is an error
+  // (recorded work holds the slots' addresses)
+  int frozen = 0;
 };
 
 namespace kupsmi {

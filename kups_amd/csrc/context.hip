@@ -13,6 +13,9 @@ int scratch(kupsmi_ctx* ctx, int slot, size_t bytes, void** out) {
   KUPSMI_REQUIRE(slot >= 0 && slot < 16, "bad scratch slot");
   Scratch& s = ctx->slots[slot];
   if (s.bytes < bytes) {
+    KUPSMI_REQUIRE(!ctx->frozen,
+                   "scratch of a frozen context cannot grow (recorded work uses its buffers): "
+                   "size it before freezing, or use another context");
     // Grow geometrically so a slowly growing workload reallocates rarely.
     size_t nb = s.bytes * 2 > bytes ? s.bytes * 2 : bytes;
     nb = (nb + 255) & ~(size_t)255;
@@ -50,6 +53,12 @@ extern "C" int kupsmi_ctx_record_pairs(kupsmi_ctx* ctx, int64_t capacity, int32_
   KUPSMI_REQUIRE(capacity == 0 || (ij && shift), "ij and shift buffers required");
   ctx->rec = PairRecord{capacity, ij, shift, terms, count};
   ctx->rec_armed = 1;
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_ctx_freeze(kupsmi_ctx* ctx, int frozen) {
+  KUPSMI_REQUIRE(ctx != nullptr, "null context");
+  ctx->frozen = frozen != 0;
   return KUPSMI_OK;
 }
 

@@ -36,6 +36,8 @@ struct kupsmi_verlet {
   // armed by kupsmi_verlet_record_pairs, taken by the next evaluation
   kupsmi::PairRecord rec;
   int rec_armed = 0;
+  // set by kupsmi_verlet_freeze: reallocating the buffers is an error
+  int frozen = 0;
 };
 
 namespace kupsmi {
@@ -603,6 +605,12 @@ extern "C" int kupsmi_verlet_record_pairs(kupsmi_verlet* v, int64_t capacity, in
   return KUPSMI_OK;
 }
 
+extern "C" int kupsmi_verlet_freeze(kupsmi_verlet* v, int frozen) {
+  KUPSMI_REQUIRE(v != nullptr, "null handle");
+  v->frozen = frozen != 0;
+  return KUPSMI_OK;
+}
+
 extern "C" int kupsmi_verlet_invalidate(kupsmi_verlet* v, void* stream) {
   KUPSMI_REQUIRE(v != nullptr, "null handle");
   if (!v->mem || v->n_sys == 0) return KUPSMI_OK;
@@ -651,6 +659,8 @@ extern "C" int kupsmi_pair_forces_verlet(
   if (tiles < 1) tiles = 1;
   if (!v->mem || v->n_sys != n_sys || v->n_atoms != n_atoms || v->cap != max_neighbors ||
       v->tiles < tiles) {
+    KUPSMI_REQUIRE(!v->frozen,
+                   "a frozen Verlet handle cannot be resized (recorded work uses its buffers)");
     if (v->mem) {
       KUPSMI_HIP(hipStreamSynchronize(st));
       KUPSMI_HIP(hipFree(v->mem));

@@ -192,6 +192,8 @@ class EwaldLJPropagator:
         self.max_neighbors = max_neighbors
         self.verlet = None
         self._dev = None
+        # library context of the force calls (None: the device's shared one)
+        self.context = None
 
     def _prep(self, state: MDState):
         dev = state.positions.device
@@ -218,6 +220,7 @@ class EwaldLJPropagator:
 
     def forces(self, state: MDState, e2, elr, vir=None):
         lib = _lib.load()
+        ctx = self.context.handle if self.context is not None else _lib.ctx()
         if self.verlet is not None:
             _lib.check(lib.kupsmi_pair_forces_verlet(
                 self.verlet.handle, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n,
@@ -228,7 +231,7 @@ class EwaldLJPropagator:
                 self.verlet.max_neighbors, 0, _lib.ptr(state.forces), _lib.ptr(e2), _lib.ptr(vir)))
         else:
             _lib.check(lib.kupsmi_pair_forces(
-                _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n,
+                ctx, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n,
                 self._maxn, _lib.ptr(state.positions), _lib.ptr(state.charges),
                 _lib.ptr(state.species), self.params.n_species, _lib.ptr(self._sig),
                 _lib.ptr(self._eps), _lib.ptr(state.cutoff), None, _lib.ptr(self._alpha),
@@ -236,7 +239,7 @@ class EwaldLJPropagator:
                 _lib.ptr(e2), _lib.ptr(vir), None))
         mx, my, mz = self.ewald.grid
         _lib.check(lib.kupsmi_ewald_recip(
-            _lib.ctx(), _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
+            ctx, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
             _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.tril),
             _lib.ptr(self._alpha), _lib.ptr(self._kmax), mx, my, mz, 1, _lib.ptr(state.forces),
             _lib.ptr(elr), _lib.ptr(vir), None))
@@ -327,25 +330,46 @@ class GraphedSteps:
     pair and reciprocal-space forces, Verlet-list maintenance) is recorded;
     the step counter of the random streams is a device int64 that the
     recorded kernels read and advance, set from ``state.step`` before each
-    replay, so replays draw the same noise as eager steps would.  Capture
-    first evaluates the forces once eagerly so every scratch buffer and
-    Verlet list is allocated before recording (the recorded launches
-    allocate nothing).  The observables of the last replay are in ``obs``."""
+    replay, so replays draw the same noise as eager steps would.  The
+    recorded launches use scratch and a Verlet list of their own (a private
+    library context and Verlet handle, frozen after capture) and hold the
+    propagator's parameter tables: no eager call, of this propagator or any
+    other, can reallocate or free memory the graph reads.
+    Capture first evaluates the forces once eagerly with those so they are
+    sized before recording (the recorded launches allocate nothing).  The
+    observables of the last replay are in ``obs``."""
 
     def __init__(self, prop: EwaldLJPropagator, state: MDState, n_steps: int):
+        from ..core.verlet import VerletList
+
         self.prop, self.state, self.n = prop, state, int(n_steps)
         prop._prep(state)
-        f0 = state.forces.clone()
-        prop.evaluate_forces(state)
-        state.forces.copy_(f0)
-        es = prop._self_energy(state)
-        dev = state.positions.device
-        self.step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
-        torch.cuda.synchronize()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self.obs = prop._steps(state, self.n, self.step_dev, es)
-        torch.cuda.synchronize()
+        self.context = _lib.Context()
+        self.verlet = (VerletList(prop.verlet.skin, prop.verlet.max_neighbors)
+                       if prop.verlet is not None else None)
+        saved = prop.context, prop.verlet
+        prop.context, prop.verlet = self.context, self.verlet
+        try:
+            f0 = state.forces.clone()
+            prop.evaluate_forces(state)
+            state.forces.copy_(f0)
+            es = prop._self_energy(state)
+            dev = state.positions.device
+            self.step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self.obs = prop._steps(state, self.n, self.step_dev, es)
+            torch.cuda.synchronize()
+        finally:
+            prop.context, prop.verlet = saved
+        self.context.freeze()
+        if self.verlet is not None:
+            self.verlet.freeze()
+        # the parameter tables the recorded launches read: held here, so a
+        # later _prep of the propagator (another device or state) that
+        # replaces them cannot free memory the graph uses
+        self._tables = (prop._sig, prop._eps, prop._alpha, prop._rc, prop._kmax)
 
     def replay(self) -> EwaldObservables:
         self.step_dev.fill_(int(self.state.step))

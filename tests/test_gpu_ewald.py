@@ -301,11 +301,25 @@ def test_ewald_md_graph_replays_match_eager_steps(gpu, integ):
     pa.evaluate_forces(sa)
     pb.evaluate_forces(sb)
     graph = pb.capture(sb, 3)
-    for _ in range(3):
+    # a larger batch evaluated eagerly with the same propagator and with the
+    # shared context grows their buffers; the graph's own are untouched
+    big = workloads.nacl_melts(ms=[7, 7, 6], seed=10)
+    sbig = make_md_state(big.positions.astype(np.float32), big.masses, big.species, big.sizes,
+                         big.tril, 12.0, time_step_fs=1.0, temperature=1200.0,
+                         charges=big.charges)
+    epbig = EwaldParameters.box(big.n_sys, alpha=0.3, cutoff=12.0, nmax=8)
+    for r in range(3):
         oa = pa(sa, 3)
         ob = graph.replay()
         torch.cuda.synchronize()
         assert float(torch.ones(4096, device=gpu).sum().item()) == 4096.0  # eager work
+        if r == 0:
+            pb.ewald, ep_saved = epbig, pb.ewald
+            pb._dev = None
+            pb.evaluate_forces(sbig)
+            pb.ewald = ep_saved
+            pb._dev = None
+            torch.cuda.synchronize()
         for name in ("positions", "momenta", "forces", "tril"):
             torch.testing.assert_close(getattr(sb, name), getattr(sa, name), rtol=0, atol=0)
         torch.testing.assert_close(ob.potential_energy, oa.potential_energy, rtol=0, atol=0)
@@ -313,6 +327,41 @@ def test_ewald_md_graph_replays_match_eager_steps(gpu, integ):
         assert sb.step == sa.step
     if integ == "npt":
         assert not torch.equal(sb.tril, torch.as_tensor(b.tril, device=gpu))
+
+
+def test_frozen_context_and_verlet_handle_refuse_to_grow(gpu):
+    """A frozen context (the one a captured graph records with) fails a call
+    that needs more scratch instead of freeing buffers the graph holds; so
+    does a frozen Verlet handle asked to resize."""
+    from kups_amd import _lib
+    from kups_amd.core.verlet import VerletList
+
+    small = workloads.nacl_melts(ms=[3], seed=1)
+    big = workloads.nacl_melts(ms=[5, 5], seed=1)
+    ctx = _lib.Context()
+    prop = EwaldLJPropagator(NACL_LJ, EwaldParameters.box(1, alpha=0.3, cutoff=12.0, nmax=4),
+                             VELOCITY_VERLET, skin=1.0)
+    prop.context = ctx
+
+    def st(b):
+        return make_md_state(b.positions.astype(np.float32), b.masses, b.species, b.sizes, b.tril,
+                             12.0, time_step_fs=1.0, temperature=300.0, charges=b.charges)
+
+    e_small = prop.evaluate_forces(st(small))[0].clone()
+    ctx.freeze()
+    prop.verlet.freeze()
+    np.testing.assert_array_equal(prop.evaluate_forces(st(small))[0].cpu().numpy(),
+                                  e_small.cpu().numpy())  # same size: no growth needed
+    prop.ewald = EwaldParameters.box(2, alpha=0.3, cutoff=12.0, nmax=4)
+    prop._dev = None
+    with pytest.raises(_lib.KupsmiError, match="frozen"):
+        prop.evaluate_forces(st(big))
+    prop.verlet.freeze(False)
+    with pytest.raises(_lib.KupsmiError, match="frozen context"):
+        prop.evaluate_forces(st(big))
+    ctx.freeze(False)
+    prop.evaluate_forces(st(big))
+    torch.cuda.synchronize()
 
 
 def test_pair_tail_corrected_lj_vs_oracle(gpu):
