@@ -368,6 +368,25 @@ int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
                        double* structure_factor);
 
 /*
+ * kupsmi_ewald_recip without a context: the same kernels and results, with
+ * the scratch in a file-scope device array of 64 MiB (one per GPU, held
+ * until the process ends) instead of context memory, so the call allocates
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Limit: the scratch a call needs,
+ * n_sys x nk x (24 + 16 ceil(max_atoms_per_sys / 512)) bytes with
+ * nk = (mx + 1)(2 my + 1)(2 mz + 1) (less 16 n_sys nk when structure_factor
+ * is given), must fit in 64 MiB, else KUPSMI_ERR_ARG with a message.  Every
+ * call on one GPU uses that GPU's single array: calls that overlap on a GPU
+ * (other streams, other threads) race; serialise them.
+ */
+int kupsmi_ewald_recip_noalloc(void* stream, int n_sys, const int32_t* sys_off, int n_atoms,
+                               int max_atoms_per_sys, const float* pos, const float* charges,
+                               const double* tril, const double* alpha, const double* kmax,
+                               int mx, int my, int mz, int accumulate, float* forces,
+                               double* energy, double* virial, double* structure_factor);
+
+/*
  * Elementwise integrator phases of the HBM-resident MD path
  * (md/integrators.py:162-402): phase 0 = B(dt/2) A(dt) (integrator 0) or
  * B(dt/2) A(dt/2) O A(dt/2) (integrator 1) and the fractional wrap; phase 1 =

@@ -17,6 +17,8 @@
 //                     virial, and S'(k) = P(k) S(k) for the force pass.
 //   ek_force_kernel   F_i = -2 K q_i sum_k k Im(S'(k) exp(-i k.r_i)): one thread
 //                     per atom, S' of its system resident in LDS.
+#include <mutex>
+#include <string>
 #include "common.hpp"
 
 namespace kupsmi {
@@ -441,13 +443,45 @@ __global__ __launch_bounds__(EK_THREADS) void ek_force_kernel(EkArgs a) {
 
 using namespace kupsmi;
 
-extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
-                                  int n_atoms, int max_atoms_per_sys, const float* pos,
-                                  const float* charges, const double* tril, const double* alpha,
-                                  const double* kmax, int mx, int my, int mz, int accumulate,
-                                  float* forces, double* energy, double* virial,
-                                  double* structure_factor) {
-  KUPSMI_REQUIRE(ctx != nullptr, "null context");
+// Scratch of kupsmi_ewald_recip_noalloc: a file-scope device array (every
+// GPU the library runs on holds its own copy until the process ends), found
+// per device with hipGetSymbolAddress on the device's first call and used
+// through that address.  Every call on one GPU shares the array: calls that
+// overlap on a GPU race.
+constexpr size_t EK_STATIC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[EK_STATIC_SCRATCH_BYTES];
+
+namespace kupsmi {
+
+constexpr int EK_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[EK_MAX_DEVICES] = {};
+
+// The device address of this device's copy of `scratch` (looked up once per
+// device, with the device current).
+static int static_scratch(void** out) {
+  int dev = 0;
+  KUPSMI_HIP(hipGetDevice(&dev));
+  KUPSMI_REQUIRE(dev >= 0 && dev < EK_MAX_DEVICES, "device index beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    KUPSMI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    KUPSMI_REQUIRE(p != nullptr, "no device address for the static scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return KUPSMI_OK;
+}
+
+// Both entry points: sizes, then the scratch of the context (ctx) or the
+// static array (ctx == nullptr; sizes beyond it are rejected), then the
+// launches.
+static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
+                       int n_atoms, int max_atoms_per_sys, const float* pos,
+                       const float* charges, const double* tril, const double* alpha,
+                       const double* kmax, int mx, int my, int mz, int accumulate,
+                       float* forces, double* energy, double* virial, double* structure_factor) {
   KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0 && max_atoms_per_sys >= 0, "negative size");
   KUPSMI_REQUIRE(mx >= 0 && my >= 0 && mz >= 0 && mx <= EK_NM && my <= EK_NM && mz <= EK_NM,
                  "k grid half widths must be in [0, 12]");
@@ -467,8 +501,22 @@ extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   const size_t sk_bytes = structure_factor ? 0 : ((sizeof(double) * 2 * nsk + 255) & ~(size_t)255);
   const size_t skp_bytes = (sizeof(float2) * nsk + 255) & ~(size_t)255;
   const size_t part_bytes = sizeof(double) * 2 * nsk * (size_t)a.chunks;
-  int rc = scratch(ctx, 4, sk_bytes + skp_bytes + part_bytes, &buf);
-  if (rc) return rc;
+  const size_t need = sk_bytes + skp_bytes + part_bytes;
+  if (ctx) {
+    int rc = scratch(ctx, 4, need, &buf);
+    if (rc) return rc;
+  } else {
+    if (need > EK_STATIC_SCRATCH_BYTES) {
+      set_error("kupsmi_ewald_recip_noalloc: the call needs " + std::to_string(need) +
+                " bytes of scratch (n_sys x k vectors x (24 + 16 x ceil(max_atoms_per_sys / " +
+                std::to_string(EK_CHUNK) + ")) bytes), more than the " +
+                std::to_string(EK_STATIC_SCRATCH_BYTES) + " of its static array: " +
+                "split the batch or use kupsmi_ewald_recip");
+      return KUPSMI_ERR_ARG;
+    }
+    int rc = static_scratch(&buf);
+    if (rc) return rc;
+  }
   char* b = static_cast<char*>(buf);
   a.sk = structure_factor ? structure_factor : reinterpret_cast<double*>(b);
   a.skp = reinterpret_cast<float2*>(b + sk_bytes);
@@ -487,4 +535,29 @@ extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, cons
     KUPSMI_CHECK_LAUNCH();
   }
   return KUPSMI_OK;
+}
+
+}  // namespace kupsmi
+
+extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
+                                  int n_atoms, int max_atoms_per_sys, const float* pos,
+                                  const float* charges, const double* tril, const double* alpha,
+                                  const double* kmax, int mx, int my, int mz, int accumulate,
+                                  float* forces, double* energy, double* virial,
+                                  double* structure_factor) {
+  KUPSMI_REQUIRE(ctx != nullptr, "null context");
+  return kupsmi::ewald_recip(ctx, stream, n_sys, sys_off, n_atoms, max_atoms_per_sys, pos,
+                             charges, tril, alpha, kmax, mx, my, mz, accumulate, forces, energy,
+                             virial, structure_factor);
+}
+
+extern "C" int kupsmi_ewald_recip_noalloc(void* stream, int n_sys, const int32_t* sys_off,
+                                          int n_atoms, int max_atoms_per_sys, const float* pos,
+                                          const float* charges, const double* tril,
+                                          const double* alpha, const double* kmax, int mx, int my,
+                                          int mz, int accumulate, float* forces, double* energy,
+                                          double* virial, double* structure_factor) {
+  return kupsmi::ewald_recip(nullptr, stream, n_sys, sys_off, n_atoms, max_atoms_per_sys, pos,
+                             charges, tril, alpha, kmax, mx, my, mz, accumulate, forces, energy,
+                             virial, structure_factor);
 }
