@@ -1,31 +1,28 @@
-"""Benchmark: batched LJ molecular dynamics and learned-proposal GCMC on one MI355X.
+"""Benchmark on one MI355X: batched LJ+Ewald MD (the headline), batched LJ MD
+and learned-proposal GCMC.
 
-Workload (the batched-MD configuration of the reference path): 2048
-independent cubic PBC boxes, atom counts N ~ U{64..4096} (seed 0, ~4.26 M
-atoms), FCC sites at density 0.8 sigma^-3 with 0.05 sigma Gaussian jitter,
-argon LJ (sigma 3.405 A, eps/kB 119.8 K, cutoff 2.5 sigma, hard, no shift),
-Maxwell-Boltzmann momenta at 94.4 K, NVE velocity Verlet, dt = 2 fs, FP32
-positions/forces with FP64 per-system energies, neighbor set rebuilt every
-step.  A "step" is one MD step of every system; all timed steps run in one
-persistent launch (``--steps-per-launch`` splits them).
+Headline (``value``, ``ms_per_step``, ``config``, ``roofline``,
+``cpu_baseline``): the charged-melt configuration, 512 NaCl-like melts of
+N = 8 m^3 ions (m ~ U{5..10}, seed 0, ~2.1 M ions), rock-salt a = 5.64 A with
+0.1 A jitter, charges +-1, LJ sigma 2.5 / 4.4 A (eps 0.01 eV), real-space
+cutoff 12 A, Ewald alpha = 0.3 / A, k grid |n| <= 8 per axis (2600 nonzero
+k), BAOAB Langevin at 1200 K (gamma 1/ps, dt 1 fs); real space over a Verlet
+list (skin autotuned), k-space structure factors on the matrix cores.  A
+"step" is one MD step of every melt (one force evaluation).  ``--steps`` /
+``--warmup`` apply to it.
 
-Charged-melt leg (``lj_ewald_md`` object; ``--no-ewald`` skips it): 512
-NaCl-like melts of N = 8 m^3 ions (m ~ U{5..10}, seed 0, ~2.1 M ions),
-rock-salt a = 5.64 A with 0.1 A jitter, charges +-1, LJ sigma 2.5 / 4.4 A
-(eps 0.01 eV), real-space cutoff 12 A, Ewald alpha = 0.3 / A, k grid
-|n| <= 8 per axis (2600 nonzero k), BAOAB Langevin at 1200 K (gamma 1/ps,
-dt 1 fs); real space over a Verlet list with a 1 A skin, k-space structure
-factors on the matrix cores.
+``batched_lj_md`` (``--no-lj`` skips it): 2048 independent cubic PBC boxes,
+N ~ U{64..4096} (seed 0, ~4.36 M atoms), FCC sites at density 0.8 sigma^-3
+with 0.05 sigma jitter, argon LJ (cutoff 2.5 sigma), Maxwell-Boltzmann
+momenta at 94.4 K, NVE velocity Verlet, dt = 2 fs; all ``--lj-steps`` steps in
+one persistent launch.
 
-GCMC leg (``gcmc`` object of the line; ``--no-gcmc`` skips it): the
-learned-proposal workload, 4096 rigid hosts of 1500-3000 atoms (jittered cubic
-lattice, 30 % vacancies, 25-30 A triclinic cells, charges N(0, 0.3)
-neutralised), CO2 at 298 K and 0.1-10 bar, LJ + Ewald.  10^6 insertion samples
-from the plain kernel train the MLP proposal on the GPU; after equilibration
-with it, the uniform and the learned chains run the same number of trials from
-the same state (same random stream), and the line reports trials/s, accepted
-moves/s and accepted insertions/s of both, their time-averaged loadings with a
-paired z score, and the tracked-energy error against a full re-evaluation.
+``gcmc`` (``--no-gcmc`` skips it): the learned-proposal workload, 4096 rigid
+hosts of 1500-3000 atoms (jittered cubic lattice, 30 % vacancies, 25-30 A
+triclinic cells, charges N(0, 0.3) neutralised), CO2 at 298 K and 0.1-10 bar,
+LJ + Ewald; 10^6 insertion samples from the plain kernel train the MLP
+proposal on the GPU; after equilibration with it, the uniform and the
+learned chains run the same number of trials from the same state.
 
 Prints one JSON line (rank 0, N = 1 only).
 """
@@ -45,6 +42,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "atom-steps/s (batched LJ+Ewald MD) + GCMC accepted moves/s, learned vs plain"
+# algorithmic work per unit (conventions of the hot-path measurement table)
 # Algorithmic work per atom-step of this workload: 30 FLOP per unique in-cutoff
 # pair (26.18 unique pairs per atom at rho* = 0.8, rc = 2.5 sigma) + 12 FLOP of
 # integration; 48 B of FP32 position and momentum read + write.
@@ -57,9 +55,13 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--steps-per-launch", type=int, default=0, help="0: all timed steps in one launch")
+    ap.add_argument("--steps", type=int, default=30, help="timed LJ+Ewald steps (headline)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed LJ+Ewald steps")
+    ap.add_argument("--no-lj", action="store_true")
+    ap.add_argument("--lj-steps", type=int, default=500)
+    ap.add_argument("--lj-warmup", type=int, default=20)
+    ap.add_argument("--steps-per-launch", type=int, default=0,
+                    help="LJ leg: 0 runs all timed steps in one launch")
     ap.add_argument("--n-sys", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-every", type=int, default=8,
@@ -68,11 +70,10 @@ def parse():
     ap.add_argument("--skin-sigma", type=float, default=None,
                     help="Verlet skin of the MD leg in sigma (default: autotuned; the traffic "
                          "probe defaults to 0.3)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="start with a 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="1 GiB device copy (counter calibration) + one MD launch, no JSON line")
-    ap.add_argument("--no-ewald", action="store_true")
-    ap.add_argument("--ewald-steps", type=int, default=30)
-    ap.add_argument("--ewald-warmup", type=int, default=5)
     ap.add_argument("--ewald-cpu-systems", type=int, default=16)
     ap.add_argument("--no-gcmc", action="store_true")
     ap.add_argument("--gcmc-chains", type=int, default=4096)
@@ -121,6 +122,7 @@ def cpu_baseline(batch, params, args):
         "sample": (f"{len(idx)} of {batch.n_sys} systems (every {args.cpu_sample_every}th, N>=100), "
                    f"{atoms} atoms x {args.cpu_steps} FP32 VV steps, cell list rebuilt every step, "
                    f"OpenMP over systems, {el:.1f} s; CPU restatement of kUPS, not kUPS"),
+        "cpu": cpu_info(),
     }
 
 
@@ -155,13 +157,50 @@ def ewald_cpu_baseline(b, args):
     oracle.ewald_self(q, off, al)
     el = time.perf_counter() - t0
     return {
-        "value": n / el, "unit": "ion-steps/s", "cores": oracle.num_threads(), "kind": "port",
-        "sample": f"first {k} of {b.n_sys} melts ({n} ions), one FP64 force evaluation with a "
-                  f"dense neighbor list, {el:.1f} s; CPU restatement of kUPS, not kUPS",
+        "value": n / el, "unit": "atom-steps/s", "cores": oracle.num_threads(), "kind": "port",
+        "sample": f"first {k} of {b.n_sys} melts ({n} ions), one FP64 force evaluation (LJ over "
+                  f"a cell list, real-space Ewald over the dense neighbor list the reference's "
+                  f"applications use, k-space over 2600 k, self term), {el:.1f} s; the "
+                  f"integrator is negligible beside it; CPU restatement of kUPS, not kUPS",
+        "cpu": cpu_info(),
     }
 
 
+# real-space part of an ion-step: 45 FLOP per unique pair within 12 A (161
+# per ion at the rock-salt density 8 / 5.64^3 A^-3): the algorithmic work of
+# vl_pair_kernel, the dominant kernel of the step
+FLOP_PAIR_PER_ION_STEP = 45.0 * 161
+
+
+def cpu_info() -> dict:
+    """Host CPU model and thread counts (lscpu when present)."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Thread(s) per core", "Socket(s)", "Core(s) per socket"):
+                info[k.lower().replace(" ", "_").replace("(s)", "s")] = v
+    except Exception:
+        pass
+    return info
+
+
+def load_profile(name: str) -> dict | None:
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text())
+    except Exception:
+        return None
+
+
 def ewald_leg(args):
+    """The headline: LJ + Ewald MD of the charged melts."""
     import torch
 
     from kups_amd import workloads
@@ -182,30 +221,52 @@ def ewald_leg(args):
     tuned = tune_ewald_skin(lj, ep, st, BAOAB_LANGEVIN)
     prop = EwaldLJPropagator(lj, ep, BAOAB_LANGEVIN, skin=tuned.best)
     prop.evaluate_forces(st)
-    if args.ewald_warmup:
-        prop(st, args.ewald_warmup)
+    if args.warmup:
+        prop(st, args.warmup)
     torch.cuda.synchronize()
+    prop.verlet.profile(True)  # HIP events around the main pair pass of every step
     t0 = time.perf_counter()
-    obs = prop(st, args.ewald_steps)
+    obs = prop(st, args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    pair_ms, pair_calls = prop.verlet.profile(False)
     ions = int(b.sizes.sum())
     rebuilds, overflow = prop.verlet.counters()
-    value = ions * args.ewald_steps / wall
-    achieved = FLOP_PER_ION_STEP * value / 1e12
+    value = ions * args.steps / wall
+    pair_avg_ms = pair_ms / max(pair_calls, 1)
+    achieved = FLOP_PAIR_PER_ION_STEP * ions / (pair_avg_ms * 1e-3) / 1e12
+    step_tflops = FLOP_PER_ION_STEP * value / 1e12
+    traffic = load_profile("vl_pair_kernel_traffic.json")
     res = {
-        "metric": "ion-steps/s (batched LJ+Ewald MD)", "value": value, "unit": "ion-steps/s",
-        "ms_per_step": wall * 1e3 / args.ewald_steps, "steps": args.ewald_steps,
-        "warmup": args.ewald_warmup, "dtype": "fp32 pairs and k sums (fp64 accumulation)",
-        "workload": f"LJ+Ewald melts: {b.description}, rc 12 A, alpha 0.3, |n| <= 8, BAOAB 1200 K, "
-                    f"dt 1 fs, Verlet skin {tuned.best:g} A (autotuned)",
-        "n_ions": ions,
+        "value": value, "ms_per_step": wall * 1e3 / args.steps,
+        "config": {
+            "workload": f"LJ+Ewald MD: {b.description}, rock-salt a=5.64 A, jitter 0.1 A, "
+                        "charges +-1, LJ sigma 2.5/4.4 A eps 0.01 eV (LB mixing), rc 12 A, "
+                        "Ewald alpha 0.3/A, |n| <= 8 per axis (2600 k), BAOAB 1200 K, "
+                        f"gamma 1/ps, dt 1 fs, Verlet skin {tuned.best:g} A (autotuned), the "
+                        "exact cutoff pair set every step",
+            "n_systems": b.n_sys, "n_ions": ions, "seed": 0,
+        },
+        "roofline": {
+            "bound": "valu", "kernel": "vl_pair_kernel (main pass)",
+            "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
+            "kernel_ms_per_launch": pair_avg_ms, "launches_timed": pair_calls,
+            "note": f"{FLOP_PAIR_PER_ION_STEP:.0f} algorithmic FLOP per ion per launch (45 per "
+                    "unique real-space pair, 161 pairs); kernel time from HIP events around each "
+                    "launch on the library's stream; peak = FP32 vector peak (its packed-FP32 "
+                    "half is not used)",
+        },
+        "step_roofline": {
+            "achieved": step_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": step_tflops / PEAK_FP32_TFLOPS,
+            "note": f"{FLOP_PER_ION_STEP:.0f} algorithmic FLOP per ion-step (real space + "
+                    "2600 k vectors), whole step",
+        },
         "autotune": {"skin_A": tuned.best, "ms_per_step": tuned.table, "cached": tuned.cached},
         "verlet": {"system_rebuilds_total": rebuilds, "overflowed_lists": overflow},
         "final_potential_energy_eV": float(obs.potential_energy[:, -1].sum().item()),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "note": f"{FLOP_PER_ION_STEP:.0f} algorithmic FLOP per ion-step (whole step)"},
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = ewald_cpu_baseline(b, args)
@@ -213,6 +274,15 @@ def ewald_leg(args):
     torch.cuda.empty_cache()
     return res
 
+
+# GCMC trial work (the hot-path measurement table): per site evaluation (a
+# guest site of the old or the new configuration) 45 FLOP per host atom
+# within 12 A plus 16 FLOP per k vector (structure-factor change); 6 FLOP per
+# k vector per trial for the long-range energy change
+GCMC_FLOP_PER_PAIR = 45.0
+GCMC_FLOP_PER_SITE_K = 16.0
+GCMC_FLOP_PER_TRIAL_K = 6.0
+SPHERE_12A = 4.0 / 3.0 * np.pi * 12.0**3
 
 # MLP 32 -> 128 -> 128 -> 128 -> 1: multiply-adds per descriptor row
 MLP_FLOP_PER_ROW = 2.0 * (32 * 128 + 128 * 128 + 128 * 128 + 128)
@@ -236,6 +306,7 @@ def gcmc_cpu_baseline(g, snap, args):
         "sample": f"{len(idx)} of {g.n_chain} equilibrated chains x {args.gcmc_cpu_steps} trials, "
                   f"uniform insertion, FP64, OpenMP over chains, {el:.1f} s; CPU restatement of "
                   "kUPS, not kUPS",
+        "cpu": cpu_info(),
     }
 
 
@@ -292,14 +363,27 @@ def gcmc_leg(args):
         sync()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
+        kev = []  # events around each gcmc_kernel launch
         e0.record(stream)
         for _ in range(nchunk):
+            a0 = torch.cuda.Event(enable_timing=True)
+            a1 = torch.cuda.Event(enable_timing=True)
+            a0.record(stream)
             g.run(args.gcmc_chunk, proposal=prop)
+            a1.record(stream)
+            kev.append((a0, a1))
             acc += g.loading
         e1.record(stream)
         sync()
         el = e0.elapsed_time(e1) * 1e-3
+        kern_s = sum(a.elapsed_time(b) for a, b in kev) * 1e-3
         s1 = g.stats
+        work = s1["work_trials"] - s0["work_trials"]
+        sites = s1["site_evaluations"] - s0["site_evaluations"]
+        rho = g.sizes / g.volume
+        flop = float(np.sum(sites * (rho * SPHERE_12A * GCMC_FLOP_PER_PAIR +
+                                     g.k_counts * GCMC_FLOP_PER_SITE_K) +
+                            work * g.k_counts * GCMC_FLOP_PER_TRIAL_K))
         trials = int((s1["trials"] - s0["trials"]).sum())
         moved = int((s1["accepted"] - s0["accepted"]).sum())
         ins = int((s1["insertions"] - s0["insertions"]).sum())
@@ -317,6 +401,19 @@ def gcmc_leg(args):
             "loading_sem": float(loads[name].std(ddof=1) / np.sqrt(g.n_chain)),
             "energy_error_max_eV": float(err.max()),
             "energy_error_max_rel": float((err / np.maximum(energy, 1.0)).max()),
+            "work_trials": int(work.sum()),
+            "noop_trials": trials - int(work.sum()),
+            "work_trials_per_s": int(work.sum()) / el,
+            "roofline": {
+                "bound": "valu", "kernel": "gcmc_kernel",
+                "achieved": flop / kern_s / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": flop / kern_s / 1e12 / PEAK_FP32_TFLOPS, "traffic": None,
+                "kernel_ms_per_launch": kern_s * 1e3 / len(kev),
+                "note": "algorithmic FLOP of the trials that evaluated sites (45 per host "
+                        "atom within 12 A and 16 per k vector per site evaluation, 6 per k "
+                        "vector per trial); trials on empty chains that evaluate nothing "
+                        "(noop_trials) count no work; host LDS-resident",
+            },
         }
     d = loads["learned"] - loads["uniform"]
     z = float(d.mean() / (d.std(ddof=1) / np.sqrt(len(d)))) if d.std() > 0 else 0.0
@@ -350,16 +447,108 @@ def gcmc_leg(args):
 
 
 def load_traffic(skin_sigma: float):
-    """Measured HBM bytes of the 500-step launch at this skin (the profile
+    """Measured HBM bytes of the LJ leg's launch at this skin (the profile
     holds one entry per autotuner candidate), or None."""
-    f = ROOT / "profiles" / "md_lj_kernel_traffic.json"
-    if not f.exists():
-        return None
-    try:
-        data = json.loads(f.read_text())
-    except Exception:
+    data = load_profile("md_lj_kernel_traffic.json")
+    if data is None:
         return None
     return data.get("by_skin_sigma", {}).get(f"{skin_sigma:.2f}")
+
+
+def lj_leg(args, traffic_probe=False):
+    """Batched LJ MD of 2048 boxes, all timed steps in one persistent launch."""
+    import torch
+
+    from kups_amd import workloads
+    from kups_amd.core.autotune import tune_md_skin
+    from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
+    from kups_amd.md.integrators import make_velocity_verlet_step
+    from kups_amd.potential.lennard_jones import LennardJonesParameters
+
+    params = LennardJonesParameters.from_dict(
+        workloads.AR_CUTOFF, {"Ar": (workloads.AR_SIGMA, workloads.AR_EPS)})
+    batch = workloads.batched_lj_boxes(n_sys=args.n_sys, seed=0)
+    p0 = maxwell_boltzmann_momenta(batch.masses, batch.sizes, 94.4, np.random.default_rng(0))
+    st = make_md_state(batch.positions.astype(np.float32), batch.masses, batch.species,
+                       batch.sizes, batch.tril, workloads.AR_CUTOFF, time_step_fs=2.0,
+                       temperature=94.4, momenta=p0.astype(np.float32))
+    prop = make_velocity_verlet_step(params)
+    prop.evaluate_forces(st)
+    tuned = None
+    if args.skin_sigma is not None or traffic_probe:
+        skin_sigma = 0.3 if args.skin_sigma is None else args.skin_sigma
+    else:
+        # skin chosen by measurement (a few hundred steps on this state,
+        # restored afterwards)
+        tuned = tune_md_skin(params, st)
+        skin_sigma = tuned.best / workloads.AR_SIGMA
+    prop = make_velocity_verlet_step(params, skin=skin_sigma * workloads.AR_SIGMA)
+    steps = args.lj_steps
+    if traffic_probe:
+        prop(st, steps, record=True)
+        torch.cuda.synchronize()
+        print(f"traffic probe: {int(batch.sizes.sum())} atoms, {steps} steps in one launch, "
+              f"skin {skin_sigma:.2f} sigma", flush=True)
+        return None
+    if args.lj_warmup:
+        prop(st, args.lj_warmup, record=False)
+    torch.cuda.synchronize()
+    spl = args.steps_per_launch or steps
+    launches = []
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(spl, steps - done)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        obs = prop(st, k, record=True)
+        ev1.record(stream)
+        launches.append((ev0, ev1, k))
+        done += k
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    atoms = int(batch.sizes.sum())
+    value = atoms * steps / wall
+    # per-launch kernel time (events bracket the order + geometry kernels too,
+    # which take a few microseconds)
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in launches]))
+    steps_per_launch = float(np.mean([k for _, _, k in launches]))
+    achieved_tflops = FLOP_PER_ATOM_STEP * atoms * steps_per_launch / (kern_ms * 1e-3) / 1e12
+    etot = (obs.potential_energy[:, -1] + obs.kinetic_energy[:, -1]).double().sum().item()
+    traffic = load_traffic(skin_sigma) if spl == 500 else None
+    res = {
+        "metric": "atom-steps/s (batched LJ MD)", "value": value, "unit": "atom-steps/s",
+        "steps": steps, "warmup": args.lj_warmup, "ms_per_step": wall * 1e3 / steps,
+        "workload": "batched LJ MD: 2048 cubic PBC boxes, N~U{64..4096} (seed 0), rho*=0.8, "
+                    "jitter 0.05 sigma, argon LJ rc=2.5 sigma, NVE velocity Verlet dt=2 fs, the "
+                    "exact cutoff pair set every step (Verlet lists, "
+                    f"skin {skin_sigma:.2f} sigma{' autotuned' if tuned else ''}, rebuilt on the "
+                    "skin/2 displacement bound)",
+        "n_systems": batch.n_sys, "n_atoms": atoms, "steps_per_launch": spl,
+        "roofline": {
+            "bound": "valu", "kernel": "md_lj_kernel",
+            "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+            "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
+            "kernel_ms_per_launch": kern_ms,
+            "note": f"{FLOP_PER_ATOM_STEP:.0f} algorithmic FLOP per atom-step; state "
+                    "LDS-resident, Verlet lists in HBM; traffic from counters of a 500-step "
+                    "launch (null for other launch lengths)",
+        },
+        "final_total_energy_eV": etot,
+    }
+    if tuned is not None:
+        res["autotune"] = {"skin_sigma": skin_sigma,
+                           "ms_per_step": {f"{k / workloads.AR_SIGMA:.2f}": v
+                                           for k, v in tuned.table.items()},
+                           "cached": tuned.cached}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(batch, params, args)
+    del st, prop
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -371,15 +560,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from kups_amd import workloads
-    from kups_amd.md.data import make_md_state, maxwell_boltzmann_momenta
-    from kups_amd.core.autotune import tune_md_skin
-    from kups_amd.md.integrators import make_velocity_verlet_step
-    from kups_amd.potential.lennard_jones import LennardJonesParameters
-
-    params = LennardJonesParameters.from_dict(
-        workloads.AR_CUTOFF, {"Ar": (workloads.AR_SIGMA, workloads.AR_EPS)})
-    if args.traffic_probe:
+    if args.traffic_probe or args.copy_calibration:
         # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
         # calibrate the counters against 1 GiB read + 1 GiB written
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -387,107 +568,32 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    batch = workloads.batched_lj_boxes(n_sys=args.n_sys, seed=0)
-    p0 = maxwell_boltzmann_momenta(batch.masses, batch.sizes, 94.4, np.random.default_rng(0))
-    st = make_md_state(batch.positions.astype(np.float32), batch.masses, batch.species,
-                       batch.sizes, batch.tril, workloads.AR_CUTOFF, time_step_fs=2.0,
-                       temperature=94.4, momenta=p0.astype(np.float32))
-    prop = make_velocity_verlet_step(params)
-    prop.evaluate_forces(st)
-    tuned = None
-    if args.skin_sigma is not None or args.traffic_probe:
-        skin_sigma = 0.3 if args.skin_sigma is None else args.skin_sigma
-    else:
-        # skin chosen by measurement (a few hundred steps on this state,
-        # restored afterwards)
-        tuned = tune_md_skin(params, st)
-        skin_sigma = tuned.best / workloads.AR_SIGMA
-    prop = make_velocity_verlet_step(params, skin=skin_sigma * workloads.AR_SIGMA)
     if args.traffic_probe:
-        prop(st, args.steps, record=True)
-        torch.cuda.synchronize()
-        print(f"traffic probe: {int(batch.sizes.sum())} atoms, {args.steps} steps in one launch, "
-              f"skin {skin_sigma:.2f} sigma",
-              flush=True)
+        lj_leg(args, traffic_probe=True)
         return
-    if args.warmup:
-        prop(st, args.warmup, record=False)
-    torch.cuda.synchronize()
-    spl = args.steps_per_launch or args.steps
-    launches = []
-    stream = torch.cuda.current_stream()
-    t0 = time.perf_counter()
-    done = 0
-    while done < args.steps:
-        k = min(spl, args.steps - done)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        ev0.record(stream)
-        obs = prop(st, k, record=True)
-        ev1.record(stream)
-        launches.append((ev0, ev1, k))
-        done += k
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    atoms = int(batch.sizes.sum())
-    value = atoms * args.steps / wall
-    # per-launch kernel time (events bracket the order + geometry kernels too,
-    # which take a few microseconds)
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in launches]))
-    steps_per_launch = float(np.mean([k for _, _, k in launches]))
-    achieved_tflops = FLOP_PER_ATOM_STEP * atoms * steps_per_launch / (kern_ms * 1e-3) / 1e12
-    etot = (obs.potential_energy[:, -1] + obs.kinetic_energy[:, -1]).double().sum().item()
-    traffic = load_traffic(skin_sigma)
+    head = ewald_leg(args)
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "atom-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (fp64 per-system energies)",
-        "data": "synthetic (seeded FCC boxes with Gaussian jitter, Maxwell-Boltzmann momenta)",
-        "config": {
-            "workload": "batched LJ MD: 2048 cubic PBC boxes, N~U{64..4096} (seed 0), "
-                        "rho*=0.8, jitter 0.05 sigma, argon LJ rc=2.5 sigma, NVE velocity "
-                        "Verlet dt=2 fs, the exact cutoff pair set every step (Verlet lists, "
-                        f"skin {skin_sigma:.2f} sigma{' autotuned' if tuned else ''}, rebuilt "
-                        "on the skin/2 displacement bound)",
-            "n_systems": batch.n_sys,
-            "n_atoms": atoms,
-            "steps_per_launch": spl,
-        },
-        "roofline": {
-            "bound": "mfma",
-            "achieved": achieved_tflops,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
-            "kernel": "md_lj_kernel",
-            "note": "FP32 FLOP ceiling (157.3 TF/s is the FP32 peak of both the vector "
-                    "ALUs and the matrix cores; this kernel is VALU work, state LDS-resident, "
-                    "Verlet lists in HBM; "
-                    f"{FLOP_PER_ATOM_STEP:.0f} algorithmic FLOP per atom-step)",
-            "kernel_ms_per_launch": kern_ms,
-        },
-        "final_total_energy_eV": etot,
+        "dtype": "fp32 (pairs, k-space MFMA; fp64 energies, S(k) and self terms)",
+        "data": "synthetic (seeded jittered rock-salt melts, Maxwell-Boltzmann momenta)",
+        "config": head["config"],
+        "roofline": head["roofline"],
     }
-    if tuned is not None:
-        line["autotune"] = {"skin_sigma": skin_sigma,
-                            "ms_per_step": {f"{k / workloads.AR_SIGMA:.2f}": v
-                                            for k, v in tuned.table.items()},
-                            "cached": tuned.cached}
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(batch, params, args)
-    del st, prop
-    torch.cuda.empty_cache()
-    if not args.no_ewald:
-        line["lj_ewald_md"] = ewald_leg(args)
+    if "cpu_baseline" in head:
+        line["cpu_baseline"] = head["cpu_baseline"]
+    for k in ("step_roofline", "autotune", "verlet", "final_potential_energy_eV"):
+        line[k] = head[k]
+    if not args.no_lj:
+        line["batched_lj_md"] = lj_leg(args)
     if not args.no_gcmc:
         line["gcmc"] = gcmc_leg(args)
     print(json.dumps(line), flush=True)

@@ -280,6 +280,16 @@ int kupsmi_normals(void* stream, uint64_t seed, int64_t step, int n_atoms,
                    float* eta);
 
 /*
+ * The bare Philox4x32-10 generator that every random stream of the library
+ * draws from: out[i] = Philox4x32-10(counters[4 i..], keys[2 i..]) for
+ * n independent (counter, key) pairs.  For known-answer checks of the
+ * shared stream (Salmon et al. SC'11; Random123's published vectors).  No
+ * counterpart in the reference (it draws from jax.random).
+ */
+int kupsmi_philox(void* stream, int n, const uint32_t* counters, const uint32_t* keys,
+                  uint32_t* out);
+
+/*
  * LJ + Ewald real-space energies, forces and virials of systems of any size
  * (<= 16384 atoms each), state in HBM.  Replaces, for one evaluation, the LJ
  * energy (potential/classical/lennard_jones.py:149-178) and the Ewald
@@ -337,6 +347,12 @@ int kupsmi_verlet_counters(kupsmi_verlet* v, void* stream, unsigned long long* o
  * atoms, list entries or a larger system) fails instead of reallocating
  * them; for handles used by recorded work. */
 int kupsmi_verlet_freeze(kupsmi_verlet* v, int frozen);
+/* Kernel timing of the main pair pass (the kernel that evaluates the list):
+ * returns in *total_ms / *calls the summed HIP-event time and the number of
+ * calls timed since the previous kupsmi_verlet_profile (synchronising on
+ * their events), then enables (enable != 0) or disables the timing of later
+ * calls.  No counterpart in the reference; for measurement. */
+int kupsmi_verlet_profile(kupsmi_verlet* v, int enable, double* total_ms, long long* calls);
 /* Arms the handle's next kupsmi_pair_forces_verlet to record its pair set,
  * as kupsmi_ctx_record_pairs does for the context's calls. */
 int kupsmi_verlet_record_pairs(kupsmi_verlet* v, int64_t capacity, int32_t* ij, int32_t* shift,
@@ -453,9 +469,11 @@ int kupsmi_step_advance(void* stream, int64_t* step_dev, int64_t n);
  *   move_weights   HOST array[4]: unnormalised weights of translation, rotation,
  *                  reinsertion, exchange (default 1, 1, 1, 3 in kUPS's mcmc app)
  *   width_upper    HOST array[4]: upper bounds of the adapted step widths
- *   stats          [n_chain x 10] int64: trials[4], accepted[4] (moves that changed the
+ *   stats          [n_chain x 12] int64: trials[4], accepted[4] (moves that changed the
  *                  configuration; a move of an empty chain is accepted but not counted),
- *                  insertions, deletions
+ *                  insertions, deletions, trials that evaluated any site (a displacement or
+ *                  deletion of an empty chain evaluates none), site evaluations (sites of
+ *                  the old and the new configuration summed over trials)
  *   trace          optional [n_trace x n_steps x 4] f64 (move, dE, log p, accepted)
  * Random numbers: Philox4x32-10 keyed by seed, counters (chain, step, tag); the
  * first trial's step counter is step0, or *step_dev when step_dev is not NULL

@@ -28,6 +28,7 @@ _SIGNATURES = {
     "kupsmi_ctx_destroy": ([vp], C.c_int),
     "kupsmi_ctx_freeze": ([vp, i32], C.c_int),
     "kupsmi_verlet_freeze": ([vp, i32], C.c_int),
+    "kupsmi_verlet_profile": ([vp, i32, vp, vp], C.c_int),
     "kupsmi_neighborlist": ([vp, vp, i32, vp, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
     "kupsmi_neighborlist_cells": (
         [vp, vp, i32, vp, i32, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
@@ -46,6 +47,7 @@ _SIGNATURES = {
         C.c_int,
     ),
     "kupsmi_normals": ([vp, u64, i64, i32, vp], C.c_int),
+    "kupsmi_philox": ([vp, i32, vp, vp, vp], C.c_int),
     "kupsmi_pair_forces": (
         [vp, vp, i32, vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp,
          vp, vp, vp],

@@ -48,6 +48,16 @@ class VerletList:
         (for handles used by recorded work)."""
         _lib.check(_lib.load().kupsmi_verlet_freeze(self._h, int(frozen)))
 
+    def profile(self, enable: bool = True) -> tuple[float, int]:
+        """(summed HIP-event milliseconds, calls) of the main pair pass since
+        the previous call (synchronises on those events), then turns the
+        timing of later calls on or off."""
+        ms = C.c_double()
+        n = C.c_longlong()
+        _lib.check(_lib.load().kupsmi_verlet_profile(self._h, int(enable), C.byref(ms),
+                                                     C.byref(n)))
+        return ms.value, n.value
+
     def invalidate(self):
         _lib.check(_lib.load().kupsmi_verlet_invalidate(self._h, _lib.stream_ptr()))
 

@@ -35,6 +35,10 @@
 
 namespace kupsmi {
 
+// per-chain statistics: trials[4], accepted[4], insertions, deletions, trials
+// that evaluated sites, site evaluations
+constexpr int GC_NSTATS = 12;
+
 constexpr int GC_THREADS = 512;
 constexpr int GC_WAVES = GC_THREADS / 64;
 constexpr int GC_MSITE = 4;      // sites per molecule
@@ -196,7 +200,7 @@ struct GcChain {
   uint32_t hist[16];
   int hist_idx[4];
   int cnt[GC_NSPEC];
-  long long stats[10];
+  long long stats[GC_NSTATS];
 };
 
 // The trial formed by lane 0 and read by the workgroup.
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         cs.hist_idx[k] = a.hist_idx[(size_t)ch * 4 + k];
       }
       for (int k = 0; k < 16; ++k) cs.hist[k] = a.hist[(size_t)ch * 16 + k];
-      for (int k = 0; k < 10; ++k) cs.stats[k] = a.stats[(size_t)ch * 10 + k];
+      for (int k = 0; k < GC_NSTATS; ++k) cs.stats[k] = a.stats[(size_t)ch * GC_NSTATS + k];
     }
     for (int j = tid; j < nh; j += GC_THREADS) {
       host[j] = a.host[h0 + j];
@@ -560,6 +564,12 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
           trc[1] = dE; trc[2] = logp; trc[3] = accept;
         }
         cs.stats[move] += 1;
+        // trials that evaluated any site, and the site evaluations (old and
+        // new configurations) they made: the work behind the trial count
+        if (nsite) {
+          cs.stats[10] += 1;
+          cs.stats[11] += nsite;
+        }
         if (accept) {
           if (nsite) cs.stats[4 + move] += 1;  // moves that changed the configuration
           energy += dE;
@@ -631,7 +641,7 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         a.hist_idx[(size_t)ch * 4 + k] = cs.hist_idx[k];
       }
       for (int k = 0; k < 16; ++k) a.hist[(size_t)ch * 16 + k] = cs.hist[k];
-      for (int k = 0; k < 10; ++k) a.stats[(size_t)ch * 10 + k] = cs.stats[k];
+      for (int k = 0; k < GC_NSTATS; ++k) a.stats[(size_t)ch * GC_NSTATS + k] = cs.stats[k];
     }
   }
 }

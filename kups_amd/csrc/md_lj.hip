@@ -931,6 +931,14 @@ __global__ void fill_ones_zeros(float* o, float* z, int n) {
   if (i < 3 * n) z[i] = 0.0f;
 }
 
+__global__ void philox_kernel(int n, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c[4] = {ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3]};
+  philox4x32_10(c, key[2 * i], key[2 * i + 1]);
+  for (int k = 0; k < 4; ++k) out[4 * i + k] = c[k];
+}
+
 __global__ void normals_kernel(uint64_t seed, int64_t step, int n, float* eta) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1067,6 +1075,16 @@ extern "C" int kupsmi_normals(void* stream, uint64_t seed, int64_t step, int n_a
   KUPSMI_REQUIRE(n_atoms >= 0, "n_atoms < 0");
   if (n_atoms == 0) return KUPSMI_OK;
   normals_kernel<<<(n_atoms + 255) / 256, 256, 0, (hipStream_t)stream>>>(seed, step, n_atoms, eta);
+  KUPSMI_CHECK_LAUNCH();
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_philox(void* stream, int n, const uint32_t* counters, const uint32_t* keys,
+                             uint32_t* out) {
+  KUPSMI_REQUIRE(n >= 0, "n < 0");
+  if (n == 0) return KUPSMI_OK;
+  KUPSMI_REQUIRE(counters && keys && out, "null buffer");
+  philox_kernel<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, counters, keys, out);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }

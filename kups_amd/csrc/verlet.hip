@@ -26,6 +26,8 @@
 // Positions, cells (tril), charges, species, exclusion ids and sizes must stay
 // fixed between rebuilds except for the positions; kupsmi_verlet_invalidate
 // forces a rebuild of every system.
+#include <utility>
+#include <vector>
 #include "pair_common.hpp"
 
 struct kupsmi_verlet {
@@ -38,6 +40,9 @@ struct kupsmi_verlet {
   int rec_armed = 0;
   // set by kupsmi_verlet_freeze: reallocating the buffers is an error
   int frozen = 0;
+  // kupsmi_verlet_profile: HIP events around the main pair pass of each call
+  int profile = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 };
 
 namespace kupsmi {
@@ -582,6 +587,10 @@ extern "C" int kupsmi_verlet_create(kupsmi_verlet** out) {
 
 extern "C" int kupsmi_verlet_destroy(kupsmi_verlet* v) {
   if (!v) return KUPSMI_OK;
+  for (auto& e : v->events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
   if (v->mem) {
     int cur = 0;
     KUPSMI_HIP(hipGetDevice(&cur));
@@ -591,6 +600,27 @@ extern "C" int kupsmi_verlet_destroy(kupsmi_verlet* v) {
     KUPSMI_HIP(e);
   }
   delete v;
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_verlet_profile(kupsmi_verlet* v, int enable, double* total_ms,
+                                     long long* calls) {
+  KUPSMI_REQUIRE(v != nullptr, "null handle");
+  double ms = 0.0;
+  long long n = 0;
+  for (auto& e : v->events) {
+    KUPSMI_HIP(hipEventSynchronize(e.second));
+    float t = 0.f;
+    KUPSMI_HIP(hipEventElapsedTime(&t, e.first, e.second));
+    ms += t;
+    ++n;
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  v->events.clear();
+  if (total_ms) *total_ms = ms;
+  if (calls) *calls = n;
+  v->profile = enable != 0;
   return KUPSMI_OK;
 }
 
@@ -741,7 +771,17 @@ extern "C" int kupsmi_pair_forces_verlet(
     else vl_pair_kernel<false, false, false, P><<<grid, VL_TILE, 0, st>>>(a);      \
   }                                                                                \
   KUPSMI_CHECK_LAUNCH();
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (v->profile) {
+    KUPSMI_HIP(hipEventCreate(&ev0));
+    KUPSMI_HIP(hipEventCreate(&ev1));
+    KUPSMI_HIP(hipEventRecord(ev0, st));
+  }
   VL_PASS(0)
+  if (v->profile) {
+    KUPSMI_HIP(hipEventRecord(ev1, st));
+    v->events.emplace_back(ev0, ev1);
+  }
   VL_PASS(1)
 #undef VL_PASS
   vl_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);

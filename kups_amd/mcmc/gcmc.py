@@ -141,6 +141,7 @@ class RigidGCMC:
             pk[s, :len(k)] = P
             gidx[s, :len(k)] = (k[:, 0] * w + (k[:, 1] + self.nm)) * w + (k[:, 2] + self.nm)
         self.nk = nk
+        self.k_counts = np.array([len(k) for k in kl])  # k vectors of each chain
         self._kgrid_index = gidx
         # species: host types then guest types
         self.n_species = lj.n_species
@@ -175,7 +176,7 @@ class RigidGCMC:
             width=torch.full((n, 4), 0.1, dtype=f32, device=dev),
             hist=torch.zeros((n, 4, 4), dtype=torch.int32, device=dev),
             hist_idx=torch.zeros((n, 4), dtype=i32, device=dev),
-            stats=torch.zeros((n, 10), dtype=torch.int64, device=dev),
+            stats=torch.zeros((n, 12), dtype=torch.int64, device=dev),
         )
         # move weights and width bounds are host arrays of the entry point
         self._w_host = np.ascontiguousarray(self.weights, dtype=np.float64)
@@ -497,7 +498,8 @@ class RigidGCMC:
     @property
     def stats(self) -> dict:
         s = self.d["stats"].cpu().numpy()
-        return {"trials": s[:, :4], "accepted": s[:, 4:8], "insertions": s[:, 8], "deletions": s[:, 9]}
+        return {"trials": s[:, :4], "accepted": s[:, 4:8], "insertions": s[:, 8],
+                "deletions": s[:, 9], "work_trials": s[:, 10], "site_evaluations": s[:, 11]}
 
 
 class GraphedTrials:

@@ -834,6 +834,11 @@ static void gc_draw(uint64_t seed, uint32_t a, int64_t t, uint32_t tag, uint32_t
 void orc_philox(uint64_t seed, uint32_t a, int64_t t, uint32_t tag, uint32_t* out) {
   gc_draw(seed, a, t, tag, out);
 }
+/* The bare generator: out = Philox4x32-10(counter ctr[4], key key[2]). */
+void orc_philox_raw(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  for (int k = 0; k < 4; ++k) out[k] = ctr[k];
+  philox10(out, key[0], key[1]);
+}
 
 /* ------------------------------------------------------------- CSVR ---- */
 /* Stochastic velocity rescaling (md/integrators.py:470-585, Bussi, Donadio

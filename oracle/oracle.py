@@ -407,6 +407,14 @@ def gcmc_state(g, chains=None) -> dict:
         grid=(nm, nm, nm), w_move=g.weights, upper=g.upper)
 
 
+def philox_raw(ctr, key) -> np.ndarray:
+    """Philox4x32-10 of counter ctr[4] under key key[2] (the bare generator)."""
+    c, k = np.ascontiguousarray(ctr, np.uint32), np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox_raw(_p(c), _p(k), _p(out))
+    return out
+
+
 def philox(seed: int, a: int, t: int, tag: int) -> np.ndarray:
     """The four Philox4x32-10 words of counter (a, t, t >> 32, tag)."""
     out = np.zeros(4, np.uint32)
