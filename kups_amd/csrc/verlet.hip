@@ -192,7 +192,13 @@ __global__ __launch_bounds__(PC_BIN_THREADS) void vl_bin_kernel(VlArgs a) {
 }
 
 // Lists of the rebuilt systems: one lane per sorted atom appends every
-// candidate of the 125-cell stencil within rc + skin (FP64 test).
+// candidate of the 125-cell stencil within rc + skin.  The test is FP32 with
+// a relative margin of 1e-5 on (rc + skin)^2: the list only has to hold
+// every pair that can come within rc before the next rebuild (a few pairs
+// just beyond rc + skin cost nothing), the pair pass decides the cutoff.
+// Entries are written in groups of four (one 16-byte store per group, groups
+// entry-major over the atoms); the last group is padded with the atom's own
+// slot, masked off by the count.
 __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
   __shared__ SysGeom g;
   const int s = blockIdx.y;
@@ -204,15 +210,25 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
   const int slot = blockIdx.x * VL_TILE + threadIdx.x;
   if (slot >= n) return;
   const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
-  const double rl = g.rc;  // rc + skin
-  const double rl2 = rl * rl * (1.0 + 1e-9);
+  const float rl = (float)g.rc;  // rc + skin
+  const float rl2 = rl * rl * (1.0f + 1e-5f);
   const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
   const int d = i0 + slot;
   const float4 ri = a.spos[d];
   a.sref[d] = ri;
   const int c = a.scell[d];
   const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+  uint4* list4 = reinterpret_cast<uint4*>(a.list);
+  const size_t stride = (size_t)a.n_atoms;
+  const int cap = a.cap;  // a multiple of four
+  // the group being filled, as a shift register
+  uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
   int cnt = 0;
+  auto push = [&](uint32_t e) {
+    b0 = b1; b1 = b2; b2 = b3; b3 = e;
+    if ((cnt & 3) == 3 && cnt < cap) list4[(size_t)(cnt >> 2) * stride + d] = make_uint4(b0, b1, b2, b3);
+    ++cnt;
+  };
   for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
     int zz = cz + dz;
     const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
@@ -225,9 +241,9 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
         int xx = cx + dx;
         const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
         xx -= sx * ncx;
-        double Sx, Sy, Sz;
-        tmul(g.L, (double)sx, (double)sy, (double)sz, Sx, Sy, Sz);
-        const double qx = (double)ri.x - Sx, qy = (double)ri.y - Sy, qz = (double)ri.z - Sz;
+        float Sx, Sy, Sz;
+        shift_vec32(g.Lf, sx, sy, sz, Sx, Sy, Sz);
+        const float qx = ri.x - Sx, qy = ri.y - Sy, qz = ri.z - Sz;
         const bool center = dx == 0 && dy == 0 && dz == 0;
         const bool packable = sx >= -VL_SHIFT_MAX && sx <= VL_SHIFT_MAX && sy >= -VL_SHIFT_MAX &&
                               sy <= VL_SHIFT_MAX && sz >= -VL_SHIFT_MAX && sz <= VL_SHIFT_MAX;
@@ -236,15 +252,20 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
         for (int t = tb; t < te; ++t) {
           if (center && t == slot) continue;
           const float4 rj = a.spos[i0 + t];
-          const double ex = (double)rj.x - qx, ey = (double)rj.y - qy, ez = (double)rj.z - qz;
-          if (!(ex * ex + ey * ey + ez * ez < rl2)) continue;
-          if (!packable) cnt = a.cap + 1;  // cannot be stored: use the stencil
-          if (cnt < a.cap)
-            a.list[(size_t)cnt * a.n_atoms + d] = vl_pack(t, sx, sy, sz, a.sspec[i0 + t]);
-          ++cnt;
+          const float ex = rj.x - qx, ey = rj.y - qy, ez = rj.z - qz;
+          if (!(norm2_32(ex, ey, ez) < rl2)) continue;
+          if (!packable) cnt = cap + 1;  // cannot be stored: use the stencil
+          push(vl_pack(t, sx, sy, sz, a.sspec[i0 + t]));
         }
       }
     }
+  }
+  // pad and store the last, partial group
+  if ((cnt & 3) && cnt < cap) {
+    const uint32_t pad = vl_pack(slot, 0, 0, 0, a.sspec[d]);
+    const int full = cnt;
+    while (cnt & 3) push(pad);
+    cnt = full;
   }
   a.nlist[d] = cnt;
   if (cnt > a.cap) atomicAdd(a.counters + 1, 1ull);
@@ -287,6 +308,8 @@ __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)
   mid = 0;
   if (EXCL && g.rep && ((pp.has_lj_excl && xlj_j == xlj_i) || (pp.has_coul_excl && xcl_j == xcl_i)))
     min_img = band_min_image(g, tj, sx, sy, sz);
+  // one reciprocal square root serves both terms
+  const float ir0 = __builtin_amdgcn_rsqf(r2);
   if (pp.has_lj) {
     const bool lo = r2 < band[0];
     bool m = !lo && r2 < band[1];
@@ -298,7 +321,8 @@ __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)
     kept |= keep ? 1 : 0;
     mid |= m ? 1 : 0;
     const int p = si * pp.ns + sj;
-    const float ir2 = keep ? __builtin_amdgcn_rcpf(r2) : 0.0f;
+    const float ir = keep ? ir0 : 0.0f;
+    const float ir2 = ir * ir;
     float c6 = pp.sig2[p] * ir2;
     c6 = c6 * c6 * c6;
     elj += pp.e4[p] * c6 * (c6 - 1.0f);
@@ -314,13 +338,14 @@ __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)
     if (EXCL && pp.has_coul_excl && min_img && xcl_j == xcl_i) keep = false;
     kept |= keep ? 2 : 0;
     mid |= m ? 2 : 0;
-    const float ir = keep ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-    const float x = pp.alpha * (r2 * ir);
-    const float ec = erfc_cheb(x);
+    const float ir = keep ? ir0 : 0.0f;
+    const float x = pp.alpha * (r2 * ir0);  // alpha r
+    const float ex = __expf(-x * x);
+    const float ec = erfcx_fit(x) * ex;  // erfc(alpha r)
     const float qq = ri.w * rj.w * K;
     ecl += qq * ec * ir;
     // dU/dr = qq (-2a/sqrt(pi) e^{-a^2 r^2} - erfc / r) / r; u'/r = dU/dr / r
-    gsum += qq * (-pp.two_a_sqrtpi * __expf(-x * x) - ec * ir) * ir * ir;
+    gsum += qq * (-pp.two_a_sqrtpi * ex - ec * ir0) * ir * ir0;
   }
   return gsum;
 }
@@ -416,35 +441,35 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
         accumulate(gsum, ex, ey, ez);
       }
     } else if (cnt <= a.cap) {
-      // four entries per trip; while group q is evaluated, the positions of
-      // group q + 1 and the list words of group q + 2 are in flight (indices
-      // past the end are clamped to the last entry and masked off)
-      const uint32_t* lp = a.list + d;
+      // one group of four entries per trip (one 16-byte load); while group q
+      // is evaluated, the positions of group q + 1 and the list words of
+      // group q + 2 are in flight (groups past the end are clamped to the
+      // last one; entries past the count are masked off)
+      const uint4* lp = reinterpret_cast<const uint4*>(a.list) + d;
       const size_t stride = (size_t)a.n_atoms;
-      const int last = cnt - 1;
-      auto words = [&](int k0, uint32_t (&e)[4]) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = lp[(size_t)min(k0 + u, last) * stride];
+      const int ng = (cnt + 3) >> 2;
+      auto group = [&](int q) -> uint4 { return lp[(size_t)min(q, ng - 1) * stride]; };
+      auto positions = [&](uint4 e, float4 (&r)[4]) {
+        r[0] = a.spos[i0 + (int)(e.x & 0x7fffu)];
+        r[1] = a.spos[i0 + (int)(e.y & 0x7fffu)];
+        r[2] = a.spos[i0 + (int)(e.z & 0x7fffu)];
+        r[3] = a.spos[i0 + (int)(e.w & 0x7fffu)];
       };
-      auto positions = [&](const uint32_t (&e)[4], float4 (&r)[4]) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) r[u] = a.spos[i0 + (int)(e[u] & 0x7fffu)];
-      };
-      uint32_t ea[4], eb[4];
+      uint4 ga = make_uint4(0, 0, 0, 0), gb = ga;
       float4 ra[4];
       if (cnt > 0) {
-        words(0, ea);
-        words(4, eb);
-        positions(ea, ra);
+        ga = group(0);
+        gb = group(1);
+        positions(ga, ra);
       }
-      for (int k0 = 0; k0 < cnt; k0 += 4) {
+      for (int q = 0; q < ng; ++q) {
         float4 rb[4];
-        uint32_t ec[4];
-        positions(eb, rb);
-        words(k0 + 8, ec);
+        positions(gb, rb);
+        const uint4 gc = group(q + 2);
+        const uint32_t ew[4] = {ga.x, ga.y, ga.z, ga.w};
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const uint32_t e = ea[u];
+          const uint32_t e = ew[u];
           const float4 rj = ra[u];
           const int t = (int)(e & 0x7fffu);
           const int sx = (int)((e >> 15) & 15u) - 8, sy = (int)((e >> 19) & 15u) - 8,
@@ -454,7 +479,7 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
           const float ex = (rj.x + Sx) - ri.x, ey = (rj.y + Sy) - ri.y, ez = (rj.z + Sz) - ri.z;
           // a masked-off entry gets a huge finite r^2 (above every cutoff
           // band; times the zero 1/r of a rejected pair it stays 0)
-          const float r2 = k0 + u < cnt ? norm2_32(ex, ey, ez) : 3.0e38f;
+          const float r2 = 4 * q + u < cnt ? norm2_32(ex, ey, ez) : 3.0e38f;
           int kept, mid;
           const float gsum = vl_entry_terms<EXCL, PASS>(
               pp, band, g, t, ri, rj, sx, sy, sz, r2, si, (int)(e >> 27),
@@ -470,11 +495,9 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
           accumulate(gsum, ex, ey, ez);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          ea[u] = eb[u];
-          eb[u] = ec[u];
-          ra[u] = rb[u];
-        }
+        for (int u = 0; u < 4; ++u) ra[u] = rb[u];
+        ga = gb;
+        gb = gc;
       }
     } else {
       // overflowed list: the build's 125-cell stencil (cells sized for rc + skin)
@@ -687,7 +710,8 @@ extern "C" int kupsmi_pair_forces_verlet(
   hipStream_t st = (hipStream_t)stream;
   int tiles = (max_atoms_per_sys + VL_TILE - 1) / VL_TILE;
   if (tiles < 1) tiles = 1;
-  if (!v->mem || v->n_sys != n_sys || v->n_atoms != n_atoms || v->cap != max_neighbors ||
+  const int cap4 = (max_neighbors + 3) & ~3;  // lists are stored in groups of four
+  if (!v->mem || v->n_sys != n_sys || v->n_atoms != n_atoms || v->cap != cap4 ||
       v->tiles < tiles) {
     KUPSMI_REQUIRE(!v->frozen,
                    "a frozen Verlet handle cannot be resized (recorded work uses its buffers)");
@@ -696,11 +720,11 @@ extern "C" int kupsmi_pair_forces_verlet(
       KUPSMI_HIP(hipFree(v->mem));
       v->mem = nullptr;
     }
-    VlLayout lay(n_sys, n_atoms, max_neighbors, tiles);
+    VlLayout lay(n_sys, n_atoms, cap4, tiles);
     void* p = nullptr;
     KUPSMI_HIP(hipMalloc(&p, lay.total));
     v->mem = p; v->bytes = lay.total;
-    v->n_sys = n_sys; v->n_atoms = n_atoms; v->cap = max_neighbors; v->tiles = tiles;
+    v->n_sys = n_sys; v->n_atoms = n_atoms; v->cap = cap4; v->tiles = tiles;
     char* b = static_cast<char*>(p);
     vl_invalidate_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(n_sys, reinterpret_cast<int*>(b + lay.valid));
     KUPSMI_CHECK_LAUNCH();
