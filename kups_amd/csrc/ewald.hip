@@ -10,8 +10,9 @@
 //   ek_sk_kernel      S(k) = sum_i q_i exp(i k.r_i) as an FP32 GEMM on the
 //                     matrix cores (v_mfma_f32_16x16x4_f32), one workgroup
 //                     per (system, 512-atom chunk), atoms streamed through LDS
-//                     in tiles of 8..64 (what 64 KB holds) with their per-axis
-//                     phase tables, operands k-contiguous for float4 fragments;
+//                     in tiles of up to 128 as per-axis phase planes that
+//                     both GEMM operands are read from; S(-n2) comes from
+//                     the conjugate symmetry of the z factor;
 //                     ek_sk_reduce_kernel sums the chunks in fixed order.
 //   ek_energy_kernel  P(k) (FP64), E = K sum_k P |S|^2, the reciprocal-space
 //                     virial, and S'(k) = P(k) S(k) for the force pass.
@@ -68,81 +69,56 @@ __device__ inline double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// c u^n for n = 0..m into out[0], out[st], .., out[m st] (FP64 recurrence,
-// rounded once).
-__device__ inline void powers64_pos(double2 u, double c, int m, float2* out, int st = 1) {
-  double2 p = make_double2(c, 0.0);
-  for (int n = 0; n <= m; ++n) {
-    out[n * st] = make_float2((float)p.x, (float)p.y);
-    p = zmul(p, u);
-  }
-}
-
-// u^n for n = -m..m into out[0..2m] with stride st (n = 0 at out[m st];
-// FP64, rounded once).
-__device__ inline void powers64(double2 u, int m, float2* out, int st = 1) {
-  out[m * st] = make_float2(1.f, 0.f);
-  double2 p = make_double2(1.0, 0.0);
-  for (int n = 1; n <= m; ++n) {
-    p = zmul(p, u);
-    out[(m + n) * st] = make_float2((float)p.x, (float)p.y);
-    out[(m - n) * st] = make_float2((float)p.x, (float)-p.y);
-  }
-}
-
-// Powers u^n for n = -m..m into out[0..2m] (n = 0 at out[m]).
-__device__ inline void powers(float2 u, int m, float2* out) {
-  out[m] = make_float2(1.f, 0.f);
-  float2 p = make_float2(1.f, 0.f);
-  for (int n = 1; n <= m; ++n) {
-    p = cmul(p, u);
-    out[m + n] = p;
-    out[m - n] = make_float2(p.x, -p.y);
-  }
-}
-
-// S(k) as a real FP32 GEMM on the matrix cores.  With m = (n0, n1)
-// (M = (mx+1)(2my+1) rows) and A_i(m) = q_i X_i(n0) Y_i(n1):
-//   S(m, n2) = sum_i A_i(m) Z_i(n2),
-// stacked over (atom, re/im) on K = 2 x atoms and (re, im) of S on N = 2 wz:
-//   S_re = sum Re A Re Z - Im A Im Z,  S_im = sum Re A Im Z + Im A Re Z.
-// One workgroup per (system, chunk of EK_CHUNK atoms).  Per tile of TA atoms
-// the workgroup builds the per-axis phase tables (FP64, rounded once; stored
-// n-major so that two atoms are one 16-byte word), then packs both GEMM
-// operands k-contiguous into LDS: A as [m][k] and B as [column][k], row pitch
-// ldk = 2 TA + 4 floats (a quad count that is odd, so sixteen rows at the same
-// k land on sixteen disjoint quads of banks).  K is summed in blocks of 16:
-// lane (r = l % 16, q = l / 16) loads one float4 of A row r and one of B column
-// r at k = k0 + 4q .. k0 + 4q + 3, and component j feeds the j-th
-// v_mfma_f32_16x16x4_f32 of the block (each MFMA sums k0 + 4q + j over q; A and
-// B use the same k order, so the four MFMAs together sum the whole block).
-// D of an MFMA: lane l holds row 4 (l / 16) + r, column l % 16.  The 4 waves
-// each own every 4th 16-row tile of M and all N tiles; their FP32 sums are
-// added into FP64 registers after at most 64 atoms (K = 128), and the chunk's
-// partial S goes to HBM for a fixed-order reduce.
+// S(k) as a real FP32 GEMM on the matrix cores, using the z-axis symmetry
+// Z(-n2) = conj Z(n2).  Rows (m, a): m = (n0, n1) over [0..mx] x [-my..my],
+// a = re / im of A_i(m) = q_i X_i(n0) Y_i(n1); columns (n2, b): n2 in
+// [0..mz], b = re / im of Z_i(n2); K = atoms:
+//   C[(m,a)][(n2,b)] = sum_i A_ia(m) Z_ib(n2)
+//   S(m, +n2) = (C_rr - C_ii) + i (C_ri + C_ir)
+//   S(m, -n2) = (C_rr + C_ii) + i (C_ir - C_ri)
+// One workgroup per (system, chunk of EK_CHUNK atoms, group of row tiles).
+// Per tile of TA atoms the workgroup writes the per-axis phase factors (FP64
+// recurrences, rounded once) to LDS as float planes [n][atom] with a row pitch
+// of TA + 4 floats; the GEMM operands are then read straight from the planes:
+// the A value of row (m, a) is xr yr - xi yi (a = re) or xr yi + xi yr
+// (a = im), one multiply and one FMA from planes chosen per lane (yr / -yi or
+// yi / yr), so neither operand is packed.  K runs in blocks of 16 atoms: lane
+// (r = l % 16, q = l / 16) reads float4s of atoms k0 + 4q .. k0 + 4q + 3 and
+// component j feeds the j-th v_mfma_f32_16x16x4_f32 (each MFMA sums
+// k0 + 4q + j over q; the four together cover the block); the MFMAs of one j
+// over all (row, column) tiles issue back to back, so no accumulator waits
+// on the previous MFMA.  D of an MFMA: lane l holds rows 4 (l / 16) + r,
+// column l % 16, so rows (m, re) and (m, im) sit in one lane and columns
+// (n2, re) and (n2, im) in lanes l and l ^ 1: one exchange gives the even
+// lane S(m, +n2) and the odd lane S(m, -n2).  A chunk's sums stay in the
+// FP32 accumulators (at most 512 unit-modulus terms times q each); its
+// partial S goes to HBM in FP64 for a fixed-order reduce.
 constexpr int EK_SK_THREADS = 256;
 constexpr int EK_WAVES = EK_SK_THREADS / 64;
 constexpr int EK_CHUNK = 512;
+constexpr int EK_MTW_MAX = 5;  // row tiles per wave
 
 struct EkSkShape {
-  int ta, ldk;  // atoms per tile (multiple of 8), panel row pitch (floats)
-  size_t lds;   // dynamic LDS bytes
+  int ta, pitch;  // atoms per tile (multiple of 16), plane row pitch (floats)
+  int mtw, groups, nt;
+  size_t lds;     // dynamic LDS bytes
 };
 
 template <int MTW, int NT>
 __global__ __launch_bounds__(EK_SK_THREADS) void ek_sk_kernel(EkArgs a, EkSkShape sh) {
   extern __shared__ float ek_lds[];
   __shared__ double s_li[6];
-  const int TA = sh.ta, ldk = sh.ldk, TH = TA / 2;
-  const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1, wx = a.mx + 1;
-  const int M = wx * wy, NC = 2 * wz;
-  const int mt_n = (M + 15) / 16;
-  float* apk = ek_lds;                                   // [16 mt_n][ldk]
-  float* bpk = apk + (size_t)16 * mt_n * ldk;            // [16 NT][ldk]
-  float2* tx = reinterpret_cast<float2*>(bpk + (size_t)16 * NT * ldk);  // [wx][TA]
-  float2* ty = tx + (size_t)wx * TA;                     // [wy][TA]
-  float2* tz = ty + (size_t)wy * TA;                     // [wz][TA]
-  const int s = blockIdx.y, chunk = blockIdx.x;
+  const int TA = sh.ta, P = sh.pitch;
+  const int wx = a.mx + 1, wy = 2 * a.my + 1, wz = 2 * a.mz + 1, wzh = a.mz + 1;
+  const int M = wx * wy;
+  float* xr = ek_lds;                  // [wx][P]  Re q X(n0)
+  float* xi = xr + (size_t)wx * P;     // [wx][P]  Im q X(n0)
+  float* yr = xi + (size_t)wx * P;     // [wy][P]  Re Y(n1), n1 = -my..my
+  float* yi = yr + (size_t)wy * P;     // [wy][P]  Im Y
+  float* nyi = yi + (size_t)wy * P;    // [wy][P]  -Im Y
+  float* zr = nyi + (size_t)wy * P;    // [wzh][P] Re Z(n2), n2 = 0..mz
+  float* zi = zr + (size_t)wzh * P;    // [wzh][P] Im Z
+  const int s = blockIdx.y, chunk = blockIdx.x, grp = blockIdx.z;
   const int i0 = a.sys_off[s], n_all = a.sys_off[s + 1] - i0;
   const int c0 = chunk * EK_CHUNK;
   if (c0 >= n_all) return;
@@ -150,28 +126,38 @@ __global__ __launch_bounds__(EK_SK_THREADS) void ek_sk_kernel(EkArgs a, EkSkShap
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int row = lane & 15, kq = lane >> 4;
   if (tid == 0) tril_inverse(a.tril + 6 * s, s_li);
-  // zero the panels once: padding rows / columns are never written again
-  for (int e = tid; e < 16 * (mt_n + NT) * ldk; e += EK_SK_THREADS) apk[e] = 0.f;
-  double sum[MTW][NT][4];
+  // operand planes of this lane: row tile T = (grp EK_WAVES + wave) MTW + t
+  // holds rows 16 T .. 16 T + 15; rows past 2 M read row M - 1 (discarded)
+  // (float offsets into the LDS planes)
+  int pa[MTW], pb[MTW], pc[MTW];
+#pragma unroll
+  for (int t = 0; t < MTW; ++t) {
+    const int R = 16 * ((grp * EK_WAVES + wave) * MTW + t) + row;
+    const int m = min(R >> 1, M - 1), im = R & 1;
+    const int n0 = m / wy, j1 = m - n0 * wy;
+    pa[t] = n0 * P;
+    pb[t] = (int)((im ? yi : yr) - ek_lds) + j1 * P;
+    pc[t] = (int)((im ? yr : nyi) - ek_lds) + j1 * P;
+  }
+  int pz[NT];
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const int col = 16 * u + row;
+    const int n2 = min(col >> 1, a.mz);
+    pz[u] = (int)(((col & 1) ? zi : zr) - ek_lds) + n2 * P;
+  }
+  const int xoff = wx * P;  // xr -> xi
   f32x4 acc[MTW][NT];
 #pragma unroll
   for (int t = 0; t < MTW; ++t)
 #pragma unroll
-    for (int u = 0; u < NT; ++u) {
-      acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sum[t][u][r] = 0.0;
-    }
-  // tiles between FP64 flushes: at most 64 atoms in an FP32 partial
-  const int flush_every = TA >= 64 ? 1 : 64 / TA;
-  int since_flush = 0;
+    for (int u = 0; u < NT; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int base = 0; base < n; base += TA) {
     __syncthreads();
     const int cnt = n - base < TA ? n - base : TA;
-    // 1. per-axis phase tables, n-major (q folded into x); atoms past the end
-    //    are zero
+    // 1. per-axis phase planes (q folded into X); atoms past the end are zero
     for (int e = tid; e < 3 * TA; e += EK_SK_THREADS) {
-      const int axis = e / TA, at = e % TA;
+      const int axis = e / TA, at = e - axis * TA;
       if (at < cnt) {
         const float* r = a.pos + 3 * (size_t)(i0 + c0 + base + at);
         double f0, f1, f2;
@@ -180,84 +166,95 @@ __global__ __launch_bounds__(EK_SK_THREADS) void ek_sk_kernel(EkArgs a, EkSkShap
         f -= rint(f);
         double2 u;
         sincospi(2.0 * f, &u.y, &u.x);
-        if (axis == 0)
-          powers64_pos(u, (double)a.q[i0 + c0 + base + at], a.mx, tx + at, TA);
-        else if (axis == 1)
-          powers64(u, a.my, ty + at, TA);
-        else
-          powers64(u, a.mz, tz + at, TA);
+        if (axis == 0) {
+          double2 p = make_double2((double)a.q[i0 + c0 + base + at], 0.0);
+          for (int k = 0; k <= a.mx; ++k) {
+            xr[(size_t)k * P + at] = (float)p.x;
+            xi[(size_t)k * P + at] = (float)p.y;
+            p = zmul(p, u);
+          }
+        } else if (axis == 1) {
+          const int c = a.my;
+          yr[(size_t)c * P + at] = 1.f;
+          yi[(size_t)c * P + at] = 0.f;
+          nyi[(size_t)c * P + at] = 0.f;
+          double2 p = make_double2(1.0, 0.0);
+          for (int k = 1; k <= a.my; ++k) {
+            p = zmul(p, u);
+            const float re = (float)p.x, im = (float)p.y;
+            yr[(size_t)(c + k) * P + at] = re;
+            yi[(size_t)(c + k) * P + at] = im;
+            nyi[(size_t)(c + k) * P + at] = -im;
+            yr[(size_t)(c - k) * P + at] = re;
+            yi[(size_t)(c - k) * P + at] = -im;
+            nyi[(size_t)(c - k) * P + at] = im;
+          }
+        } else {
+          double2 p = make_double2(1.0, 0.0);
+          for (int k = 0; k <= a.mz; ++k) {
+            zr[(size_t)k * P + at] = (float)p.x;
+            zi[(size_t)k * P + at] = (float)p.y;
+            p = zmul(p, u);
+          }
+        }
+      } else if (axis == 0) {
+        for (int k = 0; k <= a.mx; ++k) xr[(size_t)k * P + at] = xi[(size_t)k * P + at] = 0.f;
+      } else if (axis == 1) {
+        for (int k = 0; k < wy; ++k)
+          yr[(size_t)k * P + at] = yi[(size_t)k * P + at] = nyi[(size_t)k * P + at] = 0.f;
       } else {
-        const int w = axis == 0 ? wx : (axis == 1 ? wy : wz);
-        float2* o = axis == 0 ? tx + at : (axis == 1 ? ty + at : tz + at);
-        for (int m = 0; m < w; ++m) o[(size_t)m * TA] = make_float2(0.f, 0.f);
+        for (int k = 0; k <= a.mz; ++k) zr[(size_t)k * P + at] = zi[(size_t)k * P + at] = 0.f;
       }
     }
     __syncthreads();
-    // 2. pack A[m][k] (k = 2 at + re/im), two atoms per item
-    for (int e = tid; e < M * TH; e += EK_SK_THREADS) {
-      const int m = e / TH, p = e - m * TH;
-      const int n0 = m / wy, n1 = m - n0 * wy;
-      const float4 x = *reinterpret_cast<const float4*>(tx + (size_t)n0 * TA + 2 * p);
-      const float4 y = *reinterpret_cast<const float4*>(ty + (size_t)n1 * TA + 2 * p);
-      const float2 A0 = cmul(make_float2(x.x, x.y), make_float2(y.x, y.y));
-      const float2 A1 = cmul(make_float2(x.z, x.w), make_float2(y.z, y.w));
-      *reinterpret_cast<float4*>(apk + (size_t)m * ldk + 4 * p) = make_float4(A0.x, A0.y, A1.x, A1.y);
-    }
-    //    and B[j][k]: j < wz -> Re S (Re Z, -Im Z), else Im S (Im Z, Re Z)
-    for (int e = tid; e < wz * TH; e += EK_SK_THREADS) {
-      const int j = e / TH, p = e - j * TH;
-      const float4 z = *reinterpret_cast<const float4*>(tz + (size_t)j * TA + 2 * p);
-      *reinterpret_cast<float4*>(bpk + (size_t)j * ldk + 4 * p) = make_float4(z.x, -z.y, z.z, -z.w);
-      *reinterpret_cast<float4*>(bpk + (size_t)(wz + j) * ldk + 4 * p) = make_float4(z.y, z.x, z.w, z.z);
-    }
-    __syncthreads();
-    // 3. matrix-core products over this tile's K = 2 TA, 16 k per block
-    const float* ap = apk + (size_t)(wave * 16 + row) * ldk + 4 * kq;
-    const float* bp = bpk + (size_t)row * ldk + 4 * kq;
-    for (int k0 = 0; k0 < 2 * TA; k0 += 16) {
+    // 2. matrix-core products over the tile's atoms, 16 per block
+    for (int k0 = 0; k0 < cnt; k0 += 16) {
+      const int at = k0 + 4 * kq;
       f32x4 bv[NT];
 #pragma unroll
-      for (int u = 0; u < NT; ++u)
-        bv[u] = *reinterpret_cast<const f32x4*>(bp + (size_t)u * 16 * ldk + k0);
+      for (int u = 0; u < NT; ++u) bv[u] = *reinterpret_cast<const f32x4*>(ek_lds + pz[u] + at);
+      f32x4 av[MTW];
 #pragma unroll
       for (int t = 0; t < MTW; ++t) {
-        if (wave + t * EK_WAVES >= mt_n) break;
-        const f32x4 av =
-            *reinterpret_cast<const f32x4*>(ap + (size_t)t * EK_WAVES * 16 * ldk + k0);
+        const f32x4 x0 = *reinterpret_cast<const f32x4*>(ek_lds + pa[t] + at);
+        const f32x4 x1 = *reinterpret_cast<const f32x4*>(ek_lds + pa[t] + xoff + at);
+        const f32x4 y0 = *reinterpret_cast<const f32x4*>(ek_lds + pb[t] + at);
+        const f32x4 y1 = *reinterpret_cast<const f32x4*>(ek_lds + pc[t] + at);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 4; ++j) av[t][j] = fmaf(x1[j], y1[j], x0[j] * y0[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < MTW; ++t)
 #pragma unroll
           for (int u = 0; u < NT; ++u)
-            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j], bv[u][j], acc[t][u], 0, 0, 0);
-      }
-    }
-    if (++since_flush == flush_every || base + TA >= n) {
-      since_flush = 0;
-#pragma unroll
-      for (int t = 0; t < MTW; ++t)
-#pragma unroll
-        for (int u = 0; u < NT; ++u) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) sum[t][u][r] += (double)acc[t][u][r];
-          acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t][j], bv[u][j], acc[t][u], 0, 0, 0);
     }
   }
-  // D[row 4 (l/16) + r][col l % 16] of tile (t, u): m = 16 mt + row, column j
+  // lane l of tile (t, u): acc[r] = C[(m0 + r / 2, r % 2)][(n2, b)] with
+  // m0 = 8 T + 2 (l / 16), n2 = 8 u + (l % 16) / 2, b = l % 2
   double* part = a.skpart + ((size_t)s * a.chunks + chunk) * a.nk * 2;
+  const int odd = lane & 1;
 #pragma unroll
   for (int t = 0; t < MTW; ++t) {
-    const int mt = wave + t * EK_WAVES;
-    if (mt >= mt_n) continue;
+    const int T = (grp * EK_WAVES + wave) * MTW + t;
 #pragma unroll
     for (int u = 0; u < NT; ++u) {
-      const int j = u * 16 + (lane & 15);
+      const int n2 = 8 * u + (row >> 1);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = mt * 16 + (lane >> 4) * 4 + r;
-        if (m >= M || j >= NC) continue;
-        const int k = m * wz + (j < wz ? j : j - wz);
-        part[2 * k + (j < wz ? 0 : 1)] = sum[t][u][r];
+      for (int h = 0; h < 2; ++h) {
+        const float vr = acc[t][u][2 * h], vi = acc[t][u][2 * h + 1];
+        const float wr = __shfl_xor(vr, 1), wi = __shfl_xor(vi, 1);
+        const int m = 8 * T + 2 * kq + h;
+        if (m >= M || n2 > a.mz || (odd && n2 == 0)) continue;
+        // even lane: C_rr = vr, C_ir = vi, C_ri = wr, C_ii = wi -> S(m, +n2)
+        // odd lane:  C_ri = vr, C_ii = vi, C_rr = wr, C_ir = wi -> S(m, -n2)
+        const double re = odd ? (double)wr + vi : (double)vr - wi;
+        const double im = odd ? (double)wi - vr : (double)wr + vi;
+        const int k = m * wz + a.mz + (odd ? -n2 : n2);
+        part[2 * k] = re;
+        part[2 * k + 1] = im;
       }
     }
   }
@@ -280,45 +277,41 @@ __global__ void ek_sk_reduce_kernel(EkArgs a) {
   o[1] = im;
 }
 
-template <int MTW>
-void ek_sk_launch_nt(const EkArgs& a, const EkSkShape& sh, int nt, dim3 g, hipStream_t st) {
-  switch (nt) {
-    case 1: ek_sk_kernel<MTW, 1><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
-    case 2: ek_sk_kernel<MTW, 2><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
-    case 3: ek_sk_kernel<MTW, 3><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
-    default: ek_sk_kernel<MTW, 4><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh); break;
-  }
-}
-
-// Tile size: the largest TA in {64, 56, .., 8} whose panels and tables fit
-// 64 KB (two workgroups per CU); ldk = 2 TA + 4 has an odd quad count for TA
-// a multiple of 4.
+// Tile size: the largest TA in {128, 96, 64, 48, 32, 16} whose planes fit
+// 64 KB; row tiles per wave up to EK_MTW_MAX, in as many groups as needed.
 EkSkShape ek_sk_shape(const EkArgs& a) {
-  const int wx = a.mx + 1, wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
-  const int rows = 16 * ((wx * wy + 15) / 16) + 16 * ((2 * wz + 15) / 16);
+  const int wx = a.mx + 1, wy = 2 * a.my + 1, wzh = a.mz + 1;
+  const int planes = 2 * wx + 3 * wy + 2 * wzh;
   EkSkShape sh{};
-  for (int ta = 64; ta >= 8; ta -= 8) {
-    const int ldk = 2 * ta + 4;
-    const size_t lds = sizeof(float) * (size_t)rows * ldk + sizeof(float2) * (size_t)(wx + wy + wz) * ta;
-    sh.ta = ta;
-    sh.ldk = ldk;
-    sh.lds = lds;
-    if (lds <= 64 * 1024) break;
+  const int tas[6] = {128, 96, 64, 48, 32, 16};
+  for (int c = 0; c < 6; ++c) {
+    sh.ta = tas[c];
+    sh.pitch = sh.ta + 4;
+    sh.lds = sizeof(float) * (size_t)planes * sh.pitch;
+    if (sh.lds <= 64 * 1024) break;
   }
+  const int rt = (2 * wx * wy + 15) / 16;
+  sh.mtw = min(EK_MTW_MAX, (rt + EK_WAVES - 1) / EK_WAVES);
+  sh.groups = (rt + EK_WAVES * sh.mtw - 1) / (EK_WAVES * sh.mtw);
+  sh.nt = (2 * wzh + 15) / 16;
   return sh;
 }
 
-void ek_sk_launch(const EkArgs& a, dim3 g, hipStream_t st) {
-  const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
-  const int mt = ((a.mx + 1) * wy + 15) / 16, nt = (2 * wz + 15) / 16;
-  const int mtw = (mt + EK_WAVES - 1) / EK_WAVES;
+template <int MTW>
+void ek_sk_launch_nt(const EkArgs& a, const EkSkShape& sh, dim3 g, hipStream_t st) {
+  if (sh.nt == 1) ek_sk_kernel<MTW, 1><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh);
+  else ek_sk_kernel<MTW, 2><<<g, EK_SK_THREADS, sh.lds, st>>>(a, sh);
+}
+
+void ek_sk_launch(const EkArgs& a, int chunks, int n_sys, hipStream_t st) {
   const EkSkShape sh = ek_sk_shape(a);
-  switch (mtw) {
-    case 1: ek_sk_launch_nt<1>(a, sh, nt, g, st); break;
-    case 2: ek_sk_launch_nt<2>(a, sh, nt, g, st); break;
-    case 3: ek_sk_launch_nt<3>(a, sh, nt, g, st); break;
-    case 4: ek_sk_launch_nt<4>(a, sh, nt, g, st); break;
-    default: ek_sk_launch_nt<6>(a, sh, nt, g, st); break;  // up to 21 tiles (|n| <= 12)
+  const dim3 g(chunks, n_sys, sh.groups);
+  switch (sh.mtw) {
+    case 1: ek_sk_launch_nt<1>(a, sh, g, st); break;
+    case 2: ek_sk_launch_nt<2>(a, sh, g, st); break;
+    case 3: ek_sk_launch_nt<3>(a, sh, g, st); break;
+    case 4: ek_sk_launch_nt<4>(a, sh, g, st); break;
+    default: ek_sk_launch_nt<5>(a, sh, g, st); break;
   }
 }
 
@@ -521,7 +514,7 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
   a.sk = structure_factor ? structure_factor : reinterpret_cast<double*>(b);
   a.skp = reinterpret_cast<float2*>(b + sk_bytes);
   a.skpart = reinterpret_cast<double*>(b + sk_bytes + skp_bytes);
-  ek_sk_launch(a, dim3(a.chunks, n_sys), st);
+  ek_sk_launch(a, a.chunks, n_sys, st);
   KUPSMI_CHECK_LAUNCH();
   ek_sk_reduce_kernel<<<dim3((a.nk + 255) / 256, n_sys), 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
