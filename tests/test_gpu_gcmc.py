@@ -2,8 +2,9 @@
 Philox stream.
 
 Trial by trial: identical move sequence; energy differences within
-1e-4 * max(|dE|, 0.1 eV) (FP32 pair sums vs FP64; 1e-5 eV is 4e-4 kT at
-298 K); identical accept decisions, except that a decision may differ where
+1e-5 * max(|dE|, 1 eV) where |dE| <= 50 kT (FP32 pair sums vs FP64; 1e-5 eV
+is 4e-4 kT at 298 K; beyond 50 kT the Metropolis ratio is below 2e-22) and
+within 1e-3 relative beyond; identical accept decisions, except that a decision may differ where
 |log p - log u| is below 1e-3 (a near tie that FP32 may resolve either way),
 after which the chain's trajectories diverge and the comparison of that chain
 stops.
@@ -35,6 +36,7 @@ def _small_gcmc(seed=0, n_chain=4, pressure=1e6, nmin=300, nmax=400, gmax=64):
 def test_gcmc_trial_by_trial_vs_oracle(gpu):
     g = _small_gcmc()
     st = oracle_state(g)
+    kT = g.d["kT"].cpu().numpy()
     n_steps = 300
     tr_gpu = g.run(n_steps, n_trace=g.n_chain).cpu().numpy()
     tr_orc = oracle.gcmc_run(st, n_steps, 0, g.seed, n_trace=g.n_chain)
@@ -46,7 +48,10 @@ def test_gcmc_trial_by_trial_vs_oracle(gpu):
         for t in range(n_steps):
             a, b = tr_gpu[c, t], tr_orc[c, t]
             assert a[0] == b[0]  # same move type and coin
-            assert abs(a[1] - b[1]) <= 1e-4 * max(abs(b[1]), 0.1), (c, t, a, b)
+            if abs(b[1]) <= 50 * kT[c]:
+                assert abs(a[1] - b[1]) <= 1e-5 * max(abs(b[1]), 1.0), (c, t, a, b)
+            else:
+                assert abs(a[1] - b[1]) <= 1e-3 * abs(b[1]), (c, t, a, b)
             compared += 1
             if a[3] != b[3]:
                 assert abs(b[2] - lu[t]) < 1e-3, (c, t, a, b, lu[t])
