@@ -77,6 +77,7 @@ struct SysGeom {
   int h[3];       // stencil half width per axis (1, or image half count)
   int rep;        // 1 if the system needs periodic images (image grid)
   int ncell;
+  int ortho;      // 1 for an orthorhombic cell (L10 = L20 = L21 = 0)
 };
 
 __host__ __device__ inline void tril_inverse(const double* t, double* li) {
@@ -133,6 +134,7 @@ __host__ __device__ inline void make_geom(const double* tril, double rc, int max
     img[a] = n;
   }
   g.rep = img[0] * img[1] * img[2] > 1;
+  g.ortho = t[1] == 0.0 && t[3] == 0.0 && t[4] == 0.0;
   for (int a = 0; a < 3; ++a) {
     if (g.rep) {
       g.nc[a] = 1;
@@ -374,26 +376,24 @@ __device__ inline float erfc_cheb(float x) {
   return t * __expf(p - x * x);
 }
 
-// erfcx(x) = exp(x^2) erfc(x) for x >= 0 as t Q(t), t = 1 / (1 + 0.3 x), Q
-// the degree-11 polynomial below, fitted by iteratively reweighted least
-// squares for relative error over x in [0, inf) (below 4e-9 in exact
-// arithmetic, 4e-7 evaluated in FP32; erfcx -> 1 / (sqrt(pi) x) for large x
-// is the limit t Q(t) -> Q(0) t).  erfc(x) = erfcx(x) exp(-x^2) then shares
+// erfcx(x) = exp(x^2) erfc(x) for x >= 0 as t Q(t), t = 1 / (1 + 0.6 x), Q
+// the degree-9 polynomial below, fitted by iteratively reweighted least
+// squares for relative error over x in [0, inf) (below 2e-7 in exact
+// arithmetic, 3.5e-7 evaluated in FP32; erfcx -> 1 / (sqrt(pi) x) for large
+// x is the limit t Q(t) -> Q(0) t, Q(0) = 0.6 / sqrt(pi)).  erfc(x) = erfcx(x) exp(-x^2) then shares
 // its exponential with the derivative d erfc / dx = -2 / sqrt(pi) exp(-x^2).
 __device__ inline float erfcx_fit(float x) {
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.3f, x, 1.0f));
-  float q = 2.340488881e-02f;
-  q = fmaf(q, t, -9.981341660e-02f);
-  q = fmaf(q, t, 1.273843348e-01f);
-  q = fmaf(q, t, -5.513472110e-02f);
-  q = fmaf(q, t, 6.451686472e-02f);
-  q = fmaf(q, t, 7.340134680e-02f);
-  q = fmaf(q, t, 9.385725111e-02f);
-  q = fmaf(q, t, 1.260439306e-01f);
-  q = fmaf(q, t, 1.461660415e-01f);
-  q = fmaf(q, t, 1.616604328e-01f);
-  q = fmaf(q, t, 1.692561507e-01f);
-  q = fmaf(q, t, 1.692568809e-01f);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.6f, x, 1.0f));
+  float q = 7.742052525e-02f;
+  q = fmaf(q, t, -3.791217506e-01f);
+  q = fmaf(q, t, 6.996498704e-01f);
+  q = fmaf(q, t, -5.089484453e-01f);
+  q = fmaf(q, t, 9.372118860e-03f);
+  q = fmaf(q, t, -8.299292997e-03f);
+  q = fmaf(q, t, 1.549688876e-01f);
+  q = fmaf(q, t, 2.779625654e-01f);
+  q = fmaf(q, t, 3.384810984e-01f);
+  q = fmaf(q, t, 3.385145068e-01f);
   return t * q;
 }
 

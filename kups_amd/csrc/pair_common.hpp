@@ -144,12 +144,13 @@ __device__ inline void pc_bin_system(const SysGeom& g, int i0, int n, const floa
 // The out-of-line helpers take only the partner's sorted index and the
 // shift: the copies, the caller's positions and the sorted -> caller map of
 // the workgroup's system are in band_ctx (set by the kernel), and the left
-// atom is sorted atom blockIdx.x * PC_TILE + threadIdx.x, so the rare call
+// atom is sorted atom band_ctx.t0 + threadIdx.x, so the rare call
 // costs the pair loops no registers.
 struct BandCtx {
   const float4* spos;  // copies of the system (sorted)
   const float* cpos;   // caller's positions of the system
   const int* csidx;    // sorted -> caller index within the system
+  int t0;              // the workgroup's first sorted atom
 };
 static __shared__ BandCtx band_ctx;
 
@@ -159,7 +160,7 @@ static __shared__ BandCtx band_ctx;
 __device__ inline void band_pair(const SysGeom& g, int tj, int sx, int sy, int sz,
                                  const float*& pi, const float*& pj, int& c0, int& c1, int& c2) {
   const BandCtx& c = band_ctx;
-  const int ti = blockIdx.x * PC_TILE + threadIdx.x;
+  const int ti = c.t0 + threadIdx.x;
   pi = c.cpos + 3 * (size_t)c.csidx[ti];
   pj = c.cpos + 3 * (size_t)c.csidx[tj];
   int a0, a1, a2, b0, b1, b2;

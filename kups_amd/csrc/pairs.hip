@@ -109,7 +109,8 @@ __global__ __launch_bounds__(PC_TILE) void pc_pair_kernel(PairArgs a) {
   }
   if (tid == 0) {
     g = a.geom[s];
-    band_ctx = BandCtx{a.spos + i0, a.pos + 3 * (size_t)i0, a.sidx + i0};
+    band_ctx = BandCtx{a.spos + i0, a.pos + 3 * (size_t)i0, a.sidx + i0,
+                       (int)blockIdx.x * PC_TILE};
   }
   const int ns = a.n_species;
   for (int k = tid; k < ns * ns; k += PC_TILE) {

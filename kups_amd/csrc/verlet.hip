@@ -28,6 +28,7 @@
 // forces a rebuild of every system.
 #include <utility>
 #include <vector>
+#include <type_traits>
 #include "pair_common.hpp"
 
 struct kupsmi_verlet {
@@ -85,6 +86,7 @@ struct VlArgs {
   unsigned* maxd2;      // n_sys: float bits of the max squared displacement
   unsigned long long* counters;  // [0] system rebuilds, [1] overflowed atom lists
   double* partial;      // main pass, then the band pass (n_sys x tiles x PC_NPART each)
+  int* tile_off;        // n_sys + 1: prefix of the systems' tile counts
   // per sorted atom: the number of list entries inside a cutoff band met by
   // the main pass (VL_NBAND_FULL: more than VL_NBAND, or met on the stencil
   // path), and up to VL_NBAND of those entries
@@ -96,16 +98,18 @@ struct VlArgs {
   PairRecord rec;  // parity output (REC instantiation only)
 };
 
-// List entry: partner slot (15 bits), image shift + 8 per axis (4 bits each),
-// partner species (3 bits).
+// List entry: partner slot (15 bits), image shift per axis (4-bit two's
+// complement each, bits 15, 19, 23), partner species (5 bits).
 __device__ inline uint32_t vl_pack(int t, int sx, int sy, int sz, int sp) {
-  return (uint32_t)t | ((uint32_t)(sx + 8) << 15) | ((uint32_t)(sy + 8) << 19) |
-         ((uint32_t)(sz + 8) << 23) | ((uint32_t)sp << 27);
+  return (uint32_t)t | (((uint32_t)sx & 15u) << 15) | (((uint32_t)sy & 15u) << 19) |
+         (((uint32_t)sz & 15u) << 23) | ((uint32_t)sp << 27);
 }
 
-__global__ void vl_clear_kernel(int n_sys, unsigned* maxd2) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n_sys) maxd2[s] = 0u;
+// Signed shift fields of an entry (one signed bit-field extract each).
+__device__ inline void vl_shifts(uint32_t e, int& sx, int& sy, int& sz) {
+  sx = (int)(e << 13) >> 28;
+  sy = (int)(e << 9) >> 28;
+  sz = (int)(e << 5) >> 28;
 }
 
 __global__ void vl_invalidate_kernel(int n_sys, int* valid) {
@@ -113,13 +117,62 @@ __global__ void vl_invalidate_kernel(int n_sys, int* valid) {
   if (s < n_sys) valid[s] = 0;
 }
 
+// Workgroups map to (system, tile) through the tile prefix, XCD-aware: the
+// hardware deals consecutive workgroups round-robin to the VL_XCDS dies, so
+// workgroup w takes tile (w % VL_XCDS) * per + w / VL_XCDS of the compact
+// tile order, and each die walks one contiguous stretch of systems (their
+// positions stay in that die's L2 while its CUs work on them).
+constexpr int VL_XCDS = 8;
+constexpr int VL_PREP_THREADS = 1024;
+
+__device__ inline bool vl_tile_of(const VlArgs& a, int& s, int& tile) {
+  const int T = a.tile_off[a.n_sys];
+  const int per = (T + VL_XCDS - 1) / VL_XCDS;
+  const int w = blockIdx.x, k = w / VL_XCDS;
+  const int L = (w % VL_XCDS) * per + k;
+  if (k >= per || L >= T) return false;
+  int lo = 0, hi = a.n_sys;  // the last system whose first tile is <= L
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.tile_off[mid] <= L) lo = mid; else hi = mid;
+  }
+  s = lo;
+  tile = L - a.tile_off[lo];
+  return true;
+}
+
+// One workgroup: clears the per-system displacement maxima and writes the
+// tile prefix (ceil(n_s / VL_TILE) per system; chunked block scan).
+__global__ __launch_bounds__(VL_PREP_THREADS) void vl_prep_kernel(VlArgs a) {
+  __shared__ int part[VL_PREP_THREADS];
+  const int tid = threadIdx.x, ns = a.n_sys;
+  for (int k = tid; k < ns; k += VL_PREP_THREADS) a.maxd2[k] = 0u;
+  const int chunk = (ns + VL_PREP_THREADS - 1) / VL_PREP_THREADS;
+  const int b = tid * chunk, e = min(b + chunk, ns);
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += (a.sys_off[k + 1] - a.sys_off[k] + VL_TILE - 1) / VL_TILE;
+  part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < VL_PREP_THREADS; o <<= 1) {
+    const int v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;  // exclusive prefix of this chunk
+  for (int k = b; k < e; ++k) {
+    a.tile_off[k] = run;
+    run += (a.sys_off[k + 1] - a.sys_off[k] + VL_TILE - 1) / VL_TILE;
+  }
+  if (tid == VL_PREP_THREADS - 1) a.tile_off[ns] = part[tid];
+}
+
 __global__ __launch_bounds__(VL_TILE) void vl_gather_kernel(VlArgs a) {
   __shared__ float red[VL_TILE / 64];
-  const int s = blockIdx.y;
-  if (!a.valid[s]) return;
+  int s, tile;
+  if (!vl_tile_of(a, s, tile) || !a.valid[s]) return;
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
-  if ((int)blockIdx.x * VL_TILE >= n) return;
-  const int slot = blockIdx.x * VL_TILE + threadIdx.x;
+  const int slot = tile * VL_TILE + threadIdx.x;
   const SysGeom& g = a.geom[s];
   float d2 = 0.f;
   if (slot < n) {
@@ -201,13 +254,13 @@ __global__ __launch_bounds__(PC_BIN_THREADS) void vl_bin_kernel(VlArgs a) {
 // slot, masked off by the count.
 __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
   __shared__ SysGeom g;
-  const int s = blockIdx.y;
-  if (!a.need[s]) return;
+  int s, tile;
+  if (!vl_tile_of(a, s, tile) || !a.need[s]) return;
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
-  if ((int)blockIdx.x * VL_TILE >= n || n > PC_NMAX) return;
+  if (n > PC_NMAX) return;
   if (threadIdx.x == 0) g = a.geom[s];
   __syncthreads();
-  const int slot = blockIdx.x * VL_TILE + threadIdx.x;
+  const int slot = tile * VL_TILE + threadIdx.x;
   if (slot >= n) return;
   const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
   const float rl = (float)g.rc;  // rc + skin
@@ -350,6 +403,104 @@ __device__ inline float vl_entry_terms(const PairParams& pp, const float (&band)
   return gsum;
 }
 
+// The main pass over a lane's list (no exclusion ids, no record): groups of
+// four entries, two groups per trip in ping-pong registers so that the next
+// group's positions and the group after's list words are in flight while a
+// group is evaluated (groups past the end clamp to the last one; entries past
+// the count get r^2 = 3e38, above every band, and add nothing).  Partner
+// positions are gathered with 32-bit offsets from the system's base.
+// ORTHO: the image shift is s_a L_aa per axis (the general form's cross terms
+// are exact zeros there, so both give the same bits).  SAME: the LJ and
+// Coulomb cutoffs are equal and one band test serves both terms.
+template <bool ORTHO, bool SAME, typename Acc>
+__device__ inline void vl_walk_main(const VlArgs& a, const PairParams& pp, const float (&band)[4],
+                                    const SysGeom& g, int i0, int d, int cnt, float4 ri, int si,
+                                    float qiK, float& elj, float& ecl, int& nband,
+                                    const Acc& accumulate) {
+  const uint4* lp = reinterpret_cast<const uint4*>(a.list) + d;
+  const size_t stride = (size_t)a.n_atoms;
+  const int ng = (cnt + 3) >> 2;
+  const char* sbase = reinterpret_cast<const char*>(a.spos + i0);
+  auto group = [&](int q) -> uint4 { return lp[(size_t)min(q, ng - 1) * stride]; };
+  auto pos = [&](uint32_t e) -> float4 {
+    return *reinterpret_cast<const float4*>(sbase + ((e << 4) & 0x7fff0u));
+  };
+  auto positions = [&](uint4 e, float4 (&r)[4]) {
+    r[0] = pos(e.x); r[1] = pos(e.y); r[2] = pos(e.z); r[3] = pos(e.w);
+  };
+  const float nl2e = -pp.alpha2 * 1.4426950408889634f;  // -alpha^2 log2(e)
+  auto entry = [&](uint32_t e, float4 rj, bool valid) {
+    int sx, sy, sz;
+    vl_shifts(e, sx, sy, sz);
+    float ex, ey, ez;
+    if (ORTHO) {
+      ex = fmaf((float)sx, g.Lf[0], rj.x) - ri.x;
+      ey = fmaf((float)sy, g.Lf[2], rj.y) - ri.y;
+      ez = fmaf((float)sz, g.Lf[5], rj.z) - ri.z;
+    } else {
+      float Sx, Sy, Sz;
+      shift_vec32(g.Lf, sx, sy, sz, Sx, Sy, Sz);
+      ex = (rj.x + Sx) - ri.x; ey = (rj.y + Sy) - ri.y; ez = (rj.z + Sz) - ri.z;
+    }
+    const float r2 = valid ? norm2_32(ex, ey, ez) : 3.0e38f;
+    float gsum;
+    int mid;
+    if (SAME) {
+      const bool lo = r2 < band[0];
+      mid = !lo && r2 < band[1] ? 3 : 0;
+      const float ir0 = __builtin_amdgcn_rsqf(r2);
+      const float ir = lo ? ir0 : 0.0f;
+      // LJ
+      const int p = si * pp.ns + (int)(e >> 27);
+      const float ir2 = ir * ir;
+      float c6 = pp.sig2[p] * ir2;
+      c6 = c6 * c6 * c6;
+      elj += pp.e4[p] * c6 * (c6 - 1.0f);
+      gsum = pp.e24[p] * c6 * (1.0f - 2.0f * c6) * ir2;
+      // real-space Ewald: erfc(alpha r) = erfcx(alpha r) exp(-alpha^2 r^2)
+      const float x = pp.alpha * (r2 * ir0);
+      const float ex2 = __builtin_amdgcn_exp2f(r2 * nl2e);
+      const float ec = erfcx_fit(x) * ex2;
+      const float qq = qiK * rj.w;
+      ecl += qq * ec * ir;
+      gsum += qq * (-pp.two_a_sqrtpi * ex2 - ec * ir0) * ir * ir0;
+    } else {
+      int kept;
+      gsum = vl_entry_terms<false, 0>(pp, band, g, (int)(e & 0x7fffu), ri, rj, sx, sy, sz, r2,
+                                      si, (int)(e >> 27), 0, 0, 0, 0, elj, ecl, kept, mid);
+    }
+    if (__builtin_expect(mid != 0, 0)) {
+      if (nband < VL_NBAND)
+        a.bandbuf[(size_t)d * VL_NBAND + nband] =
+            (e & 0x7ffffffu) | (mid & 1 ? VL_MID_LJ : 0u) | (mid & 2 ? VL_MID_CL : 0u);
+      ++nband;
+    }
+    accumulate(gsum, ex, ey, ez);
+  };
+  auto eval = [&](uint4 e, const float4 (&r)[4], int q) {
+    const int k = 4 * q;
+    entry(e.x, r[0], k < cnt);
+    entry(e.y, r[1], k + 1 < cnt);
+    entry(e.z, r[2], k + 2 < cnt);
+    entry(e.w, r[3], k + 3 < cnt);
+  };
+  if (cnt <= 0) return;
+  uint4 ga = group(0), gb = group(1);
+  float4 ra[4], rb[4];
+  positions(ga, ra);
+  for (int q = 0; q < ng; q += 2) {
+    positions(gb, rb);
+    const uint4 gc = group(q + 2);
+    eval(ga, ra, q);
+    if (q + 1 >= ng) break;
+    positions(gc, ra);
+    const uint4 gd = group(q + 3);
+    eval(gb, rb, q + 1);
+    ga = gc;
+    gb = gd;
+  }
+}
+
 // PASS 0: the pair terms of every list entry below the band of a cutoff, one
 // lane per sorted atom; the lane keeps up to VL_NBAND entries it met inside a
 // band.  PASS 1 (the band pass): each lane that met any adds the band pairs
@@ -362,13 +513,14 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   __shared__ float s_sig2[PC_NSPEC * PC_NSPEC], s_e4[PC_NSPEC * PC_NSPEC],
       s_e24[PC_NSPEC * PC_NSPEC];
   __shared__ double red[VL_TILE / 64][PC_NPART];
-  const int s = blockIdx.y;
+  int s, tile;
+  if (!vl_tile_of(a, s, tile)) return;
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
-  const int start = blockIdx.x * VL_TILE;
+  const int start = tile * VL_TILE;
   const int tid = threadIdx.x;
   double* part = a.partial + ((size_t)(PASS ? a.n_sys : 0) * a.tiles + (size_t)s * a.tiles +
-                              blockIdx.x) * PC_NPART;
-  if (start >= n || n > PC_NMAX) {
+                              tile) * PC_NPART;
+  if (n > PC_NMAX) {
     if (tid < PC_NPART) part[tid] = 0.0;
     return;
   }
@@ -381,7 +533,7 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   }
   if (tid == 0) {
     g = a.geom[s];
-    band_ctx = BandCtx{a.spos + i0, a.pos + 3 * (size_t)i0, a.sidx + i0};
+    band_ctx = BandCtx{a.spos + i0, a.pos + 3 * (size_t)i0, a.sidx + i0, start};
   }
   const int ns = a.n_species;
   for (int k = tid; k < ns * ns; k += VL_TILE) {
@@ -410,6 +562,7 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
     const int si = a.sspec[d];
     const int xlj = EXCL && a.has_lj_excl ? a.slj[d] : 0;
     const int xcl = EXCL && a.has_coul_excl ? a.scl[d] : 0;
+    const float qiK = ri.w * 14.399645351950548f;  // q_i HARTREE BOHR (eV A)
     auto accumulate = [&](float gsum, float ex, float ey, float ez) {
       ax += gsum * ex; ay += gsum * ey; az += gsum * ez;
       if (VIR) {
@@ -424,8 +577,8 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
       for (int k = 0; k < flag; ++k) {
         const uint32_t e = a.bandbuf[(size_t)d * VL_NBAND + k];
         const int t = (int)(e & 0x7fffu);
-        const int sx = (int)((e >> 15) & 15u) - 8, sy = (int)((e >> 19) & 15u) - 8,
-                  sz = (int)((e >> 23) & 15u) - 8;
+        int sx, sy, sz;
+        vl_shifts(e, sx, sy, sz);
         const float4 rj = a.spos[i0 + t];
         float Sx, Sy, Sz;
         shift_vec32(g.Lf, sx, sy, sz, Sx, Sy, Sz);
@@ -439,6 +592,22 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
             (e & VL_MID_LJ ? 1 : 0) | (e & VL_MID_CL ? 2 : 0));
         if (REC && kept) band_record(a.rec, g, i0, t, sx, sy, sz, kept);
         accumulate(gsum, ex, ey, ez);
+      }
+    } else if (cnt <= a.cap && PASS == 0 && !REC && !EXCL) {
+      // the hot walk: variants for orthorhombic cells (no cross terms in the
+      // image shift) and for equal LJ and Coulomb cutoffs (one band test)
+      const bool same = pp.has_lj && pp.has_coul && pp.lj_rc2 == pp.cl_rc2;
+      auto walk = [&](auto ortho_t, auto same_t) {
+        constexpr bool ORTHO = decltype(ortho_t)::value, SAME = decltype(same_t)::value;
+        vl_walk_main<ORTHO, SAME>(a, pp, band, g, i0, d, cnt, ri, si, qiK, elj, ecl, nband,
+                                  accumulate);
+      };
+      if (g.ortho) {
+        if (same) walk(std::true_type{}, std::true_type{});
+        else walk(std::true_type{}, std::false_type{});
+      } else {
+        if (same) walk(std::false_type{}, std::true_type{});
+        else walk(std::false_type{}, std::false_type{});
       }
     } else if (cnt <= a.cap) {
       // one group of four entries per trip (one 16-byte load); while group q
@@ -472,8 +641,8 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
           const uint32_t e = ew[u];
           const float4 rj = ra[u];
           const int t = (int)(e & 0x7fffu);
-          const int sx = (int)((e >> 15) & 15u) - 8, sy = (int)((e >> 19) & 15u) - 8,
-                    sz = (int)((e >> 23) & 15u) - 8;
+          int sx, sy, sz;
+          vl_shifts(e, sx, sy, sz);
           float Sx, Sy, Sz;
           shift_vec32(g.Lf, sx, sy, sz, Sx, Sy, Sz);
           const float ex = (rj.x + Sx) - ri.x, ey = (rj.y + Sy) - ri.y, ez = (rj.z + Sz) - ri.z;
@@ -556,18 +725,34 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   pc_block_partials<VL_TILE>(v, red, part);
 }
 
+// Per system: the main pass's tile partials, then the band pass's, in tile
+// order (the system's own tiles only).
 __global__ void vl_reduce_kernel(VlArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n_sys) return;
-  pc_reduce_system(s, a.tiles, a.partial, a.accumulate, a.energy, a.virial,
-                   a.partial + (size_t)a.n_sys * a.tiles * PC_NPART);
+  const int nt = a.tile_off[s + 1] - a.tile_off[s];
+  double acc[PC_NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int h = 0; h < 2; ++h) {
+    const double* p = a.partial + ((size_t)h * a.n_sys + s) * a.tiles * PC_NPART;
+    for (int t = 0; t < nt; ++t)
+      for (int k = 0; k < PC_NPART; ++k) acc[k] += p[(size_t)t * PC_NPART + k];
+  }
+  if (a.energy) {
+    double* e = a.energy + 2 * (size_t)s;
+    if (a.accumulate) { e[0] += acc[0]; e[1] += acc[1]; } else { e[0] = acc[0]; e[1] = acc[1]; }
+  }
+  if (a.virial) {
+    const double w[9] = {acc[2], acc[3], acc[4], acc[3], acc[5], acc[6], acc[4], acc[6], acc[7]};
+    double* W = a.virial + 9 * (size_t)s;
+    for (int k = 0; k < 9; ++k) W[k] = a.accumulate ? W[k] + w[k] : w[k];
+  }
 }
 
 // Byte offsets of the handle's arrays.
 struct VlLayout {
   size_t geom, spos, sref, sidx, scell, sspec, slj, scl, cstart, nlist, list, valid, need, reprc,
       maxd2,
-      counters, partial, bandflag, bandbuf, total;
+      counters, partial, bandflag, bandbuf, tile_off, total;
   VlLayout(int n_sys, int n_atoms, int cap, int tiles) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
@@ -591,6 +776,7 @@ struct VlLayout {
     partial = take(2 * sizeof(double) * ns * (size_t)tiles * PC_NPART);
     bandflag = take(na);
     bandbuf = take(sizeof(uint32_t) * na * VL_NBAND);
+    tile_off = take(sizeof(int) * (ns + 1));
     total = off;
   }
 };
@@ -761,12 +947,14 @@ extern "C" int kupsmi_pair_forces_verlet(
   a.partial = reinterpret_cast<double*>(b + lay.partial);
   a.bandflag = reinterpret_cast<uint8_t*>(b + lay.bandflag);
   a.bandbuf = reinterpret_cast<uint32_t*>(b + lay.bandbuf);
+  a.tile_off = reinterpret_cast<int*>(b + lay.tile_off);
   a.forces = forces; a.energy = energy; a.virial = virial;
   int ncu = 256;
   KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  vl_clear_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(n_sys, a.maxd2);
+  vl_prep_kernel<<<1, VL_PREP_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
-  dim3 grid(tiles, n_sys);
+  // one workgroup per (system, tile), compact: at most one partial tile per system
+  const dim3 grid((unsigned)((n_atoms + VL_TILE - 1) / VL_TILE + n_sys));
   vl_gather_kernel<<<grid, VL_TILE, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   vl_bin_kernel<<<n_sys < ncu ? n_sys : ncu, PC_BIN_THREADS, 0, st>>>(a);
