@@ -40,7 +40,7 @@ struct EkArgs {
   int chunks;       // atom chunks per system of the S(k) pass
   double* sk;       // n_sys x nk x 2 (FP64)
   double* skpart;   // n_sys x chunks x nk x 2: chunk partials of S
-  float2* skp;      // n_sys x nk, P(k) S(k)
+  float* skc;       // n_sys x M x (4 + 4 mz): force coefficients of P(k) S(k)
   float* forces;
   double* energy;
   double* virial;
@@ -327,8 +327,10 @@ __global__ __launch_bounds__(EK_THREADS) void ek_energy_kernel(EkArgs a) {
   const double K = 14.399645351950548;
   const int wy = 2 * a.my + 1, wz = 2 * a.mz + 1;
   double e = 0, w00 = 0, w01 = 0, w02 = 0, w11 = 0, w12 = 0, w22 = 0;
-  for (int k = tid; k < a.nk; k += EK_THREADS) {
-    const int n0 = k / (wy * wz), n1 = (k / wz) % wy - a.my, n2 = k % wz - a.mz;
+  // S'(k) = P(k) S(k) and the energy / virial of k; returns S'
+  auto term = [&](int m, int n2) -> double2 {
+    const int n0 = m / wy, n1 = m % wy - a.my;
+    const int k = m * wz + a.mz + n2;
     const double kv0 = 2 * M_PI * (li[0] * n0);
     const double kv1 = 2 * M_PI * (li[1] * n0 + li[2] * n1);
     const double kv2 = 2 * M_PI * (li[3] * n0 + li[4] * n1 + li[5] * n2);
@@ -345,7 +347,25 @@ __global__ __launch_bounds__(EK_THREADS) void ek_energy_kernel(EkArgs a) {
       w00 += ek * (1 - f * kv0 * kv0); w11 += ek * (1 - f * kv1 * kv1); w22 += ek * (1 - f * kv2 * kv2);
       w01 -= ek * f * kv0 * kv1; w02 -= ek * f * kv0 * kv2; w12 -= ek * f * kv1 * kv2;
     }
-    a.skp[(size_t)s * a.nk + k] = make_float2((float)(P * sk[0]), (float)(P * sk[1]));
+    return make_double2(P * sk[0], P * sk[1]);
+  };
+  // per m: S'(m, 0) (x, y, 0, 0), then for n2 = 1..mz the combinations of
+  // A = S'(m, +n2), B = S'(m, -n2) that the force pass contracts with the
+  // phase products (ek_force_kernel)
+  const int stride = 4 + 4 * a.mz;
+  float* out = a.skc + (size_t)s * (a.mx + 1) * wy * stride;
+  for (int it = tid; it < (a.mx + 1) * wy * (a.mz + 1); it += EK_THREADS) {
+    const int m = it / (a.mz + 1), n2 = it % (a.mz + 1);
+    float* o = out + (size_t)m * stride;
+    if (n2 == 0) {
+      const double2 A = term(m, 0);
+      o[0] = (float)A.x; o[1] = (float)A.y; o[2] = 0.f; o[3] = 0.f;
+    } else {
+      const double2 A = term(m, n2), B = term(m, -n2);
+      float* c = o + 4 * n2;
+      c[0] = (float)(A.y + B.y); c[1] = (float)(B.y - A.y);
+      c[2] = (float)(B.x - A.x); c[3] = (float)(-(A.x + B.x));
+    }
   }
   const double v[7] = {e, w00, w01, w02, w11, w12, w22};
   const int lane = tid & 63, wv = tid >> 6;
@@ -367,53 +387,88 @@ __global__ __launch_bounds__(EK_THREADS) void ek_energy_kernel(EkArgs a) {
   }
 }
 
-// MZ > 0: the k grid's z half width as a constant (inner loop fully
-// unrolled, (n2) factors folded); MZ = 0: read from the arguments.
-template <int MZ>
+// F_i = -2 K q_i sum_k k Im(S'(k) conj(e_i(k))), e_i = X(n0) Y(n1) Z(n2):
+// one thread per atom.  The +n2 and -n2 terms of a row (n0, n1) share the
+// products P1 = xr zr, P2 = xi zi, P3 = xr zi, P4 = xi zr of the row phase
+// x = X Y and Z(n2) (e(+n2) = (P1 - P2, P3 + P4), e(-n2) = (P1 + P2, P4 - P3)),
+// so with A = S'(+n2), B = S'(-n2) and c = (Ay + By, By - Ay, Bx - Ax,
+// -(Ax + Bx)):
+//   v+ + v- = P1 c0 + P2 c1 + P3 c2 + P4 c3
+//   v+ - v- = -P1 c1 - P2 c0 + P3 c3 + P4 c2
+// with v = Im(S' conj(e)); the four coefficients per (row, n2) come from
+// ek_energy_kernel.  MZ > 0: the z half width as a constant (Z(n2) kept in
+// registers, loop unrolled); MZ = 0: any width, Z(n2) by recurrence per
+// row.  LDS: the coefficients staged per workgroup; otherwise read from
+// global memory (grids whose coefficients exceed 64 KB).
+template <int MZ, bool LDS>
 __global__ __launch_bounds__(EK_THREADS) void ek_force_kernel(EkArgs a) {
-  extern __shared__ float2 skp[];
+  extern __shared__ float4 skc_lds[];
   const int s = blockIdx.y;
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
   const int tid = threadIdx.x;
   const int start = blockIdx.x * EK_THREADS;
   if (start >= n) return;
-  for (int k = tid; k < a.nk; k += EK_THREADS) skp[k] = a.skp[(size_t)s * a.nk + k];
-  __syncthreads();
+  const int mz = MZ > 0 ? MZ : a.mz;
+  const int wy = 2 * a.my + 1;
+  const int stride4 = 1 + mz;  // float4 per row
+  const size_t rows = (size_t)(a.mx + 1) * wy;
+  const float4* __restrict__ gsrc =
+      reinterpret_cast<const float4*>(a.skc) + (size_t)s * rows * stride4;
+  const float4* __restrict__ skc = gsrc;
+  if (LDS) {
+    for (size_t k = tid; k < rows * stride4; k += EK_THREADS) skc_lds[k] = gsrc[k];
+    __syncthreads();
+    skc = skc_lds;
+  }
   const int i = start + tid;
   if (i >= n) return;
   const double* t = a.tril + 6 * s;
   float2 ux, uy, uz;
   unit_phases(t, a.pos + 3 * (size_t)(i0 + i), ux, uy, uz);
-  const int mz = MZ > 0 ? MZ : a.mz;
-  const int wy = 2 * a.my + 1, wz = 2 * mz + 1;
-  // u^-m starting points
-  float2 uy0 = make_float2(1.f, 0.f), uz0 = make_float2(1.f, 0.f);
-  const float2 cy = make_float2(uy.x, -uy.y), cz = make_float2(uz.x, -uz.y);
+  float2 zk[MZ > 0 ? MZ : 1];
+  if (MZ > 0) {
+    zk[0] = uz;
+#pragma unroll
+    for (int k = 1; k < (MZ > 0 ? MZ : 1); ++k) zk[k] = cmul(zk[k - 1], uz);
+  }
+  // u^-my starting point
+  float2 uy0 = make_float2(1.f, 0.f);
+  const float2 cy = make_float2(uy.x, -uy.y);
   for (int m = 0; m < a.my; ++m) uy0 = cmul(uy0, cy);
-  for (int m = 0; m < mz; ++m) uz0 = cmul(uz0, cz);
   float t0 = 0.f, t1 = 0.f, t2 = 0.f;
   float2 ex = make_float2(1.f, 0.f);
-  int k = 0;
+  const float4* __restrict__ row = skc;
   for (int n0 = 0; n0 <= a.mx; ++n0) {
-    float2 exy = cmul(ex, uy0);
+    float2 x = cmul(ex, uy0);
     for (int j1 = 0; j1 < wy; ++j1) {
       const float n1 = (float)(j1 - a.my);
-      float2 e = cmul(exy, uz0);
-      float a0 = 0.f, a2 = 0.f;
+      const float4 s0 = row[0];
+      float a0 = s0.y * x.x - s0.x * x.y;
+      float a2 = 0.f;
+      auto pair = [&](float2 z, int k) {
+        const float4 c = row[1 + k];
+        const float P1 = x.x * z.x, P2 = x.y * z.y, P3 = x.x * z.y, P4 = x.y * z.x;
+        a0 = fmaf(P1, c.x, a0); a0 = fmaf(P2, c.y, a0);
+        a0 = fmaf(P3, c.z, a0); a0 = fmaf(P4, c.w, a0);
+        float dm = P3 * c.w;
+        dm = fmaf(P4, c.z, dm); dm = fmaf(-P1, c.y, dm); dm = fmaf(-P2, c.x, dm);
+        a2 = fmaf((float)(k + 1), dm, a2);
+      };
+      if (MZ > 0) {
 #pragma unroll
-      for (int j2 = 0; j2 < (MZ > 0 ? 2 * MZ + 1 : wz); ++j2) {
-        const float2 sp = skp[k + j2];
-        // Im(S' conj(e))
-        const float v = sp.y * e.x - sp.x * e.y;
-        a0 += v;
-        a2 += (float)(j2 - mz) * v;
-        e = cmul(e, uz);
+        for (int k = 0; k < (MZ > 0 ? MZ : 1); ++k) pair(zk[k], k);
+      } else {
+        float2 z = uz;
+        for (int k = 0; k < mz; ++k) {
+          pair(z, k);
+          z = cmul(z, uz);
+        }
       }
-      k += wz;
+      row += stride4;
       t0 += (float)n0 * a0;
       t1 += n1 * a0;
       t2 += a2;
-      exy = cmul(exy, uy);
+      x = cmul(x, uy);
     }
     ex = cmul(ex, ux);
   }
@@ -492,7 +547,8 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
   a.chunks = max_atoms_per_sys > 0 ? (max_atoms_per_sys + EK_CHUNK - 1) / EK_CHUNK : 1;
   void* buf = nullptr;
   const size_t sk_bytes = structure_factor ? 0 : ((sizeof(double) * 2 * nsk + 255) & ~(size_t)255);
-  const size_t skp_bytes = (sizeof(float2) * nsk + 255) & ~(size_t)255;
+  const size_t coef_floats = (size_t)(mx + 1) * (2 * my + 1) * (4 + 4 * mz);
+  const size_t skp_bytes = (sizeof(float) * coef_floats * n_sys + 255) & ~(size_t)255;
   const size_t part_bytes = sizeof(double) * 2 * nsk * (size_t)a.chunks;
   const size_t need = sk_bytes + skp_bytes + part_bytes;
   if (ctx) {
@@ -512,7 +568,7 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
   }
   char* b = static_cast<char*>(buf);
   a.sk = structure_factor ? structure_factor : reinterpret_cast<double*>(b);
-  a.skp = reinterpret_cast<float2*>(b + sk_bytes);
+  a.skc = reinterpret_cast<float*>(b + sk_bytes);
   a.skpart = reinterpret_cast<double*>(b + sk_bytes + skp_bytes);
   ek_sk_launch(a, a.chunks, n_sys, st);
   KUPSMI_CHECK_LAUNCH();
@@ -522,9 +578,13 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
   KUPSMI_CHECK_LAUNCH();
   if (forces && max_atoms_per_sys > 0) {
     dim3 g2((max_atoms_per_sys + EK_THREADS - 1) / EK_THREADS, n_sys);
-    const size_t lds = sizeof(float2) * a.nk;
-    if (a.mz == 8) ek_force_kernel<8><<<g2, EK_THREADS, lds, st>>>(a);
-    else ek_force_kernel<0><<<g2, EK_THREADS, lds, st>>>(a);
+    const size_t lds = sizeof(float) * coef_floats;
+    if (lds <= 64 * 1024) {
+      if (a.mz == 8) ek_force_kernel<8, true><<<g2, EK_THREADS, lds, st>>>(a);
+      else ek_force_kernel<0, true><<<g2, EK_THREADS, lds, st>>>(a);
+    } else {
+      ek_force_kernel<0, false><<<g2, EK_THREADS, 0, st>>>(a);
+    }
     KUPSMI_CHECK_LAUNCH();
   }
   return KUPSMI_OK;
