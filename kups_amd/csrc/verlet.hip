@@ -267,7 +267,8 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
   const float rl2 = rl * rl * (1.0f + 1e-5f);
   const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
   const int d = i0 + slot;
-  const float4 ri = a.spos[d];
+  const float4* sp = a.spos + i0;
+  const float4 ri = sp[slot];
   a.sref[d] = ri;
   const int c = a.scell[d];
   const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
@@ -282,15 +283,33 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
     if ((cnt & 3) == 3 && cnt < cap) list4[(size_t)(cnt >> 2) * stride + d] = make_uint4(b0, b1, b2, b3);
     ++cnt;
   };
+  // orthorhombic cells: skip stencil cells whose box (with its image shift)
+  // lies farther than rc + skin from the atom.  Cell k of an axis spans
+  // [(k / nc - 1/2) L, ((k + 1) / nc - 1/2) L) in the wrapped frame; the
+  // margin of 1e-4 L covers an FP32 cell assignment a few ulp off a face.
+  const bool prune = g.ortho;
+  const float lim = prune ? (rl * (1.0f + 1e-5f) + 1e-4f * fmaxf(g.Lf[0], fmaxf(g.Lf[2], g.Lf[5]))) *
+                                (rl * (1.0f + 1e-5f) + 1e-4f * fmaxf(g.Lf[0], fmaxf(g.Lf[2], g.Lf[5])))
+                          : INFINITY;
+  auto box_d2 = [&](int k, int nc, float L, float x) {
+    const float lo = ((float)k / (float)nc - 0.5f) * L, hi = ((float)(k + 1) / (float)nc - 0.5f) * L;
+    const float t = fmaxf(fmaxf(lo - x, x - hi), 0.0f);
+    return t * t;
+  };
   for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
+    const float bz = prune ? box_d2(cz + dz, ncz, g.Lf[5], ri.z) : 0.0f;
+    if (bz > lim) continue;
     int zz = cz + dz;
     const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
     zz -= sz * ncz;
     for (int dy = -g.h[1]; dy <= g.h[1]; ++dy) {
+      const float byz = bz + (prune ? box_d2(cy + dy, ncy, g.Lf[2], ri.y) : 0.0f);
+      if (byz > lim) continue;
       int yy = cy + dy;
       const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
       yy -= sy * ncy;
       for (int dx = -g.h[0]; dx <= g.h[0]; ++dx) {
+        if (byz + (prune ? box_d2(cx + dx, ncx, g.Lf[0], ri.x) : 0.0f) > lim) continue;
         int xx = cx + dx;
         const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
         xx -= sx * ncx;
@@ -302,13 +321,21 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
                               sy <= VL_SHIFT_MAX && sz >= -VL_SHIFT_MAX && sz <= VL_SHIFT_MAX;
         const int nb = (zz * ncy + yy) * ncx + xx;
         const int tb = cs[nb], te = cs[nb + 1];
-        for (int t = tb; t < te; ++t) {
-          if (center && t == slot) continue;
-          const float4 rj = a.spos[i0 + t];
-          const float ex = rj.x - qx, ey = rj.y - qy, ez = rj.z - qz;
-          if (!(norm2_32(ex, ey, ez) < rl2)) continue;
-          if (!packable) cnt = cap + 1;  // cannot be stored: use the stencil
-          push(vl_pack(t, sx, sy, sz, a.sspec[i0 + t]));
+        // four candidates per trip, their positions loaded together (indices
+        // past the cell clamp to its last atom and are skipped)
+        for (int t = tb; t < te; t += 4) {
+          float4 rj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) rj[u] = sp[min(t + u, te - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int tu = t + u;
+            if (tu >= te || (center && tu == slot)) continue;
+            const float ex = rj[u].x - qx, ey = rj[u].y - qy, ez = rj[u].z - qz;
+            if (!(norm2_32(ex, ey, ez) < rl2)) continue;
+            if (!packable) cnt = cap + 1;  // cannot be stored: use the stencil
+            push(vl_pack(tu, sx, sy, sz, a.sspec[i0 + tu]));
+          }
         }
       }
     }
