@@ -194,6 +194,11 @@ class EwaldLJPropagator:
         self._dev = None
         # library context of the force calls (None: the device's shared one)
         self.context = None
+        # reciprocal-space kernels run on a second stream beside the pair
+        # kernels, into buffers of their own, joined before the closing kick
+        self._side = None
+        self._f2 = None
+        self._vir2 = None
 
     def _prep(self, state: MDState):
         dev = state.positions.device
@@ -218,9 +223,34 @@ class EwaldLJPropagator:
                                                           self.skin)
             self.verlet = VerletList(self.skin, cap)
 
+    def _side_stream(self, state: MDState):
+        dev = state.positions.device
+        if self._side is None or self._side.device != dev:
+            self._side = torch.cuda.Stream(device=dev)
+        if self._f2 is None or self._f2.shape != state.forces.shape or self._f2.device != dev:
+            self._f2 = torch.empty_like(state.forces)
+        if self._vir2 is None or self._vir2.shape[0] != state.n_sys or self._vir2.device != dev:
+            self._vir2 = torch.empty((state.n_sys, 3, 3), dtype=torch.float64, device=dev)
+        return self._side
+
     def forces(self, state: MDState, e2, elr, vir=None):
+        """Pair forces (LJ + real-space Ewald) on the current stream and the
+        reciprocal-space Ewald on a second stream, concurrently: the k-space
+        kernels write forces and virial into buffers of their own, added to
+        the pair results after the join (the same additions, in the same
+        order, as accumulating in place).  elr is set to the k-space energy."""
         lib = _lib.load()
         ctx = self.context.handle if self.context is not None else _lib.ctx()
+        main = torch.cuda.current_stream(state.positions.device)
+        side = self._side_stream(state)
+        side.wait_stream(main)
+        mx, my, mz = self.ewald.grid
+        with torch.cuda.stream(side):
+            _lib.check(lib.kupsmi_ewald_recip(
+                ctx, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
+                _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.tril),
+                _lib.ptr(self._alpha), _lib.ptr(self._kmax), mx, my, mz, 0, _lib.ptr(self._f2),
+                _lib.ptr(elr), _lib.ptr(self._vir2) if vir is not None else None, None))
         if self.verlet is not None:
             _lib.check(lib.kupsmi_pair_forces_verlet(
                 self.verlet.handle, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n,
@@ -237,12 +267,10 @@ class EwaldLJPropagator:
                 _lib.ptr(self._eps), _lib.ptr(state.cutoff), None, _lib.ptr(self._alpha),
                 _lib.ptr(self._rc), None, _lib.ptr(state.tril), 0, _lib.ptr(state.forces),
                 _lib.ptr(e2), _lib.ptr(vir), None))
-        mx, my, mz = self.ewald.grid
-        _lib.check(lib.kupsmi_ewald_recip(
-            ctx, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n, self._maxn,
-            _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.tril),
-            _lib.ptr(self._alpha), _lib.ptr(self._kmax), mx, my, mz, 1, _lib.ptr(state.forces),
-            _lib.ptr(elr), _lib.ptr(vir), None))
+        main.wait_stream(side)
+        state.forces.add_(self._f2)
+        if vir is not None:
+            vir.add_(self._vir2)
 
     def evaluate_forces(self, state: MDState):
         self._prep(state)
@@ -310,7 +338,6 @@ class EwaldLJPropagator:
                     _lib.ptr(state.positions), None, None, _lib.ptr(state.tril),
                     _lib.ptr(state.time_step), _lib.ptr(kT), None, _lib.ptr(state.barostat),
                     _lib.ptr(vir), _lib.ptr(kin[k]), seed, t, sd))
-                elr[k].zero_()
                 self.forces(state, e2[k], elr[k], vir)
             if step_dev is not None:
                 _lib.check(lib.kupsmi_step_advance(_lib.stream_ptr(), sd, 1))
@@ -331,10 +358,11 @@ class GraphedSteps:
     the step counter of the random streams is a device int64 that the
     recorded kernels read and advance, set from ``state.step`` before each
     replay, so replays draw the same noise as eager steps would.  The
-    recorded launches use scratch and a Verlet list of their own (a private
-    library context and Verlet handle, frozen after capture) and hold the
-    propagator's parameter tables: no eager call, of this propagator or any
-    other, can reallocate or free memory the graph reads.
+    recorded launches use scratch, a Verlet list and k-space force and
+    virial buffers of their own (a private library context and Verlet
+    handle, frozen after capture) and hold the propagator's parameter
+    tables: no eager call, of this propagator or any other, can reallocate or
+    free memory the graph reads.
     Capture first evaluates the forces once eagerly with those so they are
     sized before recording (the recorded launches allocate nothing).  The
     observables of the last replay are in ``obs``."""
@@ -347,8 +375,9 @@ class GraphedSteps:
         self.context = _lib.Context()
         self.verlet = (VerletList(prop.verlet.skin, prop.verlet.max_neighbors)
                        if prop.verlet is not None else None)
-        saved = prop.context, prop.verlet
-        prop.context, prop.verlet = self.context, self.verlet
+        saved = prop.context, prop.verlet, prop._f2, prop._vir2
+        # fresh k-space force / virial buffers for the recorded launches
+        prop.context, prop.verlet, prop._f2, prop._vir2 = self.context, self.verlet, None, None
         try:
             f0 = state.forces.clone()
             prop.evaluate_forces(state)
@@ -361,8 +390,9 @@ class GraphedSteps:
             with torch.cuda.graph(self.graph):
                 self.obs = prop._steps(state, self.n, self.step_dev, es)
             torch.cuda.synchronize()
+            self._buffers = (prop._f2, prop._vir2)
         finally:
-            prop.context, prop.verlet = saved
+            prop.context, prop.verlet, prop._f2, prop._vir2 = saved
         self.context.freeze()
         if self.verlet is not None:
             self.verlet.freeze()
