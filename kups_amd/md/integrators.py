@@ -196,6 +196,8 @@ class EwaldLJPropagator:
         self.context = None
         # reciprocal-space kernels run on a second stream beside the pair
         # kernels, into buffers of their own, joined before the closing kick
+        # (overlap = False: both on the current stream, one after the other)
+        self.overlap = True
         self._side = None
         self._f2 = None
         self._vir2 = None
@@ -242,7 +244,7 @@ class EwaldLJPropagator:
         lib = _lib.load()
         ctx = self.context.handle if self.context is not None else _lib.ctx()
         main = torch.cuda.current_stream(state.positions.device)
-        side = self._side_stream(state)
+        side = self._side_stream(state) if self.overlap else main
         side.wait_stream(main)
         mx, my, mz = self.ewald.grid
         with torch.cuda.stream(side):
