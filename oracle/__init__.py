@@ -6,11 +6,14 @@ library and the CPU baseline timed next to it.  Only ``tests/``,
 the product package ``kups_amd`` never does.
 
 The compiled part (``kups_oracle.c`` -> ``liboracle.so``) restates the
-reference's algorithms in FP64; ``oracle.py`` holds the numpy restatements
-(Ewald parameter estimation, GCMC log-ratios, Peng-Robinson fugacity, the
-LJ tail correction) and the ctypes bindings.  Each function cites the
-reference file:line it follows.  Parity pins for every function are listed
-in ``tests/test_oracle.py``.
+reference's algorithms in FP64: neighbor lists and rh-subset queries, LJ
+(plain and switched), Morse, Ewald real and reciprocal space, the MD steps,
+CSVR and stochastic cell rescaling, GCMC trial moves, Philox4x32-10.
+``oracle.py`` holds the ctypes bindings and the numpy pieces of the MLP
+reference (forward pass, loss and gradient).  Each function cites the
+reference file:line it follows.  Parity pins for every function are in
+``tests/test_oracle.py``, ``tests/test_gcmc_cpu.py``,
+``tests/test_nl_query_cpu.py`` and ``tests/test_rng_barostat_cpu.py``.
 """
 
 from .oracle import *  # noqa: F401,F403
