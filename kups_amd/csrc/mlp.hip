@@ -22,7 +22,7 @@ constexpr int ML_OFF_W4 = ML_OFF_B3 + ML_H;
 constexpr int ML_OFF_B4 = ML_OFF_W4 + ML_H;
 constexpr int ML_NP = ML_OFF_B4 + 1;  // 37377
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline unsigned short f2bf(float f) {
@@ -88,14 +88,16 @@ __global__ __launch_bounds__(256) void ml_desc_kernel(int n_chain, int nv, const
 }
 
 // ---------------------------------------------------- MFMA inference ----
-// One wave per 16-row tile: three 16 x 128 layers as v_mfma_f32_16x16x16_bf16
-// (A: 16 x 16 activations, B: 16 x 16 weights, FP32 accumulators).  The eight
-// 16-column output tiles of a layer are accumulated together, so eight
-// independent MFMA chains share each A fragment; bias and SiLU are applied on
-// the accumulators (reciprocal by v_rcp_f32: the result is rounded to BF16),
-// activations are re-staged in LDS as BF16.  Weights are held transposed
-// ([out][in]) in LDS so that every B fragment is one 8-byte load; rows are
-// padded by 8 elements so the 16 rows of a fragment fall in different banks.
+// One wave per 16-row tile: three 16 x 128 layers as gfx950's
+// v_mfma_f32_16x16x32_bf16 (A: 16 x 32 activations, B: 32 x 16 weights, FP32
+// accumulators; lane l holds row / column l % 16 and eight consecutive k from
+// 8 (l / 16), the same k set on both sides).  The eight 16-column output
+// tiles of a layer are accumulated together, so eight independent MFMA chains
+// share each A fragment; bias and SiLU are applied on the accumulators
+// (reciprocal by v_rcp_f32: the result is rounded to BF16), activations are
+// re-staged in LDS as BF16.  Weights are held transposed ([out][in]) in LDS
+// so that every B fragment is one 16-byte load; rows are padded by 8
+// elements (16 bytes) so the 16 rows of a fragment fall in different banks.
 constexpr int ML_WAVES = 8;
 constexpr int ML_LD_IN = ML_IN + 8;  // padded row lengths (BF16 elements)
 constexpr int ML_LD_H = ML_H + 8;
@@ -107,17 +109,17 @@ __device__ inline float silu_bf(float z) {
 template <int KIN, int LDK>
 __device__ inline void ml_layer(const unsigned short* act, const unsigned short* wT,
                                 const float* bias, unsigned short* out, int lane) {
-  const int row = lane & 15, kq = (lane >> 4) * 4;
+  const int row = lane & 15, kq = (lane >> 4) * 8;
   f32x4 acc[ML_H / 16];
 #pragma unroll
   for (int nt = 0; nt < ML_H / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int ks = 0; ks < KIN / 16; ++ks) {
-    const bf16x4 a = *reinterpret_cast<const bf16x4*>(act + row * LDK + ks * 16 + kq);
+  for (int ks = 0; ks < KIN / 32; ++ks) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(act + row * LDK + ks * 32 + kq);
 #pragma unroll
     for (int nt = 0; nt < ML_H / 16; ++nt) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(wT + (nt * 16 + row) * LDK + ks * 16 + kq);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(wT + (nt * 16 + row) * LDK + ks * 32 + kq);
+      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
     }
   }
   // D[i][j]: i = (lane / 16) * 4 + r, j = lane % 16 (+ 16 nt)
