@@ -202,7 +202,8 @@ int kupsmi_morse_bonds(void* stream, int n_sys, const int32_t* sys_off, int n_at
  * W = -sum_directed u'(r^2) d (x) d (stress = W / V, observables/stress.py).
  * Pair arithmetic FP32, cutoff band re-decided in FP64, energies/virials
  * accumulated in FP64.  sigma/eps: (n_species x n_species) mixed tables in
- * FP64 (Lorentz-Berthelot, lennard_jones.py:125-141).
+ * FP64 (Lorentz-Berthelot, lennard_jones.py:125-141), n_species <= 16 (also
+ * for kupsmi_pair_forces_verlet).
  * Exclusion is by particle (i != j on the minimum image), as in md_lj.
  * Positions are wrapped into the cell by whole lattice vectors (in FP64,
  * rounded once to FP32) before the pairs are decided; a position already
@@ -458,8 +459,19 @@ int kupsmi_step_advance(void* stream, int64_t* step_dev, int64_t n);
  * molecule and the LJ tail change, acceptance log p = -dE/kT + fugacity term
  * (mcmc/probability.py:60-225) against log U in FP64, and the
  * acceptance-target step-width schedule (core/parameter_scheduler.py:161-208).
+ * Mixtures (mcmc/moves.py:573-773, mcmc/probability.py:117-173, 228-256):
+ * n_motif <= 4 guest species; an insertion or deletion picks the species
+ * uniformly per chain, a deletion then one guest of that species (a no-op when
+ * there is none), and the fugacity term is ln(a_t V) - ln(N_t + 1) /
+ * -ln(a_t V) + ln N_t with the chosen species' activity and count;
+ * translation, rotation and reinsertion move a guest of any species.
  *   host_xyzq      [n_host_total x 4] f32 (x, y, z, charge), chains by host_off
- *   motif_*        the guest template (m <= 4 sites, centred)
+ *   motif_*        the guest templates [n_motif x m (x 3)], centred; a species
+ *                  with fewer than m sites leaves its last rows unused
+ *   motif_nsite    HOST array[n_motif]: sites of each species (1..m <= 4)
+ *   mol_const, log_activity  [n_chain x n_motif] f64 (self + exclusion energy
+ *                  of one molecule, ln(fugacity / kT))
+ *   guest_motif    [n_chain x gmax] int32: species of each guest slot, in/out
  *   kn / pk        per chain k list: nk packed (n0 | (n1+64) << 8 | (n2+64) << 16)
  *                  and FP64 prefactors P(k) (0 = padding); nm = max |n|
  *   guest          [n_chain x gmax x m x 3] f64 (rigid-body moves in FP64, so molecules
@@ -483,13 +495,15 @@ int kupsmi_step_advance(void* stream, int64_t* step_dev, int64_t n);
 int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t step0, uint64_t seed,
                     int max_host, const int32_t* host_off, const float* host_xyzq,
                     const int32_t* host_species, const double* tril, int gmax, int m,
-                    const float* motif_pos, const float* motif_q, const int32_t* motif_species,
+                    int n_motif, const int32_t* motif_nsite, const float* motif_pos,
+                    const float* motif_q, const int32_t* motif_species,
                     int n_species, const double* sigma, const double* epsilon, double rc_lj,
                     double rc_coul, const double* alpha, const double* tail,
                     const double* mol_const, const double* kT, const double* log_activity,
                     int nk, int nm, const int32_t* kn, const double* pk,
                     const double* move_weights, const double* width_upper, double* guest,
-                    int32_t* n_guest, int32_t* counts, double* structure_factor,
+                    int32_t* n_guest, int32_t* guest_motif, int32_t* counts,
+                    double* structure_factor,
                     double* energy, float* width, uint32_t* hist, int32_t* hist_idx,
                     long long* stats, int n_trace, double* trace, const float* prop_cdf,
                     const int32_t* prop_guide, int prop_nv, int* sample_count, int sample_cap,

@@ -62,9 +62,9 @@ _SIGNATURES = {
         C.c_int,
     ),
     "kupsmi_gcmc_run": (
-        [vp, i32, i32, i64, u64, i32, vp, vp, vp, vp, i32, i32, vp, vp, vp, i32, vp, vp,
-         C.c_double, C.c_double, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, vp,
-         vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp],
+        [vp, i32, i32, i64, u64, i32, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i32, vp,
+         vp, C.c_double, C.c_double, vp, vp, vp, vp, vp, i32, i32, vp, vp, vp, vp, vp, vp,
+         vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp],
         C.c_int,
     ),
     "kupsmi_verlet_create": ([vp], C.c_int),

@@ -11,9 +11,12 @@ between warm-up and production (keeping the reference's tracked-energy
 composition); ``analyze`` reduces the samples per host as
 application/mcmc/analysis.py does, replicas combined.
 
-One adsorbate species per deck (the kernel's rigid-molecule buffer holds one
-motif).  The arithmetic is all in the kernel; this module only assembles
-inputs and reads the per-cycle samples.
+Up to four adsorbate species per deck (a gas mixture: each host's
+composition and binary interaction parameters give the Peng-Robinson
+fugacity of every species, application/mcmc/data.py:380-391; the loading,
+heat of adsorption and initial insertions are per species).  The arithmetic
+is all in the kernel; this module only assembles inputs and reads the
+per-cycle samples.
 """
 
 from __future__ import annotations
@@ -63,23 +66,25 @@ class McmcRun:
     replicas: int
     structures: list
     energy: np.ndarray | None = None  # (n_cycles, n_chain) eV
-    loading: np.ndarray | None = None  # (n_cycles, n_chain)
+    loading: np.ndarray | None = None  # (n_cycles, n_chain), all species
+    loading_per_species: np.ndarray | None = None  # (n_cycles, n_chain, n_species)
     # (n_samples, n_chain) eV/A^3 when the deck sets compute_stress
     pressure: dict | None = None
 
 
 def build(config: McmcConfig, *, replicas: int = 1, seed: int | None = None,
           device=None) -> McmcRun:
-    if len(config.adsorbates) != 1:
-        raise ValueError("the rigid GCMC kernel runs one adsorbate species per deck")
-    ads = config.adsorbates[0]
-    motif = _motif(ads)
+    if not 1 <= len(config.adsorbates) <= 4:
+        raise ValueError("the rigid GCMC kernel runs 1..4 adsorbate species per deck")
+    motifs = [_motif(a) for a in config.adsorbates]
+    mt = len(motifs)
     structs = [host_structure(h) for h in config.hosts]
-    labels = [lab for s in structs for lab in s.labels] + list(motif.species)
+    labels = [lab for s in structs for lab in s.labels] + [x for m_ in motifs for x in m_.species]
     lj = lj_table(config.lj, labels)
     index = {lab: i for i, lab in enumerate(lj.labels)}
-    guest_species = tuple(index[sym] for sym in motif.species)
+    guest_species = [tuple(index[sym] for sym in m_.species) for m_ in motifs]
     pos, q, sp, sizes, lats, pres, temps, n_init = [], [], [], [], [], [], [], []
+    comp, kij = [], []
     for h, s in zip(config.hosts, structs):
         for _ in range(replicas):
             pos.append(s.positions)
@@ -89,16 +94,30 @@ def build(config: McmcConfig, *, replicas: int = 1, seed: int | None = None,
             lats.append(s.lattice)
             pres.append(float(h.pressure))
             temps.append(float(h.temperature))
-            n_init.append(int(h.init_adsorbates[0]) if h.init_adsorbates else 0)
+            ni = list(h.init_adsorbates or [])
+            n_init.append([int(ni[t]) if t < len(ni) else 0 for t in range(mt)])
+            y = np.asarray(h.adsorbate_composition if h.adsorbate_composition else
+                           [1.0 / mt] * mt, float)
+            if y.shape != (mt,):
+                raise ValueError("adsorbate_composition needs one mole fraction per adsorbate")
+            comp.append(y)
+            k = np.zeros((mt, mt)) if h.adsorbate_interaction is None else \
+                np.asarray(h.adsorbate_interaction, float)
+            if k.shape != (mt, mt):
+                raise ValueError("adsorbate_interaction must be n_ads x n_ads")
+            kij.append(k)
     run_seed = config.run.seed if seed is None else seed
     run_seed = 0 if run_seed is None else int(run_seed)
     gcmc = RigidGCMC(np.concatenate(pos), np.concatenate(q), np.concatenate(sp), sizes,
-                     np.stack(lats), np.array(pres), np.array(temps), motif, lj, guest_species,
+                     np.stack(lats), np.array(pres), np.array(temps),
+                     motifs[0] if mt == 1 else motifs, lj,
+                     guest_species[0] if mt == 1 else guest_species,
                      ewald_precision=config.ewald.precision, ewald_cutoff=config.ewald.real_cutoff,
                      weights=config.run.weights, gmax=int(config.max_num_adsorbates),
                      seed=run_seed, tail_correction=bool(config.lj.tail_correction),
-                     device=device)
-    if any(n_init):
+                     composition=np.array(comp), interaction=np.array(kij), device=device)
+    n_init = np.array(n_init, np.int64)
+    if n_init.any():
         gcmc.insert_random(n_init, seed=run_seed)
     return McmcRun(gcmc, config, replicas, structs)
 
@@ -123,17 +142,18 @@ def run(r: McmcRun, *, num_warmup_cycles: int | None = None,
         # their FP32 pair terms leave rounding in the running FP64 energy
         g.resync()
     e = torch.empty((n_prod, g.n_chain), dtype=torch.float64, device=g.dev)
-    n = torch.empty((n_prod, g.n_chain), dtype=torch.int32, device=g.dev)
+    n = torch.empty((n_prod, g.n_chain, g.n_motif), dtype=torch.int32, device=g.dev)
     terms = {"potential": [], "tail": [], "ideal": []}
     for c in range(n_prod):
         g.run(_cycle_length(g, cfg.min_cycle_length))
         e[c].copy_(g.d["energy"])
-        n[c].copy_(g.loading)
+        n[c].copy_(g.loading_per_motif)
         if r.config.compute_stress and c % stress_every == 0:
             for k, v in g.guest_pressure().items():
                 terms[k].append(v)
     r.energy = e.cpu().numpy()
-    r.loading = n.cpu().numpy()
+    r.loading_per_species = n.cpu().numpy()
+    r.loading = r.loading_per_species.sum(-1)
     if r.config.compute_stress and terms["potential"]:
         r.pressure = {k: np.stack(v) for k, v in terms.items()}
     return r
@@ -146,7 +166,8 @@ def analyze(r: McmcRun, n_blocks: int | None = None) -> list[MCMCAnalysisResult]
     out = []
     for h_i, h in enumerate(r.config.hosts):
         cols = range(h_i * r.replicas, (h_i + 1) * r.replicas)
-        res = [analyze_mcmc(r.energy[:, c], r.loading[:, c], float(h.temperature), n_blocks,
+        res = [analyze_mcmc(r.energy[:, c], r.loading_per_species[:, c], float(h.temperature),
+                            n_blocks,
                             None if r.pressure is None else
                             {k: v[:, c] for k, v in r.pressure.items()})
                for c in cols]

@@ -44,12 +44,14 @@ constexpr int GC_WAVES = GC_THREADS / 64;
 constexpr int GC_MSITE = 4;      // sites per molecule
 constexpr int GC_NM = 12;        // max |n| per axis of the k list
 constexpr int GC_NSPEC = 16;
+constexpr int GC_NMOTIF = 4;     // guest species (motifs) of a mixture
 constexpr int GC_KPT_MAX = 8;    // k vectors per thread (nk <= 4096)
 constexpr uint32_t GC_TAG_BATCH = 0x42415443u, GC_TAG_A = 0x47434d41u,
                    GC_TAG_B = 0x47434d42u, GC_TAG_C = 0x47434d43u;
 
 struct GcArgs {
-  int n_chain, n_steps, gmax, m, nspec, nk, nm, n_trace, max_host;
+  int n_chain, n_steps, gmax, m, n_motif, nspec, nk, nm, n_trace, max_host;
+  int mnsite[GC_NMOTIF];     // sites of each motif (<= m)
   int64_t step0;
   const int64_t* step_dev;  // when set, the first trial's counter is read here (graph replays)
   uint64_t seed;
@@ -59,21 +61,22 @@ struct GcArgs {
   const float4* host;        // x, y, z, q
   const int32_t* host_spec;
   const double* tril;
-  const float* motif_pos;    // m x 3, centred
-  const float* motif_q;
-  const int32_t* motif_spec;
+  const float* motif_pos;    // n_motif x m x 3, centred
+  const float* motif_q;      // n_motif x m
+  const int32_t* motif_spec; // n_motif x m
   const double* sigma;       // nspec x nspec
   const double* epsilon;
   const double* alpha;       // per chain
   const double* tail;        // n_chain x nspec x nspec
-  const double* mol_const;   // per chain: self + exclusion energy of one molecule
+  const double* mol_const;   // n_chain x n_motif: self + exclusion energy of one molecule
   const double* kT;
-  const double* log_activity;
+  const double* log_activity;  // n_chain x n_motif
   const int32_t* kn;         // n_chain x nk packed (n0, n1 + 64, n2 + 64)
   const double* pk;          // n_chain x nk, P(k) (0 for padding)
   // state (in/out)
   double* guest;             // n_chain x gmax x m x 3 (FP64: rigid bodies stay rigid)
   int32_t* n_guest;
+  int32_t* guest_motif;      // n_chain x gmax: motif of each guest slot
   int32_t* counts;           // n_chain x nspec
   double* sk;                // n_chain x nk x 2
   double* energy;
@@ -195,7 +198,12 @@ __device__ inline int gc_sample_cdf(const float* cdf, const int32_t* guide, int 
 
 // Scheduler and bookkeeping state of the chain being run (LDS).
 struct GcChain {
-  double kT, lnfV, mol_const, tc, mtm;
+  double kT;
+  // per motif t: ln(a_t V), molecule constant, 2 (T m_t).c without the 2, and
+  // m_u T m_t (T the symmetrised tail matrix, m_t the species counts of one
+  // molecule of t, c the chain's species counts)
+  double lnfV[GC_NMOTIF], molc[GC_NMOTIF], tcm[GC_NMOTIF], mtm[GC_NMOTIF][GC_NMOTIF];
+  int nmol[GC_NMOTIF];
   float wid[4];
   uint32_t hist[16];
   int hist_idx[4];
@@ -205,19 +213,25 @@ struct GcChain {
 
 // The trial formed by lane 0 and read by the workgroup.
 struct GcTrial {
-  int move, deleting, n_old, n_new, slot, voxel, accept;
+  int move, deleting, n_old, n_new, slot, voxel, accept, motif;
   double log_ratio;  // proposal log ratio ln(q(x|x') / q(x'|x)) of the move
 };
 
 // Proposal of one trial (lane 0 of wave 0): move type, sites before/after
 // the move, written to LDS as FP32 (x, y, z, q) for the pair loop and FP64 for
 // the phase tables and the guest table.  bb/ra/rb/rc are the batch, A, B and
-// C draws; g64 is the FP64 guest table.
+// C draws; g64 is the FP64 guest table, gmot the motif of each guest slot.
+// Translation, rotation and reinsertion pick a guest of any motif; an
+// insertion or a deletion first picks the motif uniformly (per chain, from
+// ra[2]; mcmc/moves.py:600-602, 711-712), a deletion then the ra[0] % N_t-th
+// guest of that motif (no-op when there is none).
 __device__ inline void gc_propose(const GcArgs& a, const GcCell& cell, const GcChain& cs, int ch,
-                                  int ng, const double* g64, const float4* motif,
-                                  const uint32_t bb[4], const uint32_t ra[4], const uint32_t rb[4],
-                                  const uint32_t rc[4], float4* site_old, float4* site_new,
-                                  double (*site64)[3], GcTrial& tr) {
+                                  int ng, const double* g64, const uint8_t* gmot,
+                                  const float4 (*motifs)[GC_MSITE], const int* mnsite,
+                                  const uint32_t bb[4],
+                                  const uint32_t ra[4], const uint32_t rb[4], const uint32_t rc[4],
+                                  float4* site_old, float4* site_new, double (*site64)[3],
+                                  GcTrial& tr) {
   const int M = a.m;
   const double wsum = a.w_move[0] + a.w_move[1] + a.w_move[2] + a.w_move[3];
   const double um = (double)u01(bb[0]) * wsum;
@@ -229,13 +243,23 @@ __device__ inline void gc_propose(const GcArgs& a, const GcCell& cell, const GcC
   }
   const int deleting = (bb[1] >> 31) & 1;
   int n_old = 0, n_new = 0, slot = -1, voxel = -1;
+  int t = move == 3 ? (int)(ra[2] % (uint32_t)a.n_motif) : 0;
   double log_ratio = 0.0;
   const double* old = g64;
-  if ((move < 3 || deleting) && ng > 0) {
+  if (move < 3 && ng > 0) {
     slot = (int)(ra[0] % (uint32_t)ng);
-    n_old = M;
+    t = gmot[slot];
+  } else if (move == 3 && deleting && cs.nmol[t] > 0) {
+    int k = (int)(ra[0] % (uint32_t)cs.nmol[t]);
+    for (int g = 0; g < ng; ++g)
+      if (gmot[g] == t && k-- == 0) { slot = g; break; }
+  }
+  const float4* motif = motifs[t];
+  const int Mt = mnsite[t];
+  if (slot >= 0) {
+    n_old = Mt;
     old = g64 + slot * M * 3;
-    for (int s = 0; s < M; ++s) {
+    for (int s = 0; s < Mt; ++s) {
       const double* p = old + 3 * s;
       site_old[s] = make_float4((float)p[0], (float)p[1], (float)p[2], motif[s].w);
       site64[s][0] = p[0]; site64[s][1] = p[1]; site64[s][2] = p[2];
@@ -245,12 +269,12 @@ __device__ inline void gc_propose(const GcArgs& a, const GcCell& cell, const GcC
   // geometric centre of the old molecule (minimum image relative to its last site)
   double com[3] = {0.0, 0.0, 0.0};
   if (n_old && (move == 1 || move == 2 || (move == 3 && a.prop_cdf))) {
-    const double* last = old + 3 * (M - 1);
+    const double* last = old + 3 * (Mt - 1);
     double cen[3] = {0.0, 0.0, 0.0};
-    for (int s = 0; s < M; ++s) {
+    for (int s = 0; s < Mt; ++s) {
       double rel[3];
       gc_minimg(cell, last, old + 3 * s, rel);
-      for (int x = 0; x < 3; ++x) cen[x] += rel[x] / M;
+      for (int x = 0; x < 3; ++x) cen[x] += rel[x] / Mt;
     }
     for (int x = 0; x < 3; ++x) com[x] = last[x] + cen[x];
   }
@@ -271,9 +295,9 @@ __device__ inline void gc_propose(const GcArgs& a, const GcCell& cell, const GcC
     sincospif(2.0f * u3, &s3, &c3);
     const double w = cs.wid[0];
     const double g3[3] = {(double)(r0 * c1) * w, (double)(r0 * s1) * w, (double)(r1 * c3) * w};
-    for (int s = 0; s < M; ++s)
+    for (int s = 0; s < Mt; ++s)
       for (int x = 0; x < 3; ++x) snew[s][x] = old[3 * s + x] + g3[x];
-    n_new = M;
+    n_new = Mt;
   } else if ((move == 1 || move == 2) && n_old) {
     const float uq[3] = {u01(rb[0]), u01(rb[1]), u01(rb[2])};
     double R[3][3];
@@ -282,13 +306,13 @@ __device__ inline void gc_propose(const GcArgs& a, const GcCell& cell, const GcC
     double off[3] = {0.0, 0.0, 0.0};
     if (move == 2)
       tmul(cell.L, (double)u01(rc[0]), (double)u01(rc[1]), (double)u01(rc[2]), off[0], off[1], off[2]);
-    for (int s = 0; s < M; ++s) {
+    for (int s = 0; s < Mt; ++s) {
       double rr[3];
       gc_minimg(cell, com, old + 3 * s, rr);
       for (int x = 0; x < 3; ++x)
         snew[s][x] = com[x] + rr[0] * R[0][x] + rr[1] * R[1][x] + rr[2] * R[2][x] + off[x];
     }
-    n_new = M;
+    n_new = Mt;
   } else if (move == 3 && !deleting) {
     const float uq[3] = {u01(rb[0]), u01(rb[1]), u01(rb[2])};
     double R[3][3];
@@ -311,12 +335,12 @@ __device__ inline void gc_propose(const GcArgs& a, const GcCell& cell, const GcC
     double off[3];
     tmul(cell.L, fu[0], fu[1], fu[2], off[0], off[1], off[2]);
     if (ng < a.gmax) {
-      for (int s = 0; s < M; ++s) {
+      for (int s = 0; s < Mt; ++s) {
         const double tp[3] = {motif[s].x, motif[s].y, motif[s].z};
         for (int x = 0; x < 3; ++x)
           snew[s][x] = tp[0] * R[0][x] + tp[1] * R[1][x] + tp[2] * R[2][x] + off[x];
       }
-      n_new = M;
+      n_new = Mt;
     }
   }
   for (int s = 0; s < n_new; ++s) {
@@ -324,7 +348,7 @@ __device__ inline void gc_propose(const GcArgs& a, const GcCell& cell, const GcC
     site_new[s] = make_float4((float)snew[s][0], (float)snew[s][1], (float)snew[s][2], motif[s].w);
   }
   tr.move = move; tr.deleting = deleting; tr.n_old = n_old; tr.n_new = n_new; tr.slot = slot;
-  tr.voxel = voxel; tr.log_ratio = log_ratio;
+  tr.voxel = voxel; tr.log_ratio = log_ratio; tr.motif = t;
 }
 
 template <int KPT>
@@ -334,8 +358,9 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
   __shared__ GcChain cs;
   __shared__ GcTrial tr;
   __shared__ float s_sig2[GC_NSPEC * GC_NSPEC], s_e4[GC_NSPEC * GC_NSPEC];
-  __shared__ float4 s_motif[GC_MSITE];
-  __shared__ int s_mspec[GC_MSITE];
+  __shared__ float4 s_motif[GC_NMOTIF][GC_MSITE];
+  __shared__ int s_mspec[GC_NMOTIF][GC_MSITE];
+  __shared__ int s_mn[GC_NMOTIF];  // sites per motif (indexed at run time: LDS, not arguments)
   __shared__ float4 site_old[GC_MSITE], site_new[GC_MSITE];
   __shared__ double site64[2 * GC_MSITE][3];  // FP64 old then new sites
   __shared__ double2 tab[2 * GC_MSITE][3][GC_NM + 1];
@@ -346,15 +371,18 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
   float4* gst = lds + a.max_host;                            // guests (x, y, z, q) FP32
   double* g64 = reinterpret_cast<double*>(gst + a.gmax * M);  // guests FP64 (state)
   uint8_t* hspec = reinterpret_cast<uint8_t*>(g64 + a.gmax * M * 3);
+  uint8_t* gmot = hspec + ((a.max_host + 15) & ~15);        // motif of each guest slot
   for (int k = tid; k < ns * ns; k += GC_THREADS) {
     const double sg = a.sigma[k];
     s_sig2[k] = (float)(sg * sg);
     s_e4[k] = (float)(4.0 * a.epsilon[k]);
   }
-  if (tid < M) {
-    s_motif[tid] = make_float4(a.motif_pos[3 * tid], a.motif_pos[3 * tid + 1],
-                               a.motif_pos[3 * tid + 2], a.motif_q[tid]);
-    s_mspec[tid] = a.motif_spec[tid];
+  if (tid < GC_NMOTIF) s_mn[tid] = a.mnsite[tid];
+  if (tid < a.n_motif * M) {
+    const int t = tid / M, s = tid % M;
+    s_motif[t][s] = make_float4(a.motif_pos[3 * tid], a.motif_pos[3 * tid + 1],
+                                a.motif_pos[3 * tid + 2], a.motif_q[tid]);
+    s_mspec[t][s] = a.motif_spec[tid];
   }
   const float rclj2 = (float)(a.rc_lj * a.rc_lj), rccl2 = (float)(a.rc_cl * a.rc_cl);
   const float K = 14.399645351950548f;
@@ -369,22 +397,40 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         cell.L[k] = t[k]; cell.Lf[k] = (float)t[k]; cell.Lif[k] = (float)cell.Li[k];
       }
       cs.kT = a.kT[ch];
-      cs.lnfV = a.log_activity[ch] + log(fabs(t[0] * t[2] * t[5]));
-      cs.mol_const = a.mol_const[ch];
-      // LJ tail: E(c) = c.T.c; with m the species counts of one molecule and
-      // T symmetrised, E(c + dn m) - E(c) = 2 dn (T m).c + m.T.m
-      const double* T = a.tail + (size_t)ch * ns * ns;
-      int mc[GC_NSPEC];
-      for (int x = 0; x < ns; ++x) { mc[x] = 0; cs.cnt[x] = a.counts[(size_t)ch * ns + x]; }
-      for (int s = 0; s < M; ++s) mc[a.motif_spec[s]] += 1;
-      double tc = 0.0, mtm = 0.0;
-      for (int y = 0; y < ns; ++y) {
-        double tv = 0.0;
-        for (int x = 0; x < ns; ++x) tv += 0.5 * (T[x * ns + y] + T[y * ns + x]) * mc[x];
-        tc += tv * cs.cnt[y];
-        mtm += tv * mc[y];
+      const double lnV = log(fabs(t[0] * t[2] * t[5]));
+      for (int u = 0; u < a.n_motif; ++u) {
+        cs.lnfV[u] = a.log_activity[(size_t)ch * a.n_motif + u] + lnV;
+        cs.molc[u] = a.mol_const[(size_t)ch * a.n_motif + u];
+        cs.nmol[u] = 0;
       }
-      cs.tc = tc; cs.mtm = mtm;
+      const int ng0 = a.n_guest[ch];
+      for (int g = 0; g < ng0; ++g) cs.nmol[a.guest_motif[(size_t)ch * a.gmax + g]] += 1;
+      // LJ tail: E(c) = c.T.c; with m_u the species counts of one molecule of
+      // motif u and T symmetrised, E(c + dn m_u) - E(c) = 2 dn (T m_u).c + m_u.T.m_u
+      const double* T = a.tail + (size_t)ch * ns * ns;
+      for (int x = 0; x < ns; ++x) cs.cnt[x] = a.counts[(size_t)ch * ns + x];
+      for (int u = 0; u < a.n_motif; ++u) {
+        // (T m_u)_y = sum over the sites s of u of T_sym[spec(s), y]
+        double tc = 0.0;
+        for (int y = 0; y < ns; ++y) {
+          double tv = 0.0;
+          for (int s = 0; s < s_mn[u]; ++s) {
+            const int x = s_mspec[u][s];
+            tv += 0.5 * (T[x * ns + y] + T[y * ns + x]);
+          }
+          tc += tv * cs.cnt[y];
+        }
+        cs.tcm[u] = tc;
+        for (int v = 0; v < a.n_motif; ++v) {
+          double mm = 0.0;
+          for (int s = 0; s < s_mn[u]; ++s)
+            for (int r = 0; r < s_mn[v]; ++r) {
+              const int x = s_mspec[u][s], y = s_mspec[v][r];
+              mm += 0.5 * (T[x * ns + y] + T[y * ns + x]);
+            }
+          cs.mtm[u][v] = mm;
+        }
+      }
       for (int k = 0; k < 4; ++k) {
         cs.wid[k] = a.width[(size_t)ch * 4 + k];
         cs.hist_idx[k] = a.hist_idx[(size_t)ch * 4 + k];
@@ -397,11 +443,13 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
       hspec[j] = (uint8_t)a.host_spec[h0 + j];
     }
     double* gpos = a.guest + (size_t)ch * a.gmax * M * 3;
+    int32_t* gmo = a.guest_motif + (size_t)ch * a.gmax;
     int ng = a.n_guest[ch];
+    for (int g = tid; g < ng; g += GC_THREADS) gmot[g] = (uint8_t)gmo[g];
     for (int j = tid; j < ng * M; j += GC_THREADS) {
       const double x = gpos[3 * j], y = gpos[3 * j + 1], z = gpos[3 * j + 2];
       g64[3 * j] = x; g64[3 * j + 1] = y; g64[3 * j + 2] = z;
-      gst[j] = make_float4((float)x, (float)y, (float)z, a.motif_q[j % M]);
+      gst[j] = make_float4((float)x, (float)y, (float)z, a.motif_q[gmo[j / M] * M + j % M]);
     }
     // this lane's k vectors: packed n, P(k), S(k)
     double* S = a.sk + (size_t)ch * a.nk * 2;
@@ -432,8 +480,10 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         const uint32_t tag = l == 0 ? GC_TAG_BATCH : (l == 1 ? GC_TAG_A : (l == 2 ? GC_TAG_B : GC_TAG_C));
         uint32_t w[4];
         gc_draw(a.seed, l == 0 ? 0xFFFFFFFFu : (uint32_t)ch, t, tag, w);
+        // ln u, and ln(N_t + 1), ln N_t of the exchange motif t (from ra[2])
+        const int nt = cs.nmol[gc_readlane(w[2], 1) % (uint32_t)a.n_motif];
         const double x = lane == 1 ? (double)u01(w[1])
-                                   : (lane == 2 ? (double)(ng + 1) : (double)(ng > 0 ? ng : 1));
+                                   : (lane == 2 ? (double)(nt + 1) : (double)(nt > 0 ? nt : 1));
         const double lx = log(x);
         uint32_t bb[4], ra[4], rb[4], rc[4];
 #pragma unroll
@@ -447,11 +497,11 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         ln_np1 = gc_readlane(lx, 2);
         ln_n = gc_readlane(lx, 3);
         if (lane == 0)
-          gc_propose(a, cell, cs, ch, ng, g64, s_motif, bb, ra, rb, rc, site_old, site_new, site64,
-                     tr);
+          gc_propose(a, cell, cs, ch, ng, g64, gmot, s_motif, s_mn, bb, ra, rb, rc, site_old,
+                     site_new, site64, tr);
       }
       __syncthreads();
-      const int n_old = tr.n_old, n_new = tr.n_new, slot = tr.slot;
+      const int n_old = tr.n_old, n_new = tr.n_new, slot = tr.slot, tm = tr.motif;
       const int nsite = n_old + n_new;
       // ---- 2a. FP64 axis phase tables of the changed sites, n = 0..nm
       if (tid < 3 * nsite) {
@@ -480,16 +530,17 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
             rj = host[j];
             sj = hspec[j];
           } else {
-            const int gj = j - nh;
-            if (gj / M == slot) continue;
+            const int gj = j - nh, g = gj / M, sg = gj - g * M;
+            const int gm = gmot[g];
+            if (g == slot || sg >= s_mn[gm]) continue;
             rj = gst[gj];
-            sj = s_mspec[gj % M];
+            sj = s_mspec[gm][sg];
           }
           const float rjv[3] = {rj.x, rj.y, rj.z};
           for (int s = 0; s < nsite; ++s) {
             const bool is_new = s >= n_old;
             const float4 ri = is_new ? site_new[s - n_old] : site_old[s];
-            const int si = s_mspec[is_new ? s - n_old : s];
+            const int si = s_mspec[tm][is_new ? s - n_old : s];
             const float riv[3] = {ri.x, ri.y, ri.z};
             float d[3];
             gc_minimg(cell, riv, rjv, d);
@@ -546,8 +597,8 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         int dn = 0;
         if (move == 3) dn = n_new ? 1 : (n_old ? -1 : 0);
         if (dn) {
-          dE += dn * cs.mol_const + 2.0 * dn * cs.tc + cs.mtm;
-          logp += dn > 0 ? cs.lnfV - ln_np1 : -cs.lnfV + ln_n;
+          dE += dn * cs.molc[tm] + 2.0 * dn * cs.tcm[tm] + cs.mtm[tm][tm];
+          logp += dn > 0 ? cs.lnfV[tm] - ln_np1 : -cs.lnfV[tm] + ln_n;
         }
         logp += -dE / cs.kT + tr.log_ratio;
         if (a.sample_count && move == 3 && n_new && tr.voxel >= 0) {
@@ -574,8 +625,9 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
           if (nsite) cs.stats[4 + move] += 1;  // moves that changed the configuration
           energy += dE;
           if (dn) {
-            for (int s = 0; s < M; ++s) cs.cnt[s_mspec[s]] += dn;
-            cs.tc += dn * cs.mtm;
+            for (int s = 0; s < s_mn[tm]; ++s) cs.cnt[s_mspec[tm][s]] += dn;
+            for (int u = 0; u < a.n_motif; ++u) cs.tcm[u] += dn * cs.mtm[u][tm];
+            cs.nmol[tm] += dn;
             cs.stats[dn > 0 ? 8 : 9] += 1;
           }
         }
@@ -613,13 +665,19 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
         if (tid < M) {
           const int d = dst * M + tid;
           if (n_new) {
-            gst[d] = site_new[tid];
-            for (int x = 0; x < 3; ++x) g64[3 * d + x] = site64[n_old + tid][x];
+            if (tid < n_new) {
+              gst[d] = site_new[tid];
+              for (int x = 0; x < 3; ++x) g64[3 * d + x] = site64[n_old + tid][x];
+            }
           } else {
             const int src = (ng - 1) * M + tid;
             gst[d] = gst[src];
             for (int x = 0; x < 3; ++x) g64[3 * d + x] = g64[3 * src + x];
           }
+        }
+        if (tid == 0) {
+          if (n_new && !n_old) gmot[ng] = (uint8_t)tm;
+          else if (!n_new) gmot[slot] = gmot[ng - 1];
         }
         ng += n_new && !n_old ? 1 : (!n_new ? -1 : 0);
       }
@@ -632,6 +690,7 @@ __global__ __launch_bounds__(GC_THREADS) void gcmc_kernel(GcArgs a) {
       if (k < a.nk) { S[2 * k] = rS[q].x; S[2 * k + 1] = rS[q].y; }
     }
     for (int j = tid; j < ng * M * 3; j += GC_THREADS) gpos[j] = g64[j];
+    for (int g = tid; g < ng; g += GC_THREADS) gmo[g] = gmot[g];
     if (tid == 0) {
       a.n_guest[ch] = ng;
       a.energy[ch] = energy;
@@ -667,7 +726,8 @@ using namespace kupsmi;
 extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t step0,
                                uint64_t seed, int max_host, const int32_t* host_off,
                                const float* host_xyzq, const int32_t* host_species,
-                               const double* tril, int gmax, int m, const float* motif_pos,
+                               const double* tril, int gmax, int m, int n_motif,
+                               const int32_t* motif_nsite, const float* motif_pos,
                                const float* motif_q, const int32_t* motif_species, int n_species,
                                const double* sigma, const double* epsilon, double rc_lj,
                                double rc_coul, const double* alpha, const double* tail,
@@ -675,7 +735,8 @@ extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t s
                                const double* log_activity, int nk, int nm, const int32_t* kn,
                                const double* pk, const double* move_weights,
                                const double* width_upper, double* guest, int32_t* n_guest,
-                               int32_t* counts, double* structure_factor, double* energy,
+                               int32_t* guest_motif, int32_t* counts,
+                               double* structure_factor, double* energy,
                                float* width, uint32_t* hist, int32_t* hist_idx,
                                long long* stats, int n_trace, double* trace,
                                const float* prop_cdf, const int32_t* prop_guide, int prop_nv,
@@ -683,16 +744,24 @@ extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t s
                                float* sample_y, const int64_t* step_dev) {
   KUPSMI_REQUIRE(n_chain >= 0 && n_steps >= 0, "negative size");
   KUPSMI_REQUIRE(m >= 1 && m <= GC_MSITE, "molecules must have 1..4 sites");
+  KUPSMI_REQUIRE(n_motif >= 1 && n_motif <= GC_NMOTIF, "1..4 guest motifs");
+  KUPSMI_REQUIRE(motif_nsite != nullptr, "sites per motif required (host array)");
+  for (int t = 0; t < n_motif; ++t)
+    KUPSMI_REQUIRE(motif_nsite[t] >= 1 && motif_nsite[t] <= m, "a motif has 1..m sites");
+  KUPSMI_REQUIRE(guest_motif != nullptr, "guest motif table required");
   KUPSMI_REQUIRE(n_species >= 1 && n_species <= GC_NSPEC, "n_species must be in [1, 16]");
   KUPSMI_REQUIRE(nk >= 0 && nk <= GC_KPT_MAX * GC_THREADS, "at most 4096 k vectors per chain");
   KUPSMI_REQUIRE(nm >= 0 && nm <= GC_NM, "k list half width must be <= 12");
   KUPSMI_REQUIRE(gmax >= 1 && max_host >= 0, "bad buffer sizes");
   const size_t lds = sizeof(float4) * ((size_t)max_host + (size_t)gmax * m) +
-                     sizeof(double) * 3 * (size_t)gmax * m + (((size_t)max_host + 15) & ~(size_t)15);
+                     sizeof(double) * 3 * (size_t)gmax * m + (((size_t)max_host + 15) & ~(size_t)15) +
+                     (size_t)gmax;
   KUPSMI_REQUIRE(lds <= 120 * 1024, "host + guest buffer exceeds the LDS budget of a chain");
   if (n_chain == 0 || n_steps == 0) return KUPSMI_OK;
   GcArgs a{};
   a.n_chain = n_chain; a.n_steps = n_steps; a.gmax = gmax; a.m = m; a.nspec = n_species;
+  a.n_motif = n_motif;
+  for (int t = 0; t < GC_NMOTIF; ++t) a.mnsite[t] = t < n_motif ? motif_nsite[t] : 0;
   a.nk = nk; a.nm = nm; a.n_trace = n_trace; a.max_host = max_host;
   a.step0 = step0; a.step_dev = step_dev; a.seed = seed; a.rc_lj = rc_lj; a.rc_cl = rc_coul;
   for (int k = 0; k < 4; ++k) {
@@ -703,7 +772,8 @@ extern "C" int kupsmi_gcmc_run(void* stream, int n_chain, int n_steps, int64_t s
   a.host_spec = host_species; a.tril = tril; a.motif_pos = motif_pos; a.motif_q = motif_q;
   a.motif_spec = motif_species; a.sigma = sigma; a.epsilon = epsilon; a.alpha = alpha;
   a.tail = tail; a.mol_const = mol_const; a.kT = kT; a.log_activity = log_activity;
-  a.kn = kn; a.pk = pk; a.guest = guest; a.n_guest = n_guest; a.counts = counts;
+  a.kn = kn; a.pk = pk; a.guest = guest; a.n_guest = n_guest; a.guest_motif = guest_motif;
+  a.counts = counts;
   a.sk = structure_factor; a.energy = energy; a.width = width; a.hist = hist;
   a.hist_idx = hist_idx; a.stats = stats; a.trace = trace;
   KUPSMI_REQUIRE(!prop_cdf || (prop_nv >= 1 && prop_guide),

@@ -12,7 +12,7 @@ constexpr int PC_NMAX = 16384;
 constexpr int PC_MAX_NC_AXIS = 16;
 constexpr int PC_NCMAX = PC_MAX_NC_AXIS * PC_MAX_NC_AXIS * PC_MAX_NC_AXIS;
 constexpr int PC_TILE = 256;
-constexpr int PC_NSPEC = 8;
+constexpr int PC_NSPEC = 16;
 // relative half width of the band around each cutoff whose pairs are
 // re-decided on the caller's positions: it covers the rounding of a wrapped
 // copy (half an FP32 ulp per coordinate, ~6e-7 of rc^2 at rc = 12 A in a

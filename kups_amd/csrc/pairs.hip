@@ -224,7 +224,7 @@ extern "C" int kupsmi_pair_forces(kupsmi_ctx* ctx, void* stream, int n_sys, cons
   ctx->rec_armed = 0;
   KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0, "negative size");
   KUPSMI_REQUIRE(max_atoms_per_sys <= PC_NMAX, "systems of the pair path hold at most 16384 atoms");
-  KUPSMI_REQUIRE(n_species >= 1 && n_species <= PC_NSPEC, "n_species must be in [1, 8]");
+  KUPSMI_REQUIRE(n_species >= 1 && n_species <= PC_NSPEC, "n_species must be in [1, 16]");
   const bool has_lj = sigma && epsilon && lj_cutoff;
   const bool has_cl = alpha && coul_cutoff && charges;
   KUPSMI_REQUIRE(has_lj || has_cl, "nothing to compute: pass LJ tables or Ewald parameters");

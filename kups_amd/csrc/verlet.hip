@@ -909,7 +909,7 @@ extern "C" int kupsmi_pair_forces_verlet(
   KUPSMI_REQUIRE(v != nullptr, "null handle");
   KUPSMI_REQUIRE(n_sys >= 0 && n_atoms >= 0, "negative size");
   KUPSMI_REQUIRE(max_atoms_per_sys <= PC_NMAX, "systems of the pair path hold at most 16384 atoms");
-  KUPSMI_REQUIRE(n_species >= 1 && n_species <= PC_NSPEC, "n_species must be in [1, 8]");
+  KUPSMI_REQUIRE(n_species >= 1 && n_species <= PC_NSPEC, "n_species must be in [1, 16]");
   KUPSMI_REQUIRE(skin >= 0.0, "skin must be non-negative");
   KUPSMI_REQUIRE(max_neighbors >= 1, "max_neighbors must be positive");
   const bool has_lj = sigma && epsilon && lj_cutoff;

@@ -1,10 +1,10 @@
 """Rigid-molecule GCMC driver (application/simulations/mcmc_rigid.py).
 
 ``RigidGCMC`` builds the state of kups_mcmc_rigid for a batch of chains (hosts
-with the reference's exclusion semantics, one guest species, TraPPE LJ with
-Lorentz-Berthelot mixing and the global tail correction, Ewald with the
-reference's estimator, Peng-Robinson fugacity) and advances all chains with
-the LDS-resident HIP kernel ``kupsmi_gcmc_run``.  Set-up that the reference
+with the reference's exclusion semantics, one or several guest species,
+TraPPE LJ with Lorentz-Berthelot mixing and the global tail correction, Ewald
+with the reference's estimator, Peng-Robinson mixture fugacities) and
+advances all chains with the LDS-resident HIP kernel ``kupsmi_gcmc_run``.  Set-up that the reference
 does on the host (estimator, k vectors, fugacity, tail matrix) is done on the
 host here too; the initial structure factors and energies come from the GPU
 Ewald and pair kernels.
@@ -82,16 +82,36 @@ def random_rotation(rng: np.random.Generator) -> np.ndarray:
 
 
 class RigidGCMC:
-    """GCMC of one rigid guest species in a batch of rigid hosts."""
+    """GCMC of rigid guest species in a batch of rigid hosts.
+
+    ``motif`` is one Motif or a sequence of them (a mixture, at most 4
+    species of at most 4 sites), ``guest_species`` the LJ species of each
+    site (a tuple, or one tuple per motif); ``composition`` the gas-phase
+    mole fractions and ``interaction`` the Peng-Robinson binary interaction
+    parameters of a mixture (application/mcmc/data.py:122-125, 380-391)."""
 
     def __init__(self, host_positions, host_charges, host_species, host_sizes, lattices,
-                 pressure, temperature, motif: Motif, lj: LennardJonesParameters,
-                 guest_species: tuple[int, ...], *, ewald_precision: float = 1e-6,
+                 pressure, temperature, motif, lj: LennardJonesParameters,
+                 guest_species, *, ewald_precision: float = 1e-6,
                  ewald_cutoff: float = 12.0, weights=(1.0, 1.0, 1.0, 3.0), gmax: int = 64,
-                 seed: int = 0, tail_correction: bool = True, device=None):
+                 seed: int = 0, tail_correction: bool = True, composition=None,
+                 interaction=None, device=None):
         dev = device or _lib.require_gpu()
         self.dev = dev
-        self.motif = motif
+        motifs = [motif] if isinstance(motif, Motif) else list(motif)
+        if isinstance(motif, Motif):
+            guest_species = [tuple(guest_species)]
+        guest_species = [tuple(int(x) for x in g) for g in guest_species]
+        if not 1 <= len(motifs) <= 4 or len(guest_species) != len(motifs):
+            raise ValueError("1..4 motifs, with the LJ species of each one's sites")
+        for m_, g_ in zip(motifs, guest_species):
+            if len(g_) != len(m_.charges) or not 1 <= len(m_.charges) <= 4:
+                raise ValueError("a motif has 1..4 sites, each with an LJ species")
+        self.motifs = motifs
+        self.motif = motifs[0]
+        self.n_motif = len(motifs)
+        self.motif_nsite = np.array([len(m_.charges) for m_ in motifs], np.int32)
+        self.m = int(self.motif_nsite.max())
         self.lj = lj
         self.seed = int(seed)
         self.step = 0
@@ -107,16 +127,26 @@ class RigidGCMC:
         self.volume = vol
         self.temperature = np.broadcast_to(np.asarray(temperature, float), (n,)).copy()
         kT = self.temperature * BOLTZMANN_CONSTANT
-        # fugacity (mcmc/data.py:286-290): ln a = ln f - ln kT, f in eV / A^3
-        lnf = np.array([peng_robinson_log_fugacity(p, t, motif.critical_pressure,
-                                                   motif.critical_temperature,
-                                                   motif.acentric_factor)[0][0]
-                        for p, t in zip(np.broadcast_to(pressure, (n,)), self.temperature)])
-        self.log_activity = lnf + math.log(PASCAL) - np.log(kT)
+        # fugacity (mcmc/data.py:286-290, 380-391): ln a_t = ln f_t - ln kT, f in
+        # eV / A^3, f_t from the Peng-Robinson mixture at the gas composition
+        # (composition and interaction may also be given per chain)
+        mt = self.n_motif
+        frac = np.broadcast_to(np.full(mt, 1.0 / mt) if composition is None else
+                               np.asarray(composition, float), (n, mt))
+        if np.any(np.abs(frac.sum(1) - 1) > 1e-9) or np.any(frac < 0):
+            raise ValueError("composition: one mole fraction per motif, summing to 1")
+        kij = np.broadcast_to(np.zeros((mt, mt)) if interaction is None else
+                              np.asarray(interaction, float), (n, mt, mt))
+        crit = ([m_.critical_pressure for m_ in motifs], [m_.critical_temperature for m_ in motifs],
+                [m_.acentric_factor for m_ in motifs])
+        lnf = np.array([peng_robinson_log_fugacity(p, t, *crit, frac[s], kij[s])[0]
+                        for s, (p, t) in enumerate(zip(np.broadcast_to(pressure, (n,)),
+                                                       self.temperature))]).reshape(n, -1)
+        self.log_activity = lnf + math.log(PASCAL) - np.log(kT)[:, None]
         # Ewald: fixed real-space cutoff -> alpha and k_max (ewald.py:1254-1277)
         hq = np.asarray(host_charges, float)
         off = np.concatenate([[0], np.cumsum(sizes)])
-        est = [estimate_ewald_parameters(hq[off[s]:off[s + 1]] if sizes[s] else motif.charges,
+        est = [estimate_ewald_parameters(hq[off[s]:off[s + 1]] if sizes[s] else motifs[0].charges,
                                          tril[s], real_cutoff=ewald_cutoff,
                                          epsilon_total=ewald_precision) for s in range(n)]
         self.alpha = np.array([e["alpha"] for e in est])
@@ -145,8 +175,9 @@ class RigidGCMC:
         self._kgrid_index = gidx
         # species: host types then guest types
         self.n_species = lj.n_species
-        self.guest_species = np.asarray(guest_species, np.int32)
-        self.mol_const = np.array([molecule_constant(motif, a) for a in self.alpha])
+        self.guest_species_per_motif = [np.asarray(g_, np.int32) for g_ in guest_species]
+        self.guest_species = self.guest_species_per_motif[0]
+        self.mol_const = np.array([[molecule_constant(m_, a) for m_ in motifs] for a in self.alpha])
         self.tail = np.stack([tail_matrix(lj, v) for v in vol])
         self.tail_correction = bool(tail_correction)
         if not tail_correction:
@@ -160,17 +191,26 @@ class RigidGCMC:
         self.upper = np.array([self._min_half_box(lat), 1e30, 1e30, 1e30])
         t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
         f32, f64, i32 = torch.float32, torch.float64, torch.int32
+        M = self.m
+        mpos = np.zeros((self.n_motif, M, 3))
+        mq = np.zeros((self.n_motif, M))
+        msp = np.zeros((self.n_motif, M), np.int32)
+        for k, (m_, g_) in enumerate(zip(motifs, self.guest_species_per_motif)):
+            mpos[k, :len(g_)] = m_.positions
+            mq[k, :len(g_)] = m_.charges
+            msp[k, :len(g_)] = g_
         self.d = dict(
             host_off=t(off.astype(np.int32), i32),
             host=t(np.concatenate([np.asarray(host_positions, float), hq[:, None]], 1), f32),
             host_species=t(hs, i32), tril=t(tril, f64),
-            motif_pos=t(motif.positions, f32), motif_q=t(motif.charges, f32),
-            motif_species=t(self.guest_species, i32), sigma=t(lj.sigma, f64),
+            motif_pos=t(mpos, f32), motif_q=t(mq, f32),
+            motif_species=t(msp, i32), sigma=t(lj.sigma, f64),
             epsilon=t(lj.epsilon, f64), alpha=t(self.alpha, f64), tail=t(self.tail, f64),
             mol_const=t(self.mol_const, f64), kT=t(kT, f64), log_activity=t(self.log_activity, f64),
             kn=t(kn, i32), pk=t(pk, f64),
-            guest=torch.zeros((n, self.gmax, len(motif.charges), 3), dtype=f64, device=dev),
-            n_guest=torch.zeros(n, dtype=i32, device=dev), counts=t(counts, i32),
+            guest=torch.zeros((n, self.gmax, M, 3), dtype=f64, device=dev),
+            n_guest=torch.zeros(n, dtype=i32, device=dev),
+            guest_motif=torch.zeros((n, self.gmax), dtype=i32, device=dev), counts=t(counts, i32),
             sk=torch.zeros((n, nk, 2), dtype=f64, device=dev),
             energy=torch.zeros(n, dtype=f64, device=dev),
             width=torch.full((n, 4), 0.1, dtype=f32, device=dev),
@@ -181,6 +221,7 @@ class RigidGCMC:
         # move weights and width bounds are host arrays of the entry point
         self._w_host = np.ascontiguousarray(self.weights, dtype=np.float64)
         self._u_host = np.ascontiguousarray(self.upper, dtype=np.float64)
+        self._n_host = np.ascontiguousarray(self.motif_nsite, dtype=np.int32)
         self.max_host = int(sizes.max()) if n else 0
         self.recompute_state()
 
@@ -192,17 +233,27 @@ class RigidGCMC:
                    for L in lat) / 2.0
 
     def sr_intramolecular(self) -> np.ndarray:
-        """E_sr over the intramolecular pairs of one guest, per chain.  The
-        reference's full evaluation contains it for every molecule present at
-        that time, its incremental Delta E does not."""
+        """E_sr over the intramolecular pairs of one guest of each motif,
+        (n_chain, n_motif).  The reference's full evaluation contains it for
+        every molecule present at that time, its incremental Delta E does not."""
         from scipy.special import erfc
 
-        q, p = self.motif.charges, self.motif.positions
-        out = np.zeros(self.n_chain)
-        for i in range(len(q)):
-            for j in range(i + 1, len(q)):
-                r = float(np.linalg.norm(p[i] - p[j]))
-                out += TO_STANDARD_UNITS * q[i] * q[j] * erfc(self.alpha * r) / r
+        out = np.zeros((self.n_chain, self.n_motif))
+        for t, m_ in enumerate(self.motifs):
+            q, p = m_.charges, m_.positions
+            for i in range(len(q)):
+                for j in range(i + 1, len(q)):
+                    r = float(np.linalg.norm(p[i] - p[j]))
+                    out[:, t] += TO_STANDARD_UNITS * q[i] * q[j] * erfc(self.alpha * r) / r
+        return out
+
+    def motif_counts(self) -> np.ndarray:
+        """Guests of each motif in every chain, (n_chain, n_motif)."""
+        ng = self.d["n_guest"].cpu().numpy()
+        gm = self.d["guest_motif"].cpu().numpy()
+        out = np.zeros((self.n_chain, self.n_motif), np.int64)
+        for s in range(self.n_chain):
+            out[s] = np.bincount(gm[s, :ng[s]], minlength=self.n_motif)
         return out
 
     # ---- full evaluation (initial energies, structure factors, checks)
@@ -210,20 +261,24 @@ class RigidGCMC:
         """Host + guest atoms of every chain as a ParticleBatch (chain-sorted)."""
         d = self.d
         n = self.n_chain
-        M = len(self.motif.charges)
         ng = d["n_guest"].cpu().numpy()
+        gm = d["guest_motif"].cpu().numpy()
         host = d["host"].cpu().numpy()
         hoff = d["host_off"].cpu().numpy()
         hspec = d["host_species"].cpu().numpy()
         gpos = d["guest"].cpu().numpy()
+        mq = [m_.charges for m_ in self.motifs]
         pos, q, sp, grp, sizes = [], [], [], [], []
         for s in range(n):
             h = host[hoff[s]:hoff[s + 1]]
-            g = gpos[s, :ng[s]].reshape(-1, 3)
+            ms = gm[s, :ng[s]]
+            ns = self.motif_nsite[ms]
+            g = np.concatenate([gpos[s, k, :ns[k]] for k in range(ng[s])]) if ng[s] else \
+                np.zeros((0, 3))
             pos += [h[:, :3], g]
-            q += [h[:, 3], np.tile(self.motif.charges, ng[s])]
-            sp += [hspec[hoff[s]:hoff[s + 1]], np.tile(self.guest_species, ng[s])]
-            grp += [np.full(len(h), -1), np.repeat(np.arange(ng[s]), M)]
+            q += [h[:, 3]] + [mq[t] for t in ms]
+            sp += [hspec[hoff[s]:hoff[s + 1]]] + [self.guest_species_per_motif[t] for t in ms]
+            grp += [np.full(len(h), -1), np.repeat(np.arange(ng[s]), ns)]
             sizes.append(len(h) + len(g))
         return (np.concatenate(pos), np.concatenate(q), np.concatenate(sp).astype(np.int32),
                 np.concatenate(grp).astype(np.int32), np.asarray(sizes))
@@ -263,25 +318,26 @@ class RigidGCMC:
             _lib.ptr(pb.positions), None, _lib.ptr(pb.species), self.n_species, _lib.ptr(sig),
             _lib.ptr(eps), _lib.ptr(rc), _lib.ptr(excl_t), None, None, None, _lib.ptr(sb.tril), 0,
             _lib.ptr(f), _lib.ptr(e2), None, None))
-        ng = self.d["n_guest"].cpu().numpy()
+        nt = self.motif_counts()
         counts = self.d["counts"].cpu().numpy().astype(float)
         e_tail = np.einsum("sab,sa,sb->s", self.tail, counts, counts)
         # intramolecular exclusion correction of every guest molecule; the
         # atomic view above already holds E_sr(intra) of each guest, as the
         # reference's full evaluation does (ewald.py:946-967 over all pairs)
-        e_excl = ng * (self.mol_const - self._self_mol())
+        e_excl = (nt * (self.mol_const - self._self_mol())).sum(1)
         return (out.short_range + out.long_range + out.self_energy).cpu().numpy() + e_excl + \
             e2[:, 0].cpu().numpy() + e_tail
 
     def _self_mol(self):
-        q = self.motif.charges
-        return -TO_STANDARD_UNITS * self.alpha / math.sqrt(math.pi) * float(q @ q)
+        """Ewald self energy of one guest of each motif, (n_chain, n_motif)."""
+        qq = np.array([float(m_.charges @ m_.charges) for m_ in self.motifs])
+        return -TO_STANDARD_UNITS * self.alpha[:, None] / math.sqrt(math.pi) * qq[None, :]
 
     def recompute_state(self):
         """Structure factors of the current configuration (GPU) and the tracked
         energy set to the full evaluation (the composition origin moves to the
         current loading)."""
-        self._ng_ref = self.d["n_guest"].cpu().numpy().copy()
+        self._ng_ref = self.motif_counts()
         if self._structure_factors():
             self.d["energy"].copy_(torch.as_tensor(self.full_energy(), device=self.dev))
 
@@ -293,9 +349,9 @@ class RigidGCMC:
         the rounding that FP32 pair terms leave in the FP64 running energy,
         which matters after a start from overlapping molecules, where the
         early energies are astronomically large."""
-        ng = self.d["n_guest"].cpu().numpy()
+        nt = self.motif_counts()
         if self._structure_factors():
-            e = self.full_energy() - (ng - self._ng_ref) * self.sr_intramolecular()
+            e = self.full_energy() - ((nt - self._ng_ref) * self.sr_intramolecular()).sum(1)
             self.d["energy"].copy_(torch.as_tensor(e, device=self.dev))
 
     def _structure_factors(self) -> bool:
@@ -327,32 +383,43 @@ class RigidGCMC:
         evaluation holds E_sr(intra) of every guest, the incremental Delta E of
         the kernel does not, so guests added since the last full evaluation are
         accounted for."""
-        ng = self.d["n_guest"].cpu().numpy()
-        full = self.full_energy() - (ng - self._ng_ref) * self.sr_intramolecular()
+        nt = self.motif_counts()
+        full = self.full_energy() - ((nt - self._ng_ref) * self.sr_intramolecular()).sum(1)
         return self.d["energy"].cpu().numpy() - full
 
     def insert_random(self, n_init, seed: int = 0):
-        """Place n_init[s] molecules in chain s, each at a uniform fractional
-        centre with a uniformly random orientation and no overlap test
-        (application/mcmc/data.py:293-340), then rebuild S(k) and the tracked
-        energies from a full evaluation."""
-        n_init = np.broadcast_to(np.asarray(n_init, np.int64), (self.n_chain,))
+        """Place n_init[s] molecules in chain s (n_init[s, t] of motif t when
+        given per motif), each at a uniform fractional centre with a uniformly
+        random orientation and no overlap test (application/mcmc/data.py:
+        293-340), then rebuild S(k) and the tracked energies from a full
+        evaluation."""
+        n_init = np.asarray(n_init, np.int64)
+        if n_init.ndim < 2:
+            n_init = np.stack([np.broadcast_to(n_init, (self.n_chain,))] +
+                              [np.zeros(self.n_chain, np.int64)] * (self.n_motif - 1), 1)
+        n_init = np.broadcast_to(n_init, (self.n_chain, self.n_motif))
         ng = self.d["n_guest"].cpu().numpy()
-        if np.any(ng + n_init > self.gmax):
+        if np.any(ng + n_init.sum(1) > self.gmax):
             raise ValueError(f"initial loading exceeds the {self.gmax}-molecule buffer")
         rng = np.random.default_rng(seed)
         guest = self.d["guest"].cpu().numpy()
+        gm = self.d["guest_motif"].cpu().numpy()
         counts = self.d["counts"].cpu().numpy()
         lat = np.zeros((self.n_chain, 3, 3))
         tr = self.d["tril"].cpu().numpy()
         lat[:, [0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]] = tr
         for s in range(self.n_chain):
-            for _ in range(int(n_init[s])):
-                com = rng.uniform(-0.5, 0.5, 3) @ lat[s]
-                guest[s, ng[s]] = com + self.motif.positions @ random_rotation(rng).T
-                ng[s] += 1
-                np.add.at(counts[s], self.guest_species, 1)
+            for t, m_ in enumerate(self.motifs):
+                k = len(m_.charges)
+                for _ in range(int(n_init[s, t])):
+                    com = rng.uniform(-0.5, 0.5, 3) @ lat[s]
+                    guest[s, ng[s]] = 0.0
+                    guest[s, ng[s], :k] = com + m_.positions @ random_rotation(rng).T
+                    gm[s, ng[s]] = t
+                    ng[s] += 1
+                    np.add.at(counts[s], self.guest_species_per_motif[t], 1)
         self.d["guest"].copy_(torch.as_tensor(guest))
+        self.d["guest_motif"].copy_(torch.as_tensor(gm))
         self.d["n_guest"].copy_(torch.as_tensor(ng))
         self.d["counts"].copy_(torch.as_tensor(counts))
         self.recompute_state()
@@ -376,7 +443,6 @@ class RigidGCMC:
         d = self.d
         dev = self.dev
         n = self.n_chain
-        M = len(self.motif.charges)
         ng = d["n_guest"].long()
         vol = torch.as_tensor(self.volume, dtype=torch.float64, device=dev)
         kT = torch.as_tensor(self.temperature * BOLTZMANN_CONSTANT, dtype=torch.float64, device=dev)
@@ -391,7 +457,10 @@ class RigidGCMC:
         # guest-only configuration, chain-sorted, molecules contiguous
         keep = torch.arange(self.gmax, device=dev)[None, :] < ng[:, None]          # (n, gmax)
         g = d["guest"][keep]                                                      # (mols, M, 3)
+        mot = d["guest_motif"][keep].long()                                       # (mols,)
         chain_of_mol = torch.arange(n, device=dev)[:, None].expand(n, self.gmax)[keep]
+        nsite = torch.as_tensor(self.motif_nsite, device=dev).long()[mot]         # (mols,)
+        real = torch.arange(self.m, device=dev)[None, :] < nsite[:, None]         # (mols, M)
         tril = d["tril"]
         L = torch.zeros((n, 3, 3), dtype=torch.float64, device=dev)
         L[:, [0, 1, 1, 2, 2, 2], [0, 0, 1, 0, 1, 2]] = tril
@@ -401,15 +470,20 @@ class RigidGCMC:
         rel = g - g[:, :1]
         f = torch.einsum("mkc,mcd->mkd", rel, Lim)
         rel = torch.einsum("mkc,mcd->mkd", f - torch.round(f), Lm)
-        dvec = rel - rel.mean(dim=1, keepdim=True)                                # (mols, M, 3)
-        pos = g.reshape(-1, 3)
-        sizes = (ng * M).cpu().numpy()
+        rel = torch.where(real[:, :, None], rel, 0.0)
+        centre = rel.sum(1, keepdim=True) / nsite[:, None, None].double()
+        dvec = (rel - centre)[real]                                               # (atoms, 3)
+        pos = g[real]
+        atoms = pos.shape[0]
+        sizes = segment_sum(nsite.double(), chain_of_mol, n).round().long().cpu().numpy()
         off = offsets_from_sizes(sizes, dev)
-        q = torch.as_tensor(self.motif.charges, dtype=torch.float32, device=dev).repeat(g.shape[0])
-        sp = torch.as_tensor(self.guest_species, dtype=torch.int32, device=dev).repeat(g.shape[0])
-        mol_id = torch.arange(g.shape[0], device=dev, dtype=torch.int32).repeat_interleave(M)
+        q = d["motif_q"][mot][real].contiguous()
+        sp = d["motif_species"][mot][real].contiguous()
+        mol_id = torch.arange(g.shape[0], device=dev, dtype=torch.int32)[:, None] \
+            .expand(-1, self.m)[real].contiguous()
+        chain_of_atom = chain_of_mol[:, None].expand(-1, self.m)[real]
         p32 = pos.to(torch.float32).contiguous()
-        forces = torch.zeros((total * M, 3), dtype=torch.float32, device=dev)
+        forces = torch.zeros((atoms, 3), dtype=torch.float32, device=dev)
         e2 = torch.zeros((n, 2), dtype=torch.float64, device=dev)
         vir = torch.zeros((n, 3, 3), dtype=torch.float64, device=dev)
         alpha = torch.as_tensor(self.alpha, dtype=torch.float64, device=dev)
@@ -420,7 +494,7 @@ class RigidGCMC:
         lib = _lib.load()
         maxn = int(sizes.max())
         _lib.check(lib.kupsmi_pair_forces(
-            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), total * M, maxn, _lib.ptr(p32),
+            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), atoms, maxn, _lib.ptr(p32),
             _lib.ptr(q), _lib.ptr(sp), self.n_species, _lib.ptr(sig), _lib.ptr(eps), _lib.ptr(rc_lj),
             _lib.ptr(mol_id), _lib.ptr(alpha), _lib.ptr(rc_c), _lib.ptr(mol_id), _lib.ptr(tril), 0,
             _lib.ptr(forces), _lib.ptr(e2), _lib.ptr(vir), None))
@@ -428,16 +502,16 @@ class RigidGCMC:
         elr = torch.zeros(n, dtype=torch.float64, device=dev)
         kmax = torch.as_tensor(self.kmax, dtype=torch.float64, device=dev)
         _lib.check(lib.kupsmi_ewald_recip(
-            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), total * M, maxn, _lib.ptr(p32), _lib.ptr(q),
+            _lib.ctx(), _lib.stream_ptr(), n, _lib.ptr(off), atoms, maxn, _lib.ptr(p32), _lib.ptr(q),
             _lib.ptr(tril), _lib.ptr(alpha), _lib.ptr(kmax), nm, nm, nm, 1, _lib.ptr(forces),
             _lib.ptr(elr), _lib.ptr(vir), None))
-        dF = (dvec.reshape(-1, 3) * forces.double()).sum(-1)                      # (atoms,)
-        corr = segment_sum(dF, chain_of_mol.repeat_interleave(M), n)
+        dF = (dvec * forces.double()).sum(-1)                                     # (atoms,)
+        corr = segment_sum(dF, chain_of_atom, n)
         tr = torch.diagonal(vir, dim1=-2, dim2=-1).sum(-1)
         pot = ((tr - corr) / (3.0 * vol)).cpu().numpy()
         return {"potential": pot, "tail": tail, "ideal": ideal}
 
-    _STATE = ("guest", "n_guest", "counts", "sk", "energy", "width", "hist", "hist_idx", "stats")
+    _STATE = ("guest", "n_guest", "guest_motif", "counts", "sk", "energy", "width", "hist", "hist_idx", "stats")
 
     def snapshot(self) -> dict:
         """Device copy of the mutable chain state (guests, S(k), energies,
@@ -474,17 +548,17 @@ class RigidGCMC:
         nv = proposal.nv if proposal is not None else (samples.nv if samples is not None else 0)
         if samples is not None and proposal is not None and samples.nv != proposal.nv:
             raise ValueError("samples and proposal use different voxel grids")
-        M = len(self.motif.charges)
         _lib.check(_lib.load().kupsmi_gcmc_run(
             _lib.stream_ptr(), self.n_chain, int(n_steps), int(self.step), self.seed & (2**64 - 1),
             self.max_host, _lib.ptr(d["host_off"]), _lib.ptr(d["host"]), _lib.ptr(d["host_species"]),
-            _lib.ptr(d["tril"]), self.gmax, M, _lib.ptr(d["motif_pos"]), _lib.ptr(d["motif_q"]),
+            _lib.ptr(d["tril"]), self.gmax, self.m, self.n_motif, self._n_host.ctypes.data,
+            _lib.ptr(d["motif_pos"]), _lib.ptr(d["motif_q"]),
             _lib.ptr(d["motif_species"]), self.n_species, _lib.ptr(d["sigma"]), _lib.ptr(d["epsilon"]),
             float(self.lj.cutoff), float(self.ewald_cutoff), _lib.ptr(d["alpha"]), _lib.ptr(d["tail"]),
             _lib.ptr(d["mol_const"]), _lib.ptr(d["kT"]), _lib.ptr(d["log_activity"]), self.nk,
             self.nm, _lib.ptr(d["kn"]), _lib.ptr(d["pk"]), self._w_host.ctypes.data,
-            self._u_host.ctypes.data, _lib.ptr(d["guest"]), _lib.ptr(d["n_guest"]), _lib.ptr(d["counts"]),
-            _lib.ptr(d["sk"]), _lib.ptr(d["energy"]), _lib.ptr(d["width"]), _lib.ptr(d["hist"]),
+            self._u_host.ctypes.data, _lib.ptr(d["guest"]), _lib.ptr(d["n_guest"]),
+            _lib.ptr(d["guest_motif"]), _lib.ptr(d["counts"]), _lib.ptr(d["sk"]), _lib.ptr(d["energy"]), _lib.ptr(d["width"]), _lib.ptr(d["hist"]),
             _lib.ptr(d["hist_idx"]), _lib.ptr(d["stats"]), int(n_trace), _lib.ptr(trace),
             _lib.ptr(cdf), _lib.ptr(guide), int(nv), _lib.ptr(samples.count) if samples is not None else None,
             int(samples.capacity) if samples is not None else 0,
@@ -494,6 +568,14 @@ class RigidGCMC:
     @property
     def loading(self) -> torch.Tensor:
         return self.d["n_guest"]
+
+    @property
+    def loading_per_motif(self) -> torch.Tensor:
+        """Guests of each motif, (n_chain, n_motif) int32 on the device."""
+        d = self.d
+        live = torch.arange(self.gmax, device=self.dev)[None, :] < d["n_guest"][:, None]
+        oh = torch.nn.functional.one_hot(d["guest_motif"].long(), self.n_motif)
+        return (oh * live[:, :, None]).sum(1).to(torch.int32)
 
     @property
     def stats(self) -> dict:
@@ -536,12 +618,35 @@ class GraphedTrials:
 def make_co2_gcmc(hosts, *, gmax: int = 64, seed: int = 0, weights=(1.0, 1.0, 1.0, 3.0),
                   device=None) -> RigidGCMC:
     """GCMC of CO2 in the synthetic rigid hosts of kups_amd.workloads.gcmc_hosts."""
+    return make_mixture_gcmc(hosts, ("CO2",), gmax=gmax, seed=seed, weights=weights,
+                             device=device)
+
+
+def mixture_species(adsorbates):
+    """Motifs, LJ table (host types of kups_amd.workloads first, then each new
+    guest label once, in order of appearance) and the LJ species of every
+    motif's sites for the named adsorbates."""
     from .. import workloads as w
 
-    labels = list(w.HOST_TYPES) + ["C_co2", "O_co2"]
+    motifs = [Motif.from_dict(w.ADSORBATES[a]) for a in adsorbates]
+    labels = list(w.HOST_TYPES)
+    for m_ in motifs:
+        for sym in m_.species:
+            if sym not in labels:
+                labels.append(sym)
     lj = LennardJonesParameters.from_dict(12.0, {k: w.GCMC_LJ[k] for k in labels})
-    motif = Motif.from_dict(w.CO2)
-    guest_species = tuple(labels.index(sym) for sym in motif.species)
+    guest_species = [tuple(labels.index(sym) for sym in m_.species) for m_ in motifs]
+    return motifs, lj, guest_species
+
+
+def make_mixture_gcmc(hosts, adsorbates, composition=None, *, gmax: int = 64, seed: int = 0,
+                      weights=(1.0, 1.0, 1.0, 3.0), device=None) -> RigidGCMC:
+    """GCMC of a gas mixture (names from kups_amd.workloads.ADSORBATES, e.g.
+    ("CO2", "N2")) at the gas-phase mole fractions ``composition`` in the
+    synthetic rigid hosts of kups_amd.workloads.gcmc_hosts."""
+    motifs, lj, guest_species = mixture_species(adsorbates)
+    if len(motifs) == 1:
+        motifs, guest_species = motifs[0], guest_species[0]
     return RigidGCMC(hosts.positions, hosts.charges, hosts.species, hosts.sizes, hosts.lattices,
-                     hosts.pressure, hosts.temperature, motif, lj, guest_species, gmax=gmax,
-                     seed=seed, weights=weights, device=device)
+                     hosts.pressure, hosts.temperature, motifs, lj, guest_species, gmax=gmax,
+                     seed=seed, weights=weights, composition=composition, device=device)

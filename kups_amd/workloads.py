@@ -141,6 +141,11 @@ GCMC_LJ = {
     "Zn": (2.4616, 0.005377696700922425),
     "C_co2": (2.8, 0.0023266791910486473),
     "O_co2": (3.05, 0.006807690966401598),
+    # TraPPE united-atom methane and three-site N2, whose massless centre
+    # carries charge only (examples/lennard_jones/trappe.yaml:629-637)
+    "CH4": (3.73, 0.012753648899081475),
+    "N_n2": (3.31, 0.0031022389213981967),
+    "M_n2": (None, None),
 }
 HOST_TYPES = ("C_", "H_", "O_", "N_", "Zn")
 HOST_TYPE_P = (0.4, 0.3, 0.15, 0.1, 0.05)
@@ -153,6 +158,24 @@ CO2 = {
     "critical_pressure": 7_840_000.0,
     "acentric_factor": 0.22394,
 }
+# TraPPE methane and N2 (examples/adsorbate/ch4.yaml, n2.yaml)
+CH4 = {
+    "symbols": ("CH4",),
+    "positions": ((0.0, 0.0, 0.0),),
+    "charges": (0.0,),
+    "critical_temperature": 190.564,
+    "critical_pressure": 4_599_200.0,
+    "acentric_factor": 0.01142,
+}
+N2 = {
+    "symbols": ("N_n2", "M_n2", "N_n2"),
+    "positions": ((0.0, 0.0, 0.55), (0.0, 0.0, 0.0), (0.0, 0.0, -0.55)),
+    "charges": (-0.482, 0.964, -0.482),
+    "critical_temperature": 126.192,
+    "critical_pressure": 3_395_800.0,
+    "acentric_factor": 0.0372,
+}
+ADSORBATES = {"CO2": CO2, "CH4": CH4, "N2": N2}
 
 
 def lattice_from_lengths_angles(lengths, angles_deg) -> np.ndarray:

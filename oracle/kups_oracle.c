@@ -990,22 +990,24 @@ static void min_image(const cell_t* c, const double* a, const double* b, double*
 typedef struct {
   /* sizes */
   int n_chain, n_steps, gmax, m, nspec, mx, my, mz;
+  int n_motif; const int32_t* motif_nsite; /* guest species and their site counts */
   int64_t step0;
   uint64_t seed;
   /* hosts */
   const int32_t* host_off; const double* host_pos; const double* host_q; const int32_t* host_spec;
   const double* tril;
-  /* motif (template centred at its geometric centre) */
+  /* motifs [n_motif x m (x 3)] (templates centred at their geometric centres) */
   const double* motif_pos; const double* motif_q; const int32_t* motif_spec;
   /* potentials */
   const double* sig; const double* eps; double rc_lj; double rc_cl;
   const double* alpha; const double* kmax; const double* tail; /* n_chain x nspec x nspec */
-  const double* mol_const; /* n_chain: E_self + E_excl of one molecule */
-  const double* kT; const double* log_activity;
+  const double* mol_const; /* n_chain x n_motif: E_self + E_excl of one molecule */
+  const double* kT; const double* log_activity; /* n_chain x n_motif */
   double w_move[4]; /* translation, rotation, reinsertion, exchange */
   double upper[4];  /* step-width upper bounds */
   /* state (in/out) */
-  double* guest_pos; int32_t* n_guest; int32_t* counts; /* n_chain x nspec */
+  double* guest_pos; int32_t* n_guest; int32_t* guest_motif; /* n_chain x gmax */
+  int32_t* counts; /* n_chain x nspec */
   double* sk; double* energy;
   float* width; uint32_t* hist; int32_t* hist_idx; /* n_chain x 4 (x 4 words) */
   int64_t* stats; /* n_chain x 8: trials[4], accepted[4] */
@@ -1035,6 +1037,7 @@ static void gc_grid_n(const gc_args* a, int k, int* n0, int* n1, int* n2) {
 }
 
 int orc_gcmc_run(gc_args* a) {
+  if (a->n_motif < 1 || a->n_motif > 8 || a->m < 1 || a->m > 8) return -1;
   const double K = coulomb_k();
   const int nk = (a->mx + 1) * (2*a->my + 1) * (2*a->mz + 1);
   const int M = a->m;
@@ -1072,51 +1075,67 @@ int orc_gcmc_run(gc_args* a) {
       gc_draw(a->seed, (uint32_t)ch, t, GC_TAG_B, rb);
       gc_draw(a->seed, (uint32_t)ch, t, GC_TAG_C, rcn);
       int ng = a->n_guest[ch];
-      /* old / new sites */
+      int32_t* gm = a->guest_motif + (size_t)ch * a->gmax;
+      /* molecules of each species */
+      int nmol[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int g = 0; g < ng; ++g) nmol[gm[g]] += 1;
+      /* old / new sites; an exchange picks the species uniformly (moves.py:
+       * 600-602, 711-712), a deletion then one guest of it; displacement
+       * moves pick a guest of any species */
       double oldp[8][3], newp[8][3];
       int n_old = 0, n_new = 0, slot = -1;
+      int tm = move == 3 ? (int)(ra[2] % (uint32_t)a->n_motif) : 0;
       float* wid = a->width + (size_t)ch * 4;
-      if (move < 3 || (move == 3 && deleting)) {
-        if (ng > 0) {
-          slot = (int)(ra[0] % (uint32_t)ng);
-          n_old = M;
-          for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) oldp[s][x] = gp[(slot*M + s)*3 + x];
-        }
+      if (move < 3 && ng > 0) {
+        slot = (int)(ra[0] % (uint32_t)ng);
+        tm = gm[slot];
+      } else if (move == 3 && deleting && nmol[tm] > 0) {
+        int k = (int)(ra[0] % (uint32_t)nmol[tm]);
+        for (int g = 0; g < ng; ++g)
+          if (gm[g] == tm && k-- == 0) { slot = g; break; }
+      }
+      const int Mt = a->motif_nsite[tm];
+      const double* mpos = a->motif_pos + (size_t)tm * M * 3;
+      const double* mq = a->motif_q + (size_t)tm * M;
+      const int32_t* msp = a->motif_spec + (size_t)tm * M;
+      if (slot >= 0) {
+        n_old = Mt;
+        for (int s = 0; s < Mt; ++s) for (int x = 0; x < 3; ++x) oldp[s][x] = gp[(slot*M + s)*3 + x];
       }
       if (move == 0 && n_old) {
         /* translation N(0,1) * width per axis (moves.py:222-244) */
         float u0 = gc_u01(rcn[0]), u1 = gc_u01(rcn[1]), u2 = gc_u01(rcn[2]), u3 = gc_u01(rcn[3]);
         double r0 = sqrt(-2.0*log((double)u0)), r1 = sqrt(-2.0*log((double)u2));
         double g3[3] = {r0*cos(2*M_PI*u1), r0*sin(2*M_PI*u1), r1*cos(2*M_PI*u3)};
-        for (int s = 0; s < M; ++s) {
+        for (int s = 0; s < Mt; ++s) {
           for (int x = 0; x < 3; ++x) newp[s][x] = oldp[s][x] + g3[x] * wid[0];
           wrap_f(&c, newp[s]);
         }
-        n_new = M;
+        n_new = Mt;
       } else if ((move == 1 || move == 2) && n_old) {
         /* rotation about the geometric centre (moves.py:158-197), q^w; reinsertion
          * = full rotation + uniform fractional offset (268-295) */
         float uq[3] = {gc_u01(rb[0]), gc_u01(rb[1]), gc_u01(rb[2])};
         double R[3][3]; gc_quat(uq, move == 1 ? wid[1] : 1.0, R);
         double rel[8][3], cen[3] = {0, 0, 0};
-        for (int s = 0; s < M; ++s) {
-          min_image(&c, oldp[M-1], oldp[s], rel[s]);
-          for (int x = 0; x < 3; ++x) cen[x] += rel[s][x] / M;
+        for (int s = 0; s < Mt; ++s) {
+          min_image(&c, oldp[Mt-1], oldp[s], rel[s]);
+          for (int x = 0; x < 3; ++x) cen[x] += rel[s][x] / Mt;
         }
-        double com[3] = {oldp[M-1][0] + cen[0], oldp[M-1][1] + cen[1], oldp[M-1][2] + cen[2]};
+        double com[3] = {oldp[Mt-1][0] + cen[0], oldp[Mt-1][1] + cen[1], oldp[Mt-1][2] + cen[2]};
         wrap_f(&c, com);
         double off[3] = {0, 0, 0};
         if (move == 2) {
           double fu[3] = {gc_u01(rcn[0]), gc_u01(rcn[1]), gc_u01(rcn[2])};
           rmul(c.L, fu, off);
         }
-        for (int s = 0; s < M; ++s) {
+        for (int s = 0; s < Mt; ++s) {
           double rr[3]; min_image(&c, com, oldp[s], rr);
           for (int x = 0; x < 3; ++x)
             newp[s][x] = com[x] + rr[0]*R[0][x] + rr[1]*R[1][x] + rr[2]*R[2][x] + off[x];
           wrap_f(&c, newp[s]);
         }
-        n_new = M;
+        n_new = Mt;
       } else if (move == 3 && !deleting) {
         /* insertion: template @ R + U L (moves.py:573-675) */
         float uq[3] = {gc_u01(rb[0]), gc_u01(rb[1]), gc_u01(rb[2])};
@@ -1124,13 +1143,13 @@ int orc_gcmc_run(gc_args* a) {
         double fu[3] = {gc_u01(rcn[0]), gc_u01(rcn[1]), gc_u01(rcn[2])}, off[3];
         rmul(c.L, fu, off);
         if (ng < a->gmax) {
-          for (int s = 0; s < M; ++s) {
-            const double* tp = a->motif_pos + 3*s;
+          for (int s = 0; s < Mt; ++s) {
+            const double* tp = mpos + 3*s;
             for (int x = 0; x < 3; ++x)
               newp[s][x] = tp[0]*R[0][x] + tp[1]*R[1][x] + tp[2]*R[2][x] + off[x];
             wrap_f(&c, newp[s]);
           }
-          n_new = M;
+          n_new = Mt;
         }
       }
       /* energy difference */
@@ -1140,14 +1159,16 @@ int orc_gcmc_run(gc_args* a) {
         double sgn = pass ? 1.0 : -1.0;
         for (int s = 0; s < ns; ++s) {
           double* r = pass ? newp[s] : oldp[s];
-          double q = a->motif_q[s]; int sp = a->motif_spec[s];
+          double q = mq[s]; int sp = msp[s];
           double e = 0;
           for (int j = h0; j < h1; ++j)
             e += gc_pair(a, &c, ch, r, q, sp, a->host_pos + 3*j, a->host_q[j], a->host_spec[j]);
           for (int g = 0; g < ng; ++g) {
             if (g == slot) continue;
-            for (int s2 = 0; s2 < M; ++s2)
-              e += gc_pair(a, &c, ch, r, q, sp, gp + (g*M + s2)*3, a->motif_q[s2], a->motif_spec[s2]);
+            const int t2 = gm[g];
+            for (int s2 = 0; s2 < a->motif_nsite[t2]; ++s2)
+              e += gc_pair(a, &c, ch, r, q, sp, gp + (g*M + s2)*3, a->motif_q[t2*M + s2],
+                           a->motif_spec[t2*M + s2]);
           }
           dE += sgn * e;
         }
@@ -1161,8 +1182,8 @@ int orc_gcmc_run(gc_args* a) {
           for (int s = 0; s < ns; ++s) {
             double* r = pass ? newp[s] : oldp[s];
             double th = kv[k][0]*r[0] + kv[k][1]*r[1] + kv[k][2]*r[2];
-            dr += sgn * a->motif_q[s] * cos(th);
-            di += sgn * a->motif_q[s] * sin(th);
+            dr += sgn * mq[s] * cos(th);
+            di += sgn * mq[s] * sin(th);
           }
         }
         dS[2*k] = dr; dS[2*k+1] = di;
@@ -1174,19 +1195,20 @@ int orc_gcmc_run(gc_args* a) {
       int dn = 0;
       if (move == 3) dn = n_new ? 1 : (n_old ? -1 : 0);
       if (dn) {
-        dE += dn * a->mol_const[ch];
+        dE += dn * a->mol_const[(size_t)ch * a->n_motif + tm];
         /* tail: sum_ab T_ab (N'_a N'_b - N_a N_b) */
         int np_[16];
         for (int s = 0; s < a->nspec; ++s) np_[s] = cnt[s];
-        for (int s = 0; s < M; ++s) np_[a->motif_spec[s]] += dn;
+        for (int s = 0; s < Mt; ++s) np_[msp[s]] += dn;
         double et = 0;
         for (int x = 0; x < a->nspec; ++x)
           for (int y = 0; y < a->nspec; ++y)
             et += T[x*a->nspec + y] * ((double)np_[x]*np_[y] - (double)cnt[x]*cnt[y]);
         dE += et;
-        double lnfV = a->log_activity[ch] + log(c.vol);
-        if (dn > 0) logp += lnfV - log((double)(ng + 1));
-        else logp += -lnfV + log((double)ng);
+        /* fugacity term of the exchanged species (probability.py:117-173) */
+        double lnfV = a->log_activity[(size_t)ch * a->n_motif + tm] + log(c.vol);
+        if (dn > 0) logp += lnfV - log((double)(nmol[tm] + 1));
+        else logp += -lnfV + log((double)nmol[tm]);
       }
       logp += -dE / a->kT[ch];
       double lu = log((double)gc_u01(ra[1]));
@@ -1200,17 +1222,19 @@ int orc_gcmc_run(gc_args* a) {
       if (accept) {
         stt[4 + move] += 1;
         if (n_new && n_old) {
-          for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) gp[(slot*M + s)*3 + x] = newp[s][x];
+          for (int s = 0; s < Mt; ++s) for (int x = 0; x < 3; ++x) gp[(slot*M + s)*3 + x] = newp[s][x];
         } else if (n_new) {
-          for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) gp[(ng*M + s)*3 + x] = newp[s][x];
+          for (int s = 0; s < Mt; ++s) for (int x = 0; x < 3; ++x) gp[(ng*M + s)*3 + x] = newp[s][x];
+          gm[ng] = tm;
           a->n_guest[ch] = ng + 1;
           a->ins_acc[ch] += 1;
         } else if (n_old) {
           for (int s = 0; s < M; ++s) for (int x = 0; x < 3; ++x) gp[(slot*M + s)*3 + x] = gp[((ng-1)*M + s)*3 + x];
+          gm[slot] = gm[ng - 1];
           a->n_guest[ch] = ng - 1;
           a->del_acc[ch] += 1;
         }
-        if (dn) for (int s = 0; s < M; ++s) cnt[a->motif_spec[s]] += dn;
+        if (dn) for (int s = 0; s < Mt; ++s) cnt[msp[s]] += dn;
         for (int k = 0; k < nk; ++k) { S[2*k] += dS[2*k]; S[2*k+1] += dS[2*k+1]; }
         a->energy[ch] += dE;
       }

@@ -307,6 +307,7 @@ def md_vv_f32(n_steps, pos, mom, mass, species, sys_off, tril, cutoff, sigma, ep
 
 class _GcArgs(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("n_chain", "n_steps", "gmax", "m", "nspec", "mx", "my", "mz")] + [
+        ("n_motif", C.c_int), ("motif_nsite", C.c_void_p),
         ("step0", C.c_int64), ("seed", C.c_uint64),
         ("host_off", C.c_void_p), ("host_pos", C.c_void_p), ("host_q", C.c_void_p),
         ("host_spec", C.c_void_p), ("tril", C.c_void_p), ("motif_pos", C.c_void_p),
@@ -315,7 +316,8 @@ class _GcArgs(C.Structure):
         ("alpha", C.c_void_p), ("kmax", C.c_void_p), ("tail", C.c_void_p),
         ("mol_const", C.c_void_p), ("kT", C.c_void_p), ("log_activity", C.c_void_p),
         ("w_move", C.c_double * 4), ("upper", C.c_double * 4),
-        ("guest_pos", C.c_void_p), ("n_guest", C.c_void_p), ("counts", C.c_void_p),
+        ("guest_pos", C.c_void_p), ("n_guest", C.c_void_p), ("guest_motif", C.c_void_p),
+        ("counts", C.c_void_p),
         ("sk", C.c_void_p), ("energy", C.c_void_p), ("width", C.c_void_p), ("hist", C.c_void_p),
         ("hist_idx", C.c_void_p), ("stats", C.c_void_p), ("ins_acc", C.c_void_p),
         ("del_acc", C.c_void_p), ("n_trace", C.c_int), ("trace", C.c_void_p)]
@@ -324,14 +326,18 @@ class _GcArgs(C.Structure):
 def gcmc_run(state: dict, n_steps: int, step0: int, seed: int, n_trace: int = 0):
     """FP64 GCMC of every chain in place (see orc_gcmc_run).  ``state`` holds
     numpy arrays: host_off, host_pos, host_q, host_spec, tril, motif_pos, motif_q,
-    motif_spec, sigma, epsilon, alpha, kmax, tail, mol_const, kT, log_activity,
-    guest_pos (n, gmax, m, 3), n_guest, counts, sk (n, nk_grid, 2), energy, width
-    (n, 4) f32, hist (n, 4, 4) u32, hist_idx (n, 4), stats (n, 8) i64, ins_acc,
-    del_acc; scalars rc_lj, rc_cl, grid (mx, my, mz), w_move, upper."""
+    motif_spec, motif_nsite (n_motif), sigma, epsilon, alpha, kmax, tail, mol_const
+    (n, n_motif), kT, log_activity (n, n_motif), guest_pos (n, gmax, m, 3), n_guest,
+    guest_motif (n, gmax), counts, sk (n, nk_grid, 2), energy, width (n, 4) f32,
+    hist (n, 4, 4) u32, hist_idx (n, 4), stats (n, 8) i64, ins_acc, del_acc;
+    scalars rc_lj, rc_cl, grid (mx, my, mz), w_move, upper.  Motif arrays are
+    (n_motif, m[, 3]) with m the largest site count."""
     a = _GcArgs()
     n = len(state["n_guest"])
     gmax, m = state["guest_pos"].shape[1:3]
     a.n_chain, a.n_steps, a.gmax, a.m = n, int(n_steps), gmax, m
+    nsite = np.ascontiguousarray(state["motif_nsite"], dtype=np.int32)
+    a.n_motif, a.motif_nsite = len(nsite), _p(nsite)
     a.nspec = state["sigma"].shape[0]
     a.mx, a.my, a.mz = state["grid"]
     a.step0, a.seed = int(step0), int(seed) & (2**64 - 1)
@@ -343,7 +349,8 @@ def gcmc_run(state: dict, n_steps: int, step0: int, seed: int, n_trace: int = 0)
                     ("tail", np.float64), ("mol_const", np.float64), ("kT", np.float64),
                     ("log_activity", np.float64)):
         keep[key] = np.ascontiguousarray(state[key], dtype=dt)
-    for key, dt in (("guest_pos", np.float64), ("n_guest", np.int32), ("counts", np.int32),
+    for key, dt in (("guest_pos", np.float64), ("n_guest", np.int32), ("guest_motif", np.int32),
+                    ("counts", np.int32),
                     ("sk", np.float64), ("energy", np.float64), ("width", np.float32),
                     ("hist", np.uint32), ("hist_idx", np.int32), ("stats", np.int64),
                     ("ins_acc", np.int64), ("del_acc", np.int64)):
@@ -362,11 +369,13 @@ def gcmc_run(state: dict, n_steps: int, step0: int, seed: int, n_trace: int = 0)
         a.w_move[i] = float(state["w_move"][i])
         a.upper[i] = float(state["upper"][i])
     a.guest_pos, a.n_guest, a.counts = _p(keep["guest_pos"]), _p(keep["n_guest"]), _p(keep["counts"])
+    a.guest_motif = _p(keep["guest_motif"])
     a.sk, a.energy, a.width = _p(keep["sk"]), _p(keep["energy"]), _p(keep["width"])
     a.hist, a.hist_idx, a.stats = _p(keep["hist"]), _p(keep["hist_idx"]), _p(keep["stats"])
     a.ins_acc, a.del_acc = _p(keep["ins_acc"]), _p(keep["del_acc"])
     a.n_trace, a.trace = int(n_trace), _p(trace)
-    lib().orc_gcmc_run(C.byref(a))
+    if lib().orc_gcmc_run(C.byref(a)) != 0:
+        raise ValueError("orc_gcmc_run: unsupported sizes (1..8 motifs of 1..8 sites)")
     return trace[:n_trace]
 
 
@@ -393,11 +402,14 @@ def gcmc_state(g, chains=None) -> dict:
         host_pos=np.ascontiguousarray(host[rows, :3]), host_q=np.ascontiguousarray(host[rows, 3]),
         host_spec=spec[rows], tril=cpu("tril")[idx],
         motif_pos=cpu("motif_pos").astype(np.float64), motif_q=cpu("motif_q").astype(np.float64),
-        motif_spec=cpu("motif_species"), sigma=g.lj.sigma, epsilon=g.lj.epsilon,
-        alpha=g.alpha[idx], kmax=g.kmax[idx], tail=g.tail[idx], mol_const=g.mol_const[idx],
-        kT=cpu("kT")[idx], log_activity=g.log_activity[idx],
+        motif_spec=cpu("motif_species"), motif_nsite=np.asarray(g.motif_nsite, np.int32),
+        sigma=g.lj.sigma, epsilon=g.lj.epsilon,
+        alpha=g.alpha[idx], kmax=g.kmax[idx], tail=g.tail[idx],
+        mol_const=np.ascontiguousarray(g.mol_const[idx]),
+        kT=cpu("kT")[idx], log_activity=np.ascontiguousarray(g.log_activity[idx]),
         guest_pos=np.ascontiguousarray(cpu("guest")[idx].astype(np.float64)),
         n_guest=np.ascontiguousarray(cpu("n_guest")[idx]),
+        guest_motif=np.ascontiguousarray(cpu("guest_motif")[idx]),
         counts=np.ascontiguousarray(cpu("counts")[idx]), sk=sk,
         energy=np.ascontiguousarray(cpu("energy")[idx]), width=np.ascontiguousarray(cpu("width")[idx]),
         hist=np.ascontiguousarray(cpu("hist")[idx].view(np.uint32)),

@@ -52,18 +52,27 @@ def test_pr_binary_ideal_limit():
     np.testing.assert_allclose(lnf, np.log(y * p), atol=2e-4)
 
 
-def _empty_box_state(aV: float, L: float = 25.0, gmax: int = 256):
-    """One chain, no host, a 3-site guest without charges or LJ: an ideal gas."""
+def _empty_box_state(aV, L: float = 25.0, gmax: int = 256):
+    """One chain, no host, guests without charges or LJ: an ideal gas.  One
+    3-site species, or with ``aV`` a sequence, a mixture whose species t has
+    3 - t sites (1..3) and activity aV[t] / V."""
     kT = 298.0 * BOLTZMANN_CONSTANT
     V = L**3
+    aV = np.atleast_1d(np.asarray(aV, float))
+    nmot = len(aV)
+    mpos = np.zeros((nmot, 3, 3))
+    for t in range(nmot):
+        mpos[t, :3 - t, 0] = np.linspace(-1.16, 1.16, 3 - t) if t < 2 else 0.0
     return dict(
         host_off=np.array([0, 0]), host_pos=np.zeros((1, 3)), host_q=np.zeros(1),
         host_spec=np.zeros(1), tril=np.array([[L, 0, L, 0, 0, L]]),
-        motif_pos=np.array([[0, 0, 0], [-1.16, 0, 0], [1.16, 0, 0]]), motif_q=np.zeros(3),
-        motif_spec=np.zeros(3), sigma=np.array([[3.0]]), epsilon=np.array([[0.0]]),
+        motif_pos=mpos, motif_q=np.zeros((nmot, 3)), motif_spec=np.zeros((nmot, 3)),
+        motif_nsite=np.array([3 - t for t in range(nmot)]),
+        sigma=np.array([[3.0]]), epsilon=np.array([[0.0]]),
         alpha=np.array([0.3]), kmax=np.array([-1.0]), tail=np.zeros((1, 1, 1)),
-        mol_const=np.zeros(1), kT=np.array([kT]), log_activity=np.array([math.log(aV / V)]),
+        mol_const=np.zeros((1, nmot)), kT=np.array([kT]), log_activity=np.log(aV / V)[None],
         guest_pos=np.zeros((1, gmax, 3, 3)), n_guest=np.zeros(1, np.int32),
+        guest_motif=np.zeros((1, gmax), np.int32),
         counts=np.zeros((1, 1), np.int32), sk=np.zeros((1, 1, 2)), energy=np.zeros(1),
         width=np.full((1, 4), 0.1, np.float32), hist=np.zeros((1, 4, 4), np.uint32),
         hist_idx=np.zeros((1, 4), np.int32), stats=np.zeros((1, 8), np.int64),
@@ -108,6 +117,93 @@ def test_oracle_ideal_gas_loading():
     ns = np.array(ns[20:])
     assert ns.mean() == pytest.approx(6.0, abs=0.6)
     assert ns.var() == pytest.approx(6.0, rel=0.35)
+
+
+def test_oracle_mixture_exchange_log_ratios_exact():
+    """Two species (activities 5.5 / V and pi / V, 3 and 2 sites): every
+    exchange trial carries the fugacity term of ONE species with that
+    species' own count, ln(a_t V) - ln(N_t + 1) or -ln(a_t V) + ln N_t
+    (probability.py:116-173, the species drawn uniformly, moves.py:600-602,
+    711-712); a deletion of a species with no molecule present changes
+    nothing.  The ratio a_0 / a_1 is irrational, so the trace's log p names
+    the species; the tracked counts must end where the kernel's do."""
+    aV = (5.5, math.pi)
+    st = _empty_box_state(aV=aV)
+    st["w_move"] = (0.0, 0.0, 0.0, 1.0)
+    tr = oracle.gcmc_run(st, 3000, 0, seed=9, n_trace=1)[0]
+    n = [0, 0]
+    picked = [0, 0]
+    noop = 0
+    for code, dE, logp, acc in tr:
+        assert dE == 0.0
+        if code == 6:
+            cand = [math.log(aV[t]) - math.log(n[t] + 1) for t in (0, 1)]
+        else:
+            cand = [-math.log(aV[t]) + math.log(n[t]) if n[t] else 0.0 for t in (0, 1)]
+        hit = [t for t in (0, 1) if abs(logp - cand[t]) < 1e-12]
+        assert hit, (code, logp, cand, n)
+        t = hit[0]
+        if code == 7 and n[t] == 0:
+            noop += 1
+            assert logp == 0.0
+            continue
+        picked[t] += 1
+        if acc:
+            n[t] += 1 if code == 6 else -1
+    gm = st["guest_motif"][0, :st["n_guest"][0]]
+    assert n == [int((gm == 0).sum()), int((gm == 1).sum())]
+    assert min(picked) > 600 and noop > 0
+
+
+def test_oracle_mixture_ideal_gas_loading():
+    """Ideal mixture in an empty box: each species is an independent Poisson
+    gas, <N_t> = a_t V, whatever its site count."""
+    st = _empty_box_state(aV=(4.0, 2.0, 1.0))
+    st["w_move"] = (0.0, 0.0, 0.0, 1.0)
+    ns = []
+    step = 0
+    for _ in range(300):
+        oracle.gcmc_run(st, 100, step, seed=13)
+        step += 100
+        gm = st["guest_motif"][0, :st["n_guest"][0]]
+        ns.append(np.bincount(gm, minlength=3))
+    ns = np.array(ns[30:], float)
+    np.testing.assert_allclose(ns.mean(0), [4.0, 2.0, 1.0], rtol=0.15, atol=0.2)
+    np.testing.assert_allclose(ns.var(0), [4.0, 2.0, 1.0], rtol=0.4)
+
+
+def test_oracle_motif_padding_never_read():
+    """Interacting guests (LJ between sites): a run whose motif table and
+    guest buffer are padded to 4 sites, the padding site holding a charge and
+    a position that would change every energy if read, gives the same trace,
+    bit for bit, as the unpadded 3-site tables."""
+    a = _empty_box_state(aV=40.0)
+    a["epsilon"] = np.array([[0.005]])
+    b = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in a.items()}
+    b["motif_pos"] = np.concatenate([a["motif_pos"], [[[0.3, 0.2, 0.1]]]], 1)
+    b["motif_q"] = np.concatenate([a["motif_q"], [[5.0]]], 1)
+    b["motif_spec"] = np.concatenate([a["motif_spec"], [[0]]], 1)
+    b["guest_pos"] = np.zeros((1, a["guest_pos"].shape[1], 4, 3))
+    tra = oracle.gcmc_run(a, 800, 0, seed=4, n_trace=1)
+    trb = oracle.gcmc_run(b, 800, 0, seed=4, n_trace=1)
+    np.testing.assert_array_equal(tra, trb)
+    assert a["n_guest"][0] > 5 and np.any(tra[0, :, 1] != 0)
+
+
+def test_mixture_species_table():
+    """Each LJ label appears once; every site of every motif maps to its own
+    label's row (a site table pointing past the LJ table would read
+    another species' parameters)."""
+    from kups_amd import workloads
+    from kups_amd.mcmc.gcmc import mixture_species
+
+    motifs, lj, gs = mixture_species(("CO2", "N2", "CH4"))
+    assert len(set(lj.labels)) == len(lj.labels) == 10
+    for m_, sp in zip(motifs, gs):
+        assert [lj.labels[i] for i in sp] == list(m_.species)
+    # single CO2: the table the earlier single-species path built
+    _, lj1, gs1 = mixture_species(("CO2",))
+    assert lj1.labels == tuple(workloads.HOST_TYPES) + ("C_co2", "O_co2") and gs1 == [(5, 6, 6)]
 
 
 def test_batch_shared_move_type():
