@@ -354,6 +354,15 @@ int kupsmi_verlet_freeze(kupsmi_verlet* v, int frozen);
  * their events), then enables (enable != 0) or disables the timing of later
  * calls.  No counterpart in the reference; for measurement. */
 int kupsmi_verlet_profile(kupsmi_verlet* v, int enable, double* total_ms, long long* calls);
+/* Arms the handle's next kupsmi_pair_forces_verlet to make its stream wait
+ * (hipStreamWaitEvent) for `event` after its list upkeep (binning and any
+ * list rebuild) and before its pair passes; that call takes the gate.  With
+ * the event of kupsmi_ewald_phase_event, the list upkeep runs beside the
+ * other stream's structure-factor kernels and the pair passes beside its
+ * force pass.  Results are unchanged (ordering only).  No counterpart in the
+ * reference (a scheduling hook of the two-stream MD step); recordable in a
+ * HIP graph like the call itself. */
+int kupsmi_verlet_gate(kupsmi_verlet* v, void* event);
 /* Arms the handle's next kupsmi_pair_forces_verlet to record its pair set,
  * as kupsmi_ctx_record_pairs does for the context's calls. */
 int kupsmi_verlet_record_pairs(kupsmi_verlet* v, int64_t capacity, int32_t* ij, int32_t* shift,
@@ -383,6 +392,11 @@ int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
                        const double* kmax, int mx, int my, int mz, int accumulate,
                        float* forces, double* energy, double* virial,
                        double* structure_factor);
+/* The context's phase event (a hipEvent_t, owned by the context): every
+ * kupsmi_ewald_recip on ctx records it on its stream once the structure
+ * factors and energies are done, before the force pass.  No counterpart in
+ * the reference (a scheduling hook, see kupsmi_verlet_gate). */
+int kupsmi_ewald_phase_event(kupsmi_ctx* ctx, void** event);
 
 /*
  * kupsmi_ewald_recip without a context: the same kernels and results, with

@@ -29,6 +29,8 @@ _SIGNATURES = {
     "kupsmi_ctx_freeze": ([vp, i32], C.c_int),
     "kupsmi_verlet_freeze": ([vp, i32], C.c_int),
     "kupsmi_verlet_profile": ([vp, i32, vp, vp], C.c_int),
+    "kupsmi_verlet_gate": ([vp, vp], C.c_int),
+    "kupsmi_ewald_phase_event": ([vp, C.POINTER(vp)], C.c_int),
     "kupsmi_neighborlist": ([vp, vp, i32, vp, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),
     "kupsmi_neighborlist_cells": (
         [vp, vp, i32, vp, i32, i32, vp, vp, vp, vp, i64, vp, vp, vp], C.c_int),

@@ -56,6 +56,9 @@ struct kupsmi_ctx {
   // set by kupsmi_ctx_freeze: a slot that would have to grow is an error
   // (recorded work holds the slots' addresses)
   int frozen = 0;
+  // recorded by each kupsmi_ewald_recip on this context between its
+  // structure-factor / energy kernels and its force pass
+  hipEvent_t phase = nullptr;
 };
 
 namespace kupsmi {

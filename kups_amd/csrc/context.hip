@@ -42,6 +42,11 @@ extern "C" int kupsmi_ctx_create(kupsmi_ctx** out) {
   kupsmi_ctx* c = new (std::nothrow) kupsmi_ctx();
   KUPSMI_REQUIRE(c != nullptr, "out of host memory");
   KUPSMI_HIP(hipGetDevice(&c->device));
+  const hipError_t e = hipEventCreateWithFlags(&c->phase, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    delete c;
+    KUPSMI_HIP(e);
+  }
   *out = c;
   return KUPSMI_OK;
 }
@@ -69,6 +74,7 @@ extern "C" int kupsmi_ctx_destroy(kupsmi_ctx* ctx) {
   KUPSMI_HIP(hipSetDevice(ctx->device));
   for (auto& s : ctx->slots)
     if (s.ptr) (void)hipFree(s.ptr);
+  if (ctx->phase) (void)hipEventDestroy(ctx->phase);
   (void)hipSetDevice(prev);
   delete ctx;
   return KUPSMI_OK;

@@ -576,6 +576,7 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
   KUPSMI_CHECK_LAUNCH();
   ek_energy_kernel<<<n_sys, EK_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
+  if (ctx && ctx->phase) KUPSMI_HIP(hipEventRecord(ctx->phase, st));
   if (forces && max_atoms_per_sys > 0) {
     dim3 g2((max_atoms_per_sys + EK_THREADS - 1) / EK_THREADS, n_sys);
     const size_t lds = sizeof(float) * coef_floats;
@@ -591,6 +592,12 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
 }
 
 }  // namespace kupsmi
+
+extern "C" int kupsmi_ewald_phase_event(kupsmi_ctx* ctx, void** event) {
+  KUPSMI_REQUIRE(ctx != nullptr && event != nullptr, "null context or output");
+  *event = (void*)ctx->phase;
+  return KUPSMI_OK;
+}
 
 extern "C" int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* sys_off,
                                   int n_atoms, int max_atoms_per_sys, const float* pos,

@@ -20,39 +20,46 @@ constexpr int PC_NSPEC = 16;
 constexpr double PC_BAND64 = 4e-6;
 constexpr int PC_NPART = 8;  // e_lj, e_real, w00 w01 w02 w11 w12 w22
 
-// Exclusive scan of cnt[0..n) into start[0..n], n <= 4 * blockDim.
+// Exclusive scan of cnt[0..n) into start[0..n] by the whole workgroup, in
+// chunks of 4 * blockDim entries (four per lane) carried over; wsum holds
+// blockDim / 64 ints.
 __device__ void block_scan4(const int* cnt, int* start, int n, int* wsum) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x / 64;
-  int v4[4], own = 0;
+  int carry = 0;
+  for (int base = 0; base < n; base += 4 * (int)blockDim.x) {
+    int v4[4], own = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int c = tid * 4 + k;
-    v4[k] = c < n ? cnt[c] : 0;
-    own += v4[k];
-  }
-  int v = own;
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(v, o, 64);
-    if (lane >= o) v += y;
-  }
-  if (lane == 63) wsum[wv] = v;
-  __syncthreads();
-  if (wv == 0) {
-    int w = lane < nw ? wsum[lane] : 0;
-    for (int o = 1; o < 64; o <<= 1) {
-      int y = __shfl_up(w, o, 64);
-      if (lane >= o) w += y;
+    for (int k = 0; k < 4; ++k) {
+      const int c = base + tid * 4 + k;
+      v4[k] = c < n ? cnt[c] : 0;
+      own += v4[k];
     }
-    if (lane < nw) wsum[lane] = w;
-  }
-  __syncthreads();
-  int ex = v - own + (wv > 0 ? wsum[wv - 1] : 0);
+    int v = own;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(v, o, 64);
+      if (lane >= o) v += y;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+      int w = lane < nw ? wsum[lane] : 0;
+      for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(w, o, 64);
+        if (lane >= o) w += y;
+      }
+      if (lane < nw) wsum[lane] = w;
+    }
+    __syncthreads();
+    int ex = carry + v - own + (wv > 0 ? wsum[wv - 1] : 0);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int c = tid * 4 + k;
-    if (c < n) start[c] = ex;
-    ex += v4[k];
-    if (c == n - 1) start[n] = ex;
+    for (int k = 0; k < 4; ++k) {
+      const int c = base + tid * 4 + k;
+      if (c < n) start[c] = ex;
+      ex += v4[k];
+      if (c == n - 1) start[n] = ex;
+    }
+    carry += wsum[nw - 1];
+    __syncthreads();  // wsum is rewritten by the next chunk
   }
 }
 
@@ -85,7 +92,7 @@ struct PcSorted {
   int* cs;
 };
 
-// One workgroup (PC_BIN_THREADS) bins system i0..i0+n into cells of g:
+// One workgroup (any size up to PC_BIN_THREADS) bins system i0..i0+n into cells of g:
 // wrap, stable counting sort (ties keep atom order), sorted copies.
 __device__ inline void pc_bin_system(const SysGeom& g, int i0, int n, const float* pos,
                                      const float* q, const int32_t* species,
@@ -94,28 +101,28 @@ __device__ inline void pc_bin_system(const SysGeom& g, int i0, int n, const floa
   const int tid = threadIdx.x;
   const int ncell = g.ncell;
   if (n <= 0 || n > PC_NMAX) {
-    for (int c = tid; c <= ncell; c += PC_BIN_THREADS) o.cs[c] = 0;
+    for (int c = tid; c <= ncell; c += (int)blockDim.x) o.cs[c] = 0;
     return;
   }
-  for (int c = tid; c < ncell; c += PC_BIN_THREADS) L.cnt[c] = 0;
+  for (int c = tid; c < ncell; c += (int)blockDim.x) L.cnt[c] = 0;
   __syncthreads();
   float wx, wy, wz;
-  for (int i = tid; i < n; i += PC_BIN_THREADS) {
+  for (int i = tid; i < n; i += (int)blockDim.x) {
     const float* r = pos + 3 * (size_t)(i0 + i);
     atomicAdd(&L.cnt[cell_of(g, r[0], r[1], r[2], wx, wy, wz)], 1);
   }
   __syncthreads();
   block_scan4(L.cnt, L.start, ncell, L.wsum);
   __syncthreads();
-  for (int c = tid; c < ncell; c += PC_BIN_THREADS) L.cnt[c] = 0;
+  for (int c = tid; c < ncell; c += (int)blockDim.x) L.cnt[c] = 0;
   __syncthreads();
-  for (int i = tid; i < n; i += PC_BIN_THREADS) {
+  for (int i = tid; i < n; i += (int)blockDim.x) {
     const float* r = pos + 3 * (size_t)(i0 + i);
     const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
     L.keys[L.start[c] + atomicAdd(&L.cnt[c], 1)] = (short)i;
   }
   __syncthreads();
-  for (int i = tid; i < n; i += PC_BIN_THREADS) {
+  for (int i = tid; i < n; i += (int)blockDim.x) {
     const float* r = pos + 3 * (size_t)(i0 + i);
     const int c = cell_of(g, r[0], r[1], r[2], wx, wy, wz);
     const int b = L.start[c], e = L.start[c + 1];
@@ -129,7 +136,7 @@ __device__ inline void pc_bin_system(const SysGeom& g, int i0, int n, const floa
     if (lj_excl) o.slj[d] = lj_excl[i0 + i];
     if (cl_excl) o.scl[d] = cl_excl[i0 + i];
   }
-  for (int c = tid; c <= ncell; c += PC_BIN_THREADS) o.cs[c] = L.start[c];
+  for (int c = tid; c <= ncell; c += (int)blockDim.x) o.cs[c] = L.start[c];
 }
 
 // Band decisions on the caller's positions.  The pair kernels work on

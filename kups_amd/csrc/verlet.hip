@@ -44,6 +44,9 @@ struct kupsmi_verlet {
   // kupsmi_verlet_profile: HIP events around the main pair pass of each call
   int profile = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  // kupsmi_verlet_gate: an event the next evaluation waits for between its
+  // list upkeep and its pair passes (taken by that evaluation)
+  hipEvent_t gate = nullptr;
 };
 
 namespace kupsmi {
@@ -51,10 +54,17 @@ namespace kupsmi {
 constexpr int VL_TILE = 256;
 static_assert(VL_TILE == PC_TILE, "band_ctx locates the left atom by the pair tile");
 constexpr int VL_SHIFT_MAX = 7;  // |image shift| per axis in a packed entry
+constexpr int VL_STAGE_MAX = 8192;  // atoms of a system the list build stages in LDS
+constexpr int VL_LIST_THREADS = 1024;  // staged list build: four tiles of one system per workgroup
+constexpr int VL_BIN_THREADS = 256;
+// candidate loads in flight per lane of the list build (a rebuild of a few
+// systems runs at about one wave per SIMD beside the other stream's kernels)
+constexpr int VL_LIST_BATCH = 8;
 
 struct VlArgs {
   int n_sys, n_atoms, n_species, has_lj, has_coul, has_lj_excl, has_coul_excl, accumulate;
   int tiles, cap;
+  int stage;  // list build: atoms of a system staged in LDS (the launch's largest system)
   double skin;
   const int32_t* sys_off;
   const float* pos;
@@ -87,6 +97,7 @@ struct VlArgs {
   unsigned long long* counters;  // [0] system rebuilds, [1] overflowed atom lists
   double* partial;      // main pass, then the band pass (n_sys x tiles x PC_NPART each)
   int* tile_off;        // n_sys + 1: prefix of the systems' tile counts
+  int* tile4_off;       // n_sys + 1: the same in units of VL_LIST_THREADS atoms
   // per sorted atom: the number of list entries inside a cutoff band met by
   // the main pass (VL_NBAND_FULL: more than VL_NBAND, or met on the stencil
   // path), and up to VL_NBAND of those entries
@@ -125,20 +136,32 @@ __global__ void vl_invalidate_kernel(int n_sys, int* valid) {
 constexpr int VL_XCDS = 8;
 constexpr int VL_PREP_THREADS = 1024;
 
-__device__ inline bool vl_tile_of(const VlArgs& a, int& s, int& tile) {
-  const int T = a.tile_off[a.n_sys];
+__device__ inline bool vl_unit_of(const int* off, int n_sys, int& s, int& tile) {
+  const int T = off[n_sys];
   const int per = (T + VL_XCDS - 1) / VL_XCDS;
   const int w = blockIdx.x, k = w / VL_XCDS;
   const int L = (w % VL_XCDS) * per + k;
   if (k >= per || L >= T) return false;
-  int lo = 0, hi = a.n_sys;  // the last system whose first tile is <= L
+  int lo = 0, hi = n_sys;  // the last system whose first tile is <= L
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (a.tile_off[mid] <= L) lo = mid; else hi = mid;
+    if (off[mid] <= L) lo = mid; else hi = mid;
   }
   s = lo;
-  tile = L - a.tile_off[lo];
+  tile = L - off[lo];
   return true;
+}
+
+__device__ inline bool vl_tile_of(const VlArgs& a, int& s, int& tile) {
+  return vl_unit_of(a.tile_off, a.n_sys, s, tile);
+}
+
+// Grid of a launch mapped by vl_unit_of: the units number at most
+// T = ceil(n_atoms / unit) + n_sys, and workgroup w = VL_XCDS k + x serves
+// unit x per + k (k < per = ceil(T / VL_XCDS)), so the grid is VL_XCDS per.
+inline unsigned vl_grid(int n_atoms, int n_sys, int unit) {
+  const long long t = ((long long)n_atoms + unit - 1) / unit + n_sys;
+  return (unsigned)(VL_XCDS * ((t + VL_XCDS - 1) / VL_XCDS));
 }
 
 // One workgroup: clears the per-system displacement maxima and writes the
@@ -149,22 +172,29 @@ __global__ __launch_bounds__(VL_PREP_THREADS) void vl_prep_kernel(VlArgs a) {
   for (int k = tid; k < ns; k += VL_PREP_THREADS) a.maxd2[k] = 0u;
   const int chunk = (ns + VL_PREP_THREADS - 1) / VL_PREP_THREADS;
   const int b = tid * chunk, e = min(b + chunk, ns);
-  int sum = 0;
-  for (int k = b; k < e; ++k) sum += (a.sys_off[k + 1] - a.sys_off[k] + VL_TILE - 1) / VL_TILE;
-  part[tid] = sum;
-  __syncthreads();
-  for (int o = 1; o < VL_PREP_THREADS; o <<= 1) {
-    const int v = tid >= o ? part[tid - o] : 0;
+  // the tile prefix (VL_TILE atoms) and the list build's unit prefix
+  // (VL_LIST_THREADS atoms)
+  for (int pass = 0; pass < 2; ++pass) {
+    const int unit = pass ? VL_LIST_THREADS : VL_TILE;
+    int* out = pass ? a.tile4_off : a.tile_off;
+    int sum = 0;
+    for (int k = b; k < e; ++k) sum += (a.sys_off[k + 1] - a.sys_off[k] + unit - 1) / unit;
     __syncthreads();
-    part[tid] += v;
+    part[tid] = sum;
     __syncthreads();
+    for (int o = 1; o < VL_PREP_THREADS; o <<= 1) {
+      const int v = tid >= o ? part[tid - o] : 0;
+      __syncthreads();
+      part[tid] += v;
+      __syncthreads();
+    }
+    int run = part[tid] - sum;  // exclusive prefix of this chunk
+    for (int k = b; k < e; ++k) {
+      out[k] = run;
+      run += (a.sys_off[k + 1] - a.sys_off[k] + unit - 1) / unit;
+    }
+    if (tid == VL_PREP_THREADS - 1) out[ns] = part[tid];
   }
-  int run = part[tid] - sum;  // exclusive prefix of this chunk
-  for (int k = b; k < e; ++k) {
-    a.tile_off[k] = run;
-    run += (a.sys_off[k + 1] - a.sys_off[k] + VL_TILE - 1) / VL_TILE;
-  }
-  if (tid == VL_PREP_THREADS - 1) a.tile_off[ns] = part[tid];
 }
 
 __global__ __launch_bounds__(VL_TILE) void vl_gather_kernel(VlArgs a) {
@@ -212,7 +242,11 @@ struct VlBinLds {
 
 // Systems that need a rebuild (first call, or max displacement > skin/2):
 // geometry for rc + skin and the counting sort, one workgroup per system.
-__global__ __launch_bounds__(PC_BIN_THREADS) void vl_bin_kernel(VlArgs a) {
+// VL_BIN_THREADS lanes and at most 96 VGPRs per lane:
+// small enough to share CUs with kernels on another stream, since only the
+// few systems whose trigger fired are binned in a typical call.
+__global__ __launch_bounds__(VL_BIN_THREADS) __attribute__((amdgpu_num_vgpr(96))) void vl_bin_kernel(
+    VlArgs a) {
   __shared__ VlBinLds L;
   const int tid = threadIdx.x;
   const float half2 = (float)(0.25 * a.skin * a.skin * (1.0 - 1e-6));
@@ -252,22 +286,70 @@ __global__ __launch_bounds__(PC_BIN_THREADS) void vl_bin_kernel(VlArgs a) {
 // Entries are written in groups of four (one 16-byte store per group, groups
 // entry-major over the atoms); the last group is padded with the atom's own
 // slot, masked off by the count.
-__global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
+// STAGED: a workgroup of VL_LIST_THREADS lanes (four tiles of one system)
+// first copies the system's sorted (x, y, z, species) and cell starts into
+// LDS (a.stage atoms, VL_STAGE_MAX at most), so the candidate loads are LDS
+// reads instead of dependent global loads.  When few systems are rebuilt the
+// launch holds few waves, whose global-load latency would not be hidden; the
+// same values give the same lists either way.
+template <bool STAGED>
+__global__ __launch_bounds__(STAGED ? VL_LIST_THREADS : VL_TILE) void vl_list_kernel(VlArgs a) {
+  constexpr int NT = STAGED ? VL_LIST_THREADS : VL_TILE;
   __shared__ SysGeom g;
+  extern __shared__ float4 vl_stage[];
   int s, tile;
-  if (!vl_tile_of(a, s, tile) || !a.need[s]) return;
+  if (!(STAGED ? vl_unit_of(a.tile4_off, a.n_sys, s, tile) : vl_tile_of(a, s, tile)) || !a.need[s])
+    return;
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
   if (n > PC_NMAX) return;
   if (threadIdx.x == 0) g = a.geom[s];
+  const int* csg = a.cstart + (size_t)s * (PC_NCMAX + 1);
+  int* csl = reinterpret_cast<int*>(vl_stage + a.stage);
+  if (STAGED && n > a.stage) {
+    // a system larger than the stated largest: no list (overflow, the pair
+    // passes walk the cell stencil) rather than an LDS overrun
+    const int k = tile * NT + (int)threadIdx.x;
+    if (k < n) {
+      a.sref[i0 + k] = a.spos[i0 + k];
+      a.nlist[i0 + k] = a.cap + 1;
+      atomicAdd(a.counters + 1, 1ull);
+    }
+    return;
+  }
+  if (STAGED) {
+    // four loads in flight per lane
+    for (int k0 = threadIdx.x; k0 < n; k0 += 4 * NT) {
+      float4 p[4];
+      int sp4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = min(k0 + u * NT, n - 1);
+        p[u] = a.spos[i0 + k];
+        sp4[u] = a.sspec[i0 + k];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k0 + u * NT < n) {
+          p[u].w = __int_as_float(sp4[u]);
+          vl_stage[k0 + u * NT] = p[u];
+        }
+      }
+    }
+  }
   __syncthreads();
-  const int slot = tile * VL_TILE + threadIdx.x;
+  if (STAGED) {
+    for (int k = threadIdx.x; k <= g.ncell; k += NT) csl[k] = csg[k];
+    __syncthreads();
+  }
+  const int slot = tile * NT + threadIdx.x;
   if (slot >= n) return;
-  const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
+  const int* cs = STAGED ? csl : csg;
   const float rl = (float)g.rc;  // rc + skin
   const float rl2 = rl * rl * (1.0f + 1e-5f);
   const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
   const int d = i0 + slot;
   const float4* sp = a.spos + i0;
+  const float4* cp = STAGED ? vl_stage : sp;  // candidate positions
   const float4 ri = sp[slot];
   a.sref[d] = ri;
   const int c = a.scell[d];
@@ -321,20 +403,20 @@ __global__ __launch_bounds__(VL_TILE) void vl_list_kernel(VlArgs a) {
                               sy <= VL_SHIFT_MAX && sz >= -VL_SHIFT_MAX && sz <= VL_SHIFT_MAX;
         const int nb = (zz * ncy + yy) * ncx + xx;
         const int tb = cs[nb], te = cs[nb + 1];
-        // four candidates per trip, their positions loaded together (indices
-        // past the cell clamp to its last atom and are skipped)
-        for (int t = tb; t < te; t += 4) {
-          float4 rj[4];
+        // VL_LIST_BATCH candidates per trip, their positions loaded together
+        // (indices past the cell clamp to its last atom and are skipped)
+        for (int t = tb; t < te; t += VL_LIST_BATCH) {
+          float4 rj[VL_LIST_BATCH];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) rj[u] = sp[min(t + u, te - 1)];
+          for (int u = 0; u < VL_LIST_BATCH; ++u) rj[u] = cp[min(t + u, te - 1)];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < VL_LIST_BATCH; ++u) {
             const int tu = t + u;
             if (tu >= te || (center && tu == slot)) continue;
             const float ex = rj[u].x - qx, ey = rj[u].y - qy, ez = rj[u].z - qz;
             if (!(norm2_32(ex, ey, ez) < rl2)) continue;
             if (!packable) cnt = cap + 1;  // cannot be stored: use the stencil
-            push(vl_pack(tu, sx, sy, sz, a.sspec[i0 + tu]));
+            push(vl_pack(tu, sx, sy, sz, STAGED ? __float_as_int(rj[u].w) : a.sspec[i0 + tu]));
           }
         }
       }
@@ -779,7 +861,7 @@ __global__ void vl_reduce_kernel(VlArgs a) {
 struct VlLayout {
   size_t geom, spos, sref, sidx, scell, sspec, slj, scl, cstart, nlist, list, valid, need, reprc,
       maxd2,
-      counters, partial, bandflag, bandbuf, tile_off, total;
+      counters, partial, bandflag, bandbuf, tile_off, tile4_off, total;
   VlLayout(int n_sys, int n_atoms, int cap, int tiles) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
@@ -804,6 +886,7 @@ struct VlLayout {
     bandflag = take(na);
     bandbuf = take(sizeof(uint32_t) * na * VL_NBAND);
     tile_off = take(sizeof(int) * (ns + 1));
+    tile4_off = take(sizeof(int) * (ns + 1));
     total = off;
   }
 };
@@ -818,6 +901,12 @@ extern "C" int kupsmi_verlet_create(kupsmi_verlet** out) {
   KUPSMI_REQUIRE(out != nullptr, "null output");
   *out = new kupsmi_verlet();
   KUPSMI_HIP(hipGetDevice(&(*out)->device));
+  return KUPSMI_OK;
+}
+
+extern "C" int kupsmi_verlet_gate(kupsmi_verlet* v, void* event) {
+  KUPSMI_REQUIRE(v != nullptr, "null handle");
+  v->gate = (hipEvent_t)event;
   return KUPSMI_OK;
 }
 
@@ -975,19 +1064,39 @@ extern "C" int kupsmi_pair_forces_verlet(
   a.bandflag = reinterpret_cast<uint8_t*>(b + lay.bandflag);
   a.bandbuf = reinterpret_cast<uint32_t*>(b + lay.bandbuf);
   a.tile_off = reinterpret_cast<int*>(b + lay.tile_off);
+  a.tile4_off = reinterpret_cast<int*>(b + lay.tile4_off);
   a.forces = forces; a.energy = energy; a.virial = virial;
   int ncu = 256;
   KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   vl_prep_kernel<<<1, VL_PREP_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   // one workgroup per (system, tile), compact: at most one partial tile per system
-  const dim3 grid((unsigned)((n_atoms + VL_TILE - 1) / VL_TILE + n_sys));
+  const dim3 grid(vl_grid(n_atoms, n_sys, VL_TILE));
   vl_gather_kernel<<<grid, VL_TILE, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
-  vl_bin_kernel<<<n_sys < ncu ? n_sys : ncu, PC_BIN_THREADS, 0, st>>>(a);
+  vl_bin_kernel<<<n_sys < ncu ? n_sys : ncu, VL_BIN_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
-  vl_list_kernel<<<grid, VL_TILE, 0, st>>>(a);
-  KUPSMI_CHECK_LAUNCH();
+  {
+    // stage the systems in LDS when the largest one fits beside its cell starts
+    int lds_max = 65536;
+    KUPSMI_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t lds = sizeof(float4) * (size_t)max_atoms_per_sys + sizeof(int) * (PC_NCMAX + 1);
+    // (a gated call shares the GPU with the other stream's work meanwhile:
+    // the unstaged build's small workgroups fit beside it)
+    if (!v->gate && max_atoms_per_sys <= VL_STAGE_MAX &&
+        lds + sizeof(SysGeom) + 256 <= (size_t)lds_max) {
+      a.stage = max_atoms_per_sys;
+      const dim3 grid4(vl_grid(n_atoms, n_sys, VL_LIST_THREADS));
+      vl_list_kernel<true><<<grid4, VL_LIST_THREADS, lds, st>>>(a);
+    } else {
+      vl_list_kernel<false><<<grid, VL_TILE, 0, st>>>(a);
+    }
+    KUPSMI_CHECK_LAUNCH();
+  }
+  if (v->gate) {
+    KUPSMI_HIP(hipStreamWaitEvent(st, v->gate, 0));
+    v->gate = nullptr;
+  }
   const bool excl = a.has_lj_excl || a.has_coul_excl, vir = a.virial != nullptr;
   // main pass, then the band pass; with a pending pair record both run as the
   // recording instantiation (the same pair decisions)

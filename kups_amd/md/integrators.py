@@ -14,6 +14,7 @@ have moved by skin/2 (``skin=0``: a cell list in LDS every step).
 
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
@@ -253,6 +254,12 @@ class EwaldLJPropagator:
                 _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.tril),
                 _lib.ptr(self._alpha), _lib.ptr(self._kmax), mx, my, mz, 0, _lib.ptr(self._f2),
                 _lib.ptr(elr), _lib.ptr(self._vir2) if vir is not None else None, None))
+        if self.overlap and self.verlet is not None:
+            # the pair call's list upkeep runs beside the structure factors,
+            # its pair passes after them, beside the force pass
+            ev = C.c_void_p()
+            _lib.check(lib.kupsmi_ewald_phase_event(ctx, C.byref(ev)))
+            _lib.check(lib.kupsmi_verlet_gate(self.verlet.handle, ev))
         if self.verlet is not None:
             _lib.check(lib.kupsmi_pair_forces_verlet(
                 self.verlet.handle, _lib.stream_ptr(), state.n_sys, _lib.ptr(state.sys_off), state.n,

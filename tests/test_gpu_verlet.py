@@ -104,6 +104,22 @@ def test_verlet_matches_cells_while_atoms_move_and_wrap(gpu):
     assert vl.counters()[0] == rebuilds + 3
 
 
+@pytest.mark.parametrize("ms", [[7], [9, 5]])
+def test_verlet_every_tile_of_few_large_systems(gpu, ms):
+    """Few systems of many tiles (a single 2744-ion melt: 11 tiles of 256;
+    5832 + 1000 ions: 27): every tile of the die-interleaved workgroup order
+    is evaluated (a grid sized to the tile count alone left some out), and
+    the staged list build covers every atom."""
+    b = workloads.nacl_melts(ms=ms, seed=12)
+    c = Case(b, gpu, NACL_LJ, 12.0, alpha=0.3, rc_cl=12.0)
+    vl = VerletList(1.0, neighbor_capacity(b.sizes, b.tril, 12.0, 1.0))
+    ref = c.cells()
+    got = c.verlet(vl)
+    assert (np.abs(got[0]).max(axis=1) > 0).all()  # no atom left without forces
+    _same(got, ref)
+    assert vl.counters() == (len(ms), 0)
+
+
 def test_verlet_overflowed_lists_use_the_stencil(gpu):
     b = workloads.nacl_melts(ms=[5], seed=9)
     c = Case(b, gpu, NACL_LJ, 12.0, alpha=0.3, rc_cl=12.0)
