@@ -28,9 +28,11 @@ import torch
 
 from ..mcmc.gcmc import Motif, RigidGCMC
 from ..potential.lennard_jones import LennardJonesParameters
+from ..core.storage import HDF5StorageWriter
 from .analysis import MCMCAnalysisResult, analyze_mcmc, combine_mcmc
 from .cif import Structure, make_supercell, read_cif
 from .config import McmcConfig
+from .logging import MCMCRecord, make_mcmc_logged_data
 
 
 def host_structure(host) -> Structure:
@@ -127,10 +129,14 @@ def _cycle_length(gcmc: RigidGCMC, min_len: int) -> int:
 
 
 def run(r: McmcRun, *, num_warmup_cycles: int | None = None,
-        num_cycles: int | None = None, stress_every: int = 1) -> McmcRun:
+        num_cycles: int | None = None, stress_every: int = 1,
+        out_file: str | None = None) -> McmcRun:
     """Warm-up then production cycles; per-cycle energy and loading kept,
     and with ``compute_stress`` the guest pressure terms every
-    ``stress_every`` cycles (RigidGCMC.guest_pressure)."""
+    ``stress_every`` cycles (RigidGCMC.guest_pressure).  ``out_file``: HDF5
+    output of every production cycle (``MCMCLoggedData``,
+    application/simulations/mcmc_rigid.py:433-465; the pressure terms are
+    then evaluated every cycle)."""
     cfg = r.config.run
     n_warm = cfg.num_warmup_cycles if num_warmup_cycles is None else int(num_warmup_cycles)
     n_prod = cfg.num_cycles if num_cycles is None else int(num_cycles)
@@ -144,13 +150,29 @@ def run(r: McmcRun, *, num_warmup_cycles: int | None = None,
     e = torch.empty((n_prod, g.n_chain), dtype=torch.float64, device=g.dev)
     n = torch.empty((n_prod, g.n_chain, g.n_motif), dtype=torch.int32, device=g.dev)
     terms = {"potential": [], "tail": [], "ideal": []}
-    for c in range(n_prod):
-        g.run(_cycle_length(g, cfg.min_cycle_length))
-        e[c].copy_(g.d["energy"])
-        n[c].copy_(g.loading_per_motif)
-        if r.config.compute_stress and c % stress_every == 0:
-            for k, v in g.guest_pressure().items():
-                terms[k].append(v)
+    stress = bool(r.config.compute_stress)
+    writer = None
+    if out_file is not None:
+        stress_every = 1
+        z = np.zeros(g.n_chain)
+        writer = HDF5StorageWriter(out_file, make_mcmc_logged_data(stress),
+                                   MCMCRecord(g, {"potential": z, "tail": z, "ideal": z}), n_prod)
+        writer.__enter__()
+    try:
+        for c in range(n_prod):
+            g.run(_cycle_length(g, cfg.min_cycle_length))
+            e[c].copy_(g.d["energy"])
+            n[c].copy_(g.loading_per_motif)
+            p = None
+            if stress and c % stress_every == 0:
+                p = g.guest_pressure()
+                for k, v in p.items():
+                    terms[k].append(v)
+            if writer is not None:
+                writer.log(MCMCRecord(g, p), c)
+    finally:
+        if writer is not None:
+            writer.__exit__(None, None, None)
     r.energy = e.cpu().numpy()
     r.loading_per_species = n.cpu().numpy()
     r.loading = r.loading_per_species.sum(-1)

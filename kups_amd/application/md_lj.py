@@ -27,7 +27,9 @@ from ..md.data import MDState, make_md_state, maxwell_boltzmann_momenta
 from ..md.integrators import (BAOAB_LANGEVIN, CSVR, CSVR_NPT, VELOCITY_VERLET,
                               FusedLJPropagator)
 from ..md.observables import lattice_gradient_stress, volume
+from ..core.storage import HDF5StorageWriter
 from .analysis import MDAnalysisResult, analyze_md, combine_md
+from .logging import MDRecord, make_md_logged_data
 from .cif import Structure, read_cif
 from .config import MdConfig
 from .mcmc_rigid import lj_table
@@ -82,7 +84,10 @@ def build(config: MdConfig, *, replicas: int = 1, seed: int | None = None,
 
 
 def run(r: MdRun, *, num_warmup_steps: int | None = None, num_steps: int | None = None,
-        chunk: int = 10) -> MdRun:
+        chunk: int = 10, out_file: str | None = None) -> MdRun:
+    """Warm-up, then production in launches of ``chunk`` steps;
+    ``out_file``: HDF5 output of the production steps (``MDLoggedData``,
+    application/md/simulation.py:150-170)."""
     cfg = r.config.run
     n_warm = cfg.num_warmup_steps if num_warmup_steps is None else int(num_warmup_steps)
     n_prod = cfg.num_steps if num_steps is None else int(num_steps)
@@ -95,17 +100,32 @@ def run(r: MdRun, *, num_warmup_steps: int | None = None, num_steps: int | None 
     pe, ke, pr, pv = [], [], [], []
     system = torch.repeat_interleave(torch.arange(st.n_sys, device=st.positions.device),
                                      torch.as_tensor(st.sizes, device=st.positions.device))
-    done = 0
-    while done < n_prod:
-        k = min(chunk, n_prod - done)
-        obs = prop(st, k, record=True)
-        pe.append(obs.potential_energy.T)
-        ke.append(obs.kinetic_energy.T)
-        vol = volume(st.tril)
-        pv.append((torch.diagonal(obs.virial, dim1=-2, dim2=-1).sum(-1) / (3.0 * vol))[None, :])
-        sig = lattice_gradient_stress(st.positions, st.forces, obs.virial, st.tril, system)
-        pr.append((torch.diagonal(sig, dim1=-2, dim2=-1).sum(-1) / 3.0)[None, :])
-        done += k
+    writer = None
+    if out_file is not None:
+        z = np.zeros(st.n_sys)
+        writer = HDF5StorageWriter(out_file, make_md_logged_data(chunk, st),
+                                   MDRecord(st, z, z, np.zeros((st.n_sys, 3, 3)), z), n_prod)
+        writer.__enter__()
+    try:
+        done = 0
+        while done < n_prod:
+            k = min(chunk, n_prod - done)
+            obs = prop(st, k, record=True)
+            pe.append(obs.potential_energy.T)
+            ke.append(obs.kinetic_energy.T)
+            vol = volume(st.tril)
+            pv.append((torch.diagonal(obs.virial, dim1=-2, dim2=-1).sum(-1) / (3.0 * vol))[None, :])
+            sig = lattice_gradient_stress(st.positions, st.forces, obs.virial, st.tril, system)
+            pr.append((torch.diagonal(sig, dim1=-2, dim2=-1).sum(-1) / 3.0)[None, :])
+            if writer is not None:
+                pe_k, ke_k = pe[-1].cpu().numpy(), ke[-1].cpu().numpy()
+                sig_k, pv_k = sig.cpu().numpy(), pv[-1][0].cpu().numpy()
+                for j in range(k):
+                    writer.log(MDRecord(st, pe_k[j], ke_k[j], sig_k, pv_k), done + j)
+            done += k
+    finally:
+        if writer is not None:
+            writer.__exit__(None, None, None)
     r.potential_energy = torch.cat(pe).cpu().numpy()
     r.kinetic_energy = torch.cat(ke).cpu().numpy()
     r.pressure = torch.cat(pr).cpu().numpy()
