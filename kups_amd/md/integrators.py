@@ -199,6 +199,10 @@ class EwaldLJPropagator:
         # kernels, into buffers of their own, joined before the closing kick
         # (overlap = False: both on the current stream, one after the other)
         self.overlap = True
+        # with overlap and Verlet lists: hold the pair passes (not the list
+        # upkeep) until the structure factors are done, so the list upkeep
+        # runs beside them and the pair passes beside the force pass
+        self.gate_pairs = True
         self._side = None
         self._f2 = None
         self._vir2 = None
@@ -254,7 +258,7 @@ class EwaldLJPropagator:
                 _lib.ptr(state.positions), _lib.ptr(state.charges), _lib.ptr(state.tril),
                 _lib.ptr(self._alpha), _lib.ptr(self._kmax), mx, my, mz, 0, _lib.ptr(self._f2),
                 _lib.ptr(elr), _lib.ptr(self._vir2) if vir is not None else None, None))
-        if self.overlap and self.verlet is not None:
+        if self.overlap and self.gate_pairs and self.verlet is not None:
             # the pair call's list upkeep runs beside the structure factors,
             # its pair passes after them, beside the force pass
             ev = C.c_void_p()
