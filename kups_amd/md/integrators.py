@@ -199,10 +199,12 @@ class EwaldLJPropagator:
         # kernels, into buffers of their own, joined before the closing kick
         # (overlap = False: both on the current stream, one after the other)
         self.overlap = True
-        # with overlap and Verlet lists: hold the pair passes (not the list
-        # upkeep) until the structure factors are done, so the list upkeep
-        # runs beside them and the pair passes beside the force pass
-        self.gate_pairs = True
+        # with overlap and Verlet lists, True holds the pair passes (not the
+        # list upkeep) until the structure factors are done, so the list
+        # upkeep runs beside them and the pair passes beside the force pass;
+        # False lets the pair passes follow the list upkeep at once (measured
+        # ~2 % faster on the 512-melt workload at skins 1.0 and 1.5 A)
+        self.gate_pairs = False
         self._side = None
         self._f2 = None
         self._vir2 = None
