@@ -14,10 +14,12 @@
 //   bin     one workgroup per system that needs a rebuild (first call, or
 //           maximum displacement > skin/2): geometry for rc + skin and the
 //           counting sort of the cell-list path (cells >= (rc + skin)/2);
-//   list    one lane per atom of those systems appends (j, shift, species) of
-//           each 125-cell-stencil candidate within rc + skin (FP64 test) to
-//           its list, stored entry-major (list[k * n_atoms + d]) so a wave's
-//           loads of entry k are contiguous;
+//   scan    the cells of those systems as one compact work list;
+//   list    one wave per listed cell appends (j, shift, species) of each
+//           stencil candidate within rc + skin (FP32 test with a margin) to
+//           the lists of the cell's atoms, stored in groups of four entries,
+//           entry-major (group k of atom d at k * n_atoms + d) so a wave's
+//           loads of group k are contiguous;
 //   pair    one lane per sorted atom walks its list with the next entry and
 //           position in flight: FP64 pair vector, the pair terms of the
 //           cell-list path; an atom whose list overflowed the capacity walks
@@ -54,17 +56,11 @@ namespace kupsmi {
 constexpr int VL_TILE = 256;
 static_assert(VL_TILE == PC_TILE, "band_ctx locates the left atom by the pair tile");
 constexpr int VL_SHIFT_MAX = 7;  // |image shift| per axis in a packed entry
-constexpr int VL_STAGE_MAX = 8192;  // atoms of a system the list build stages in LDS
-constexpr int VL_LIST_THREADS = 1024;  // staged list build: four tiles of one system per workgroup
 constexpr int VL_BIN_THREADS = 256;
-// candidate loads in flight per lane of the list build (a rebuild of a few
-// systems runs at about one wave per SIMD beside the other stream's kernels)
-constexpr int VL_LIST_BATCH = 8;
 
 struct VlArgs {
   int n_sys, n_atoms, n_species, has_lj, has_coul, has_lj_excl, has_coul_excl, accumulate;
   int tiles, cap;
-  int stage;  // list build: atoms of a system staged in LDS (the launch's largest system)
   double skin;
   const int32_t* sys_off;
   const float* pos;
@@ -97,7 +93,7 @@ struct VlArgs {
   unsigned long long* counters;  // [0] system rebuilds, [1] overflowed atom lists
   double* partial;      // main pass, then the band pass (n_sys x tiles x PC_NPART each)
   int* tile_off;        // n_sys + 1: prefix of the systems' tile counts
-  int* tile4_off;       // n_sys + 1: the same in units of VL_LIST_THREADS atoms
+  int* cell_off;        // n_sys + 1: prefix of the cell counts of the systems rebuilt in a call
   // per sorted atom: the number of list entries inside a cutoff band met by
   // the main pass (VL_NBAND_FULL: more than VL_NBAND, or met on the stencil
   // path), and up to VL_NBAND of those entries
@@ -164,37 +160,52 @@ inline unsigned vl_grid(int n_atoms, int n_sys, int unit) {
   return (unsigned)(VL_XCDS * ((t + VL_XCDS - 1) / VL_XCDS));
 }
 
-// One workgroup: clears the per-system displacement maxima and writes the
-// tile prefix (ceil(n_s / VL_TILE) per system; chunked block scan).
-__global__ __launch_bounds__(VL_PREP_THREADS) void vl_prep_kernel(VlArgs a) {
-  __shared__ int part[VL_PREP_THREADS];
-  const int tid = threadIdx.x, ns = a.n_sys;
-  for (int k = tid; k < ns; k += VL_PREP_THREADS) a.maxd2[k] = 0u;
+// Exclusive prefix of width(s) over the n_sys systems into out[0..n_sys], by
+// one workgroup of VL_PREP_THREADS lanes (each lane sums a chunk of systems,
+// then a block scan of the chunk sums).
+template <class Width>
+__device__ void vl_block_prefix(int ns, int* out, Width width, int* part) {
+  const int tid = threadIdx.x;
   const int chunk = (ns + VL_PREP_THREADS - 1) / VL_PREP_THREADS;
   const int b = tid * chunk, e = min(b + chunk, ns);
-  // the tile prefix (VL_TILE atoms) and the list build's unit prefix
-  // (VL_LIST_THREADS atoms)
-  for (int pass = 0; pass < 2; ++pass) {
-    const int unit = pass ? VL_LIST_THREADS : VL_TILE;
-    int* out = pass ? a.tile4_off : a.tile_off;
-    int sum = 0;
-    for (int k = b; k < e; ++k) sum += (a.sys_off[k + 1] - a.sys_off[k] + unit - 1) / unit;
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += width(k);
+  __syncthreads();
+  part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < VL_PREP_THREADS; o <<= 1) {
+    const int v = tid >= o ? part[tid - o] : 0;
     __syncthreads();
-    part[tid] = sum;
+    part[tid] += v;
     __syncthreads();
-    for (int o = 1; o < VL_PREP_THREADS; o <<= 1) {
-      const int v = tid >= o ? part[tid - o] : 0;
-      __syncthreads();
-      part[tid] += v;
-      __syncthreads();
-    }
-    int run = part[tid] - sum;  // exclusive prefix of this chunk
-    for (int k = b; k < e; ++k) {
-      out[k] = run;
-      run += (a.sys_off[k + 1] - a.sys_off[k] + unit - 1) / unit;
-    }
-    if (tid == VL_PREP_THREADS - 1) out[ns] = part[tid];
   }
+  int run = part[tid] - sum;  // exclusive prefix of this chunk
+  for (int k = b; k < e; ++k) {
+    out[k] = run;
+    run += width(k);
+  }
+  if (tid == VL_PREP_THREADS - 1) out[ns] = part[tid];
+}
+
+// One workgroup: clears the per-system displacement maxima and writes the
+// tile prefix (ceil(n_s / VL_TILE) per system).
+__global__ __launch_bounds__(VL_PREP_THREADS) void vl_prep_kernel(VlArgs a) {
+  __shared__ int part[VL_PREP_THREADS];
+  for (int k = threadIdx.x; k < a.n_sys; k += VL_PREP_THREADS) a.maxd2[k] = 0u;
+  vl_block_prefix(a.n_sys, a.tile_off, [&](int k) {
+    return (a.sys_off[k + 1] - a.sys_off[k] + VL_TILE - 1) / VL_TILE;
+  }, part);
+}
+
+// One workgroup, after the binning: the work list of the list build, the
+// prefix of the cell counts of the systems rebuilt in this call (0 for the
+// others), so its workers walk only those cells.
+__global__ __launch_bounds__(VL_PREP_THREADS) void vl_cell_scan_kernel(VlArgs a) {
+  __shared__ int part[VL_PREP_THREADS];
+  vl_block_prefix(a.n_sys, a.cell_off, [&](int k) {
+    const int n = a.sys_off[k + 1] - a.sys_off[k];
+    return a.need[k] && n > 0 && n <= PC_NMAX ? a.geom[k].ncell : 0;
+  }, part);
 }
 
 __global__ __launch_bounds__(VL_TILE) void vl_gather_kernel(VlArgs a) {
@@ -278,159 +289,245 @@ __global__ __launch_bounds__(VL_BIN_THREADS) __attribute__((amdgpu_num_vgpr(96))
   }
 }
 
-// Lists of the rebuilt systems: one lane per sorted atom appends every
-// candidate of the 125-cell stencil within rc + skin.  The test is FP32 with
-// a relative margin of 1e-5 on (rc + skin)^2: the list only has to hold
+// Lists of the rebuilt systems, one wave per cell of the compact work list
+// (cell_off).  The wave walks the cell's stencil once per group of up to 64
+// of its atoms: row by row (dz, dy), each row of 2 h0 + 1 cells one
+// contiguous slot range of the sorted system per image shift along x (one
+// piece, two where it wraps), it copies the candidates, moved by their image
+// shift and with their packed entries, into a window of VL_LC_W in LDS (the
+// stencil's pieces back to back, so the chunks below are full whatever the
+// row lengths).  Each time the window fills, and at the end, the group's
+// atoms take it VL_LC_AB at a time (positions and counts uniform) in chunks
+// of 64 candidates; the accepted lanes of a chunk append their entries in
+// lane order (ballot and prefix count), so an atom's entries come in the
+// stencil order (dz, dy, dx) and slot order within a cell.  The test is FP32
+// with a relative margin of 1e-5 on (rc + skin)^2: the list only has to hold
 // every pair that can come within rc before the next rebuild (a few pairs
 // just beyond rc + skin cost nothing), the pair pass decides the cutoff.
-// Entries are written in groups of four (one 16-byte store per group, groups
-// entry-major over the atoms); the last group is padded with the atom's own
-// slot, masked off by the count.
-// STAGED: a workgroup of VL_LIST_THREADS lanes (four tiles of one system)
-// first copies the system's sorted (x, y, z, species) and cell starts into
-// LDS (a.stage atoms, VL_STAGE_MAX at most), so the candidate loads are LDS
-// reads instead of dependent global loads.  When few systems are rebuilt the
-// launch holds few waves, whose global-load latency would not be hidden; the
-// same values give the same lists either way.
-template <bool STAGED>
-__global__ __launch_bounds__(STAGED ? VL_LIST_THREADS : VL_TILE) void vl_list_kernel(VlArgs a) {
-  constexpr int NT = STAGED ? VL_LIST_THREADS : VL_TILE;
-  __shared__ SysGeom g;
-  extern __shared__ float4 vl_stage[];
-  int s, tile;
-  if (!(STAGED ? vl_unit_of(a.tile4_off, a.n_sys, s, tile) : vl_tile_of(a, s, tile)) || !a.need[s])
-    return;
-  const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
-  if (n > PC_NMAX) return;
-  if (threadIdx.x == 0) g = a.geom[s];
-  const int* csg = a.cstart + (size_t)s * (PC_NCMAX + 1);
-  int* csl = reinterpret_cast<int*>(vl_stage + a.stage);
-  if (STAGED && n > a.stage) {
-    // a system larger than the stated largest: no list (overflow, the pair
-    // passes walk the cell stencil) rather than an LDS overrun
-    const int k = tile * NT + (int)threadIdx.x;
-    if (k < n) {
-      a.sref[i0 + k] = a.spos[i0 + k];
-      a.nlist[i0 + k] = a.cap + 1;
-      atomicAdd(a.counters + 1, 1ull);
-    }
-    return;
-  }
-  if (STAGED) {
-    // four loads in flight per lane
-    for (int k0 = threadIdx.x; k0 < n; k0 += 4 * NT) {
-      float4 p[4];
-      int sp4[4];
+// Orthorhombic cells skip the rows and the cells of a row whose gap to the
+// atom's cell exceeds rc + skin (margin 1e-4 L for a cell assignment a few
+// ulp off a face).  Entries live in groups of four, groups entry-major over
+// the atoms (entry e of atom d at word 4 ((e >> 2) n_atoms + d) + (e & 3));
+// the last group is padded with the atom's own slot, masked off by the count.
+// WIDE: 64-bit word offsets (large batches).
+//
+// The grid is persistent, VL_XCDS workgroups per round: die x (workgroups
+// w with w % VL_XCDS == x) takes the x-th eighth of the work list, so each die
+// walks a contiguous stretch of systems, and its waves take its cells in
+// turn until the list ends (a call that rebuilds nothing costs one load per
+// wave).  The build is latency-bound (a chain of ballot, prefix count and
+// store per atom and chunk): it runs best at five waves per SIMD, so its
+// registers and LDS are sized for that.
+constexpr int VL_LC_THREADS = 256;  // four workers (waves) per workgroup
+constexpr int VL_LC_AB = 8;         // atoms of a cell per pass over the window
+constexpr int VL_LC_W = 512;        // candidates per window (8 KB of LDS per wave)
+constexpr int VL_LC_WG_PER_CU = 4;  // persistent workgroups per CU (measured: 5 is no faster)
+constexpr uint32_t VL_UNPACKABLE = 0xffffffffu;  // entry of an image shift beyond VL_SHIFT_MAX
+
+// One chunk of 64 window candidates (this lane's: position xj, yj, zj and
+// packed entry ent) against the block's nl atoms: entries appended in lane
+// order at each atom's count (entry e of block atom k at word
+// 4 (e >> 2) N + 4 k + (e & 3) of blk).  SELF: the chunk may hold the block's
+// atoms themselves, atom k at lane offset own == k.  BAD: the chunk may hold
+// entries of unpackable shifts; a pair on one sends the atom to the stencil.
+template <bool WIDE, bool SELF, bool BAD>
+__device__ __forceinline__ void vl_list_chunk(uint32_t* blk, unsigned N, int cap, int nl, int own,
+                                              float xj, float yj, float zj, uint32_t ent,
+                                              const float (&rx)[VL_LC_AB],
+                                              const float (&ry)[VL_LC_AB],
+                                              const float (&rz)[VL_LC_AB], float rl2,
+                                              int (&cnt)[VL_LC_AB]) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = min(k0 + u * NT, n - 1);
-        p[u] = a.spos[i0 + k];
-        sp4[u] = a.sspec[i0 + k];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (k0 + u * NT < n) {
-          p[u].w = __int_as_float(sp4[u]);
-          vl_stage[k0 + u * NT] = p[u];
+  for (int k = 0; k < VL_LC_AB; ++k) {
+    if (k < nl) {
+      bool acc = norm2_32(xj - rx[k], yj - ry[k], zj - rz[k]) < rl2;
+      if (SELF) acc = acc && own != k;
+      const uint64_t m = __builtin_amdgcn_ballot_w64(acc);
+      if (m) {
+        const int c0 = cnt[k];
+        cnt[k] = c0 + (int)__popcll(m);
+        if (BAD && __builtin_amdgcn_ballot_w64(acc && ent == VL_UNPACKABLE)) {
+          cnt[k] = max(cnt[k], cap + 1);
+          continue;
         }
-      }
-    }
-  }
-  __syncthreads();
-  if (STAGED) {
-    for (int k = threadIdx.x; k <= g.ncell; k += NT) csl[k] = csg[k];
-    __syncthreads();
-  }
-  const int slot = tile * NT + threadIdx.x;
-  if (slot >= n) return;
-  const int* cs = STAGED ? csl : csg;
-  const float rl = (float)g.rc;  // rc + skin
-  const float rl2 = rl * rl * (1.0f + 1e-5f);
-  const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
-  const int d = i0 + slot;
-  const float4* sp = a.spos + i0;
-  const float4* cp = STAGED ? vl_stage : sp;  // candidate positions
-  const float4 ri = sp[slot];
-  a.sref[d] = ri;
-  const int c = a.scell[d];
-  const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-  uint4* list4 = reinterpret_cast<uint4*>(a.list);
-  const size_t stride = (size_t)a.n_atoms;
-  const int cap = a.cap;  // a multiple of four
-  // the group being filled, as a shift register
-  uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-  int cnt = 0;
-  auto push = [&](uint32_t e) {
-    b0 = b1; b1 = b2; b2 = b3; b3 = e;
-    if ((cnt & 3) == 3 && cnt < cap) list4[(size_t)(cnt >> 2) * stride + d] = make_uint4(b0, b1, b2, b3);
-    ++cnt;
-  };
-  // orthorhombic cells: skip stencil cells whose box (with its image shift)
-  // lies farther than rc + skin from the atom.  Cell k of an axis spans
-  // [(k / nc - 1/2) L, ((k + 1) / nc - 1/2) L) in the wrapped frame; the
-  // margin of 1e-4 L covers an FP32 cell assignment a few ulp off a face.
-  const bool prune = g.ortho;
-  const float lim = prune ? (rl * (1.0f + 1e-5f) + 1e-4f * fmaxf(g.Lf[0], fmaxf(g.Lf[2], g.Lf[5]))) *
-                                (rl * (1.0f + 1e-5f) + 1e-4f * fmaxf(g.Lf[0], fmaxf(g.Lf[2], g.Lf[5])))
-                          : INFINITY;
-  auto box_d2 = [&](int k, int nc, float L, float x) {
-    const float lo = ((float)k / (float)nc - 0.5f) * L, hi = ((float)(k + 1) / (float)nc - 0.5f) * L;
-    const float t = fmaxf(fmaxf(lo - x, x - hi), 0.0f);
-    return t * t;
-  };
-  for (int dz = -g.h[2]; dz <= g.h[2]; ++dz) {
-    const float bz = prune ? box_d2(cz + dz, ncz, g.Lf[5], ri.z) : 0.0f;
-    if (bz > lim) continue;
-    int zz = cz + dz;
-    const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
-    zz -= sz * ncz;
-    for (int dy = -g.h[1]; dy <= g.h[1]; ++dy) {
-      const float byz = bz + (prune ? box_d2(cy + dy, ncy, g.Lf[2], ri.y) : 0.0f);
-      if (byz > lim) continue;
-      int yy = cy + dy;
-      const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
-      yy -= sy * ncy;
-      for (int dx = -g.h[0]; dx <= g.h[0]; ++dx) {
-        if (byz + (prune ? box_d2(cx + dx, ncx, g.Lf[0], ri.x) : 0.0f) > lim) continue;
-        int xx = cx + dx;
-        const int sx = xx >= 0 ? xx / ncx : -((-xx + ncx - 1) / ncx);
-        xx -= sx * ncx;
-        float Sx, Sy, Sz;
-        shift_vec32(g.Lf, sx, sy, sz, Sx, Sy, Sz);
-        const float qx = ri.x - Sx, qy = ri.y - Sy, qz = ri.z - Sz;
-        const bool center = dx == 0 && dy == 0 && dz == 0;
-        const bool packable = sx >= -VL_SHIFT_MAX && sx <= VL_SHIFT_MAX && sy >= -VL_SHIFT_MAX &&
-                              sy <= VL_SHIFT_MAX && sz >= -VL_SHIFT_MAX && sz <= VL_SHIFT_MAX;
-        const int nb = (zz * ncy + yy) * ncx + xx;
-        const int tb = cs[nb], te = cs[nb + 1];
-        // VL_LIST_BATCH candidates per trip, their positions loaded together
-        // (indices past the cell clamp to its last atom and are skipped)
-        for (int t = tb; t < te; t += VL_LIST_BATCH) {
-          float4 rj[VL_LIST_BATCH];
-#pragma unroll
-          for (int u = 0; u < VL_LIST_BATCH; ++u) rj[u] = cp[min(t + u, te - 1)];
-#pragma unroll
-          for (int u = 0; u < VL_LIST_BATCH; ++u) {
-            const int tu = t + u;
-            if (tu >= te || (center && tu == slot)) continue;
-            const float ex = rj[u].x - qx, ey = rj[u].y - qy, ez = rj[u].z - qz;
-            if (!(norm2_32(ex, ey, ez) < rl2)) continue;
-            if (!packable) cnt = cap + 1;  // cannot be stored: use the stencil
-            push(vl_pack(tu, sx, sy, sz, STAGED ? __float_as_int(rj[u].w) : a.sspec[i0 + tu]));
+        if (c0 < cap) {
+          const unsigned e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)c0));
+          if (acc && e < (unsigned)cap) {
+            if (WIDE) blk[4 * k + ((size_t)(e >> 2) * N << 2) + (e & 3)] = ent;
+            else blk[4 * k + (size_t)(__umul24(e >> 2, N << 2) + (e & 3))] = ent;
           }
         }
       }
     }
   }
-  // pad and store the last, partial group
-  if ((cnt & 3) && cnt < cap) {
-    const uint32_t pad = vl_pack(slot, 0, 0, 0, a.sspec[d]);
-    const int full = cnt;
-    while (cnt & 3) push(pad);
-    cnt = full;
+}
+
+template <bool WIDE>
+__global__ __launch_bounds__(VL_LC_THREADS) void vl_list_kernel(VlArgs a) {
+  __shared__ float4 vl_win[VL_LC_THREADS / 64][VL_LC_W];
+  const int T = a.cell_off[a.n_sys];
+  if (T == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  float4* win = vl_win[wv];
+  const int die = blockIdx.x % VL_XCDS;
+  const int wpd = (int)(gridDim.x / VL_XCDS) * (VL_LC_THREADS / 64);  // workers per die
+  const int per = (T + VL_XCDS - 1) / VL_XCDS;
+  const int u_end = min(T, (die + 1) * per);
+  const unsigned N = (unsigned)a.n_atoms;
+  const int cap = a.cap;  // a multiple of four
+  int s = -1;
+  for (int u = die * per + (int)(blockIdx.x / VL_XCDS) * (VL_LC_THREADS / 64) + wv; u < u_end;
+       u += wpd) {
+    // the system of cell u: the last one whose first cell is <= u (the
+    // units of one worker ascend, so after the first search it only advances)
+    if (s < 0) {
+      int lo = 0, hi = a.n_sys;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.cell_off[mid] <= u) lo = mid; else hi = mid;
+      }
+      s = lo;
+    } else {
+      while (a.cell_off[s + 1] <= u) ++s;
+    }
+    const int c = u - a.cell_off[s];
+    const int i0 = a.sys_off[s];
+    const int* cs = a.cstart + (size_t)s * (PC_NCMAX + 1);
+    const int a0 = cs[c], a1 = cs[c + 1];
+    if (a0 == a1) continue;
+    const SysGeom& g = a.geom[s];
+    const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+    const int h0 = g.h[0], h1 = g.h[1], h2 = g.h[2];
+    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    const float Lf[6] = {g.Lf[0], g.Lf[1], g.Lf[2], g.Lf[3], g.Lf[4], g.Lf[5]};
+    const float rl = (float)g.rc;  // rc + skin
+    const float rl2 = rl * rl * (1.0f + 1e-5f);
+    const bool prune = g.ortho;
+    const float lm = rl * (1.0f + 1e-5f) + 1e-4f * fmaxf(Lf[0], fmaxf(Lf[2], Lf[5]));
+    const float lim = prune ? lm * lm : INFINITY;
+    // squared gap along an axis between cell k and cell k + d (of width w)
+    const float wx = Lf[0] / (float)ncx, wy = Lf[2] / (float)ncy, wz = Lf[5] / (float)ncz;
+    auto gap2 = [](int d, float w) {
+      const float t = (float)max(abs(d) - 1, 0) * w;
+      return t * t;
+    };
+    const float4* sp = a.spos + i0;
+    const int* ssp = a.sspec + i0;
+    for (int ag = a0; ag < a1; ag += 64) {
+      const int na = min(64, a1 - ag);
+      int vcnt = 0;          // lane k: entries of atom ag + k so far
+      int wn = 0;            // candidates in the window
+      int G = 0;             // stream position of the window's first candidate
+      int selfp = -(1 << 30);  // stream position of slot ag (its own cell, zero shift)
+      bool bad = false;      // the window holds entries of unpackable shifts
+      // the group's atoms against the window, VL_LC_AB at a time
+      auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int kb = 0; kb < na; kb += VL_LC_AB) {
+          const int nl = min(VL_LC_AB, na - kb);
+          float rx[VL_LC_AB], ry[VL_LC_AB], rz[VL_LC_AB];
+          int cnt[VL_LC_AB];
+#pragma unroll
+          for (int k = 0; k < VL_LC_AB; ++k) {
+            const float4 p = sp[ag + kb + min(k, nl - 1)];
+            rx[k] = p.x; ry[k] = p.y; rz[k] = p.z;
+            cnt[k] = __builtin_amdgcn_readlane(vcnt, kb + min(k, nl - 1));
+          }
+          uint32_t* blk = a.list + 4 * ((size_t)i0 + ag + kb);  // group 0 of the block's atom 0
+          const int sb = selfp + kb - G;  // window offset of the block's atom 0 itself
+          for (int c0 = 0; c0 < wn; c0 += 64) {
+            const float4 q = win[min(c0 + lane, wn - 1)];
+            const float xj = c0 + lane < wn ? q.x : 1e30f;  // lanes past the window: far
+            const uint32_t ent = __float_as_uint(q.w);
+            const int own = c0 + lane - sb;
+            const bool self = sb + nl > c0 && sb < c0 + 64;
+            if (bad) {
+              vl_list_chunk<WIDE, true, true>(blk, N, cap, nl, own, xj, q.y, q.z, ent, rx, ry, rz,
+                                              rl2, cnt);
+            } else if (self) {
+              vl_list_chunk<WIDE, true, false>(blk, N, cap, nl, own, xj, q.y, q.z, ent, rx, ry, rz,
+                                               rl2, cnt);
+            } else {
+              vl_list_chunk<WIDE, false, false>(blk, N, cap, nl, own, xj, q.y, q.z, ent, rx, ry,
+                                                rz, rl2, cnt);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < VL_LC_AB; ++k)
+            if (k < nl) vcnt = lane == kb + k ? cnt[k] : vcnt;
+        }
+        __builtin_amdgcn_wave_barrier();
+        G += wn;
+        wn = 0;
+        bad = false;
+      };
+      for (int dz = -h2; dz <= h2; ++dz) {
+        const float gz = prune ? gap2(dz, wz) : 0.0f;
+        if (gz > lim) continue;
+        int zz = cz + dz;
+        const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
+        zz -= sz * ncz;
+        for (int dy = -h1; dy <= h1; ++dy) {
+          const float gyz = gz + (prune ? gap2(dy, wy) : 0.0f);
+          if (gyz > lim) continue;
+          int yy = cy + dy;
+          const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
+          yy -= sy * ncy;
+          int dxa = -h0, dxb = h0;
+          if (prune) {
+            while (dxa < 0 && gyz + gap2(dxa, wx) > lim) ++dxa;
+            while (dxb > 0 && gyz + gap2(dxb, wx) > lim) --dxb;
+          }
+          const int* row = cs + (zz * ncy + yy) * ncx;
+          // pieces of constant image shift along x, in dx order
+          for (int xs = cx + dxa; xs <= cx + dxb;) {
+            const int sx = xs >= 0 ? xs / ncx : -((-xs + ncx - 1) / ncx);
+            const int xe = min(cx + dxb, (sx + 1) * ncx - 1);
+            const int tb = row[xs - sx * ncx], te = row[xe - sx * ncx + 1];
+            xs = xe + 1;
+            if (tb == te) continue;
+            float Sx, Sy, Sz;
+            shift_vec32(Lf, sx, sy, sz, Sx, Sy, Sz);
+            const bool packable = sx >= -VL_SHIFT_MAX && sx <= VL_SHIFT_MAX && sy >= -VL_SHIFT_MAX &&
+                                  sy <= VL_SHIFT_MAX && sz >= -VL_SHIFT_MAX && sz <= VL_SHIFT_MAX;
+            const uint32_t bits = vl_pack(0, sx, sy, sz, 0);
+            // the group's atoms themselves: in their own cell at zero shift
+            if (sx == 0 && sy == 0 && sz == 0 && tb <= ag && ag < te) selfp = G + wn + (ag - tb);
+            for (int t0 = tb; t0 < te; t0 += 64) {
+              const int n = min(64, te - t0);
+              if (wn + n > VL_LC_W) flush();
+              if (lane < n) {
+                const int t = t0 + lane;
+                const float4 pj = sp[t];
+                const uint32_t ent = packable ? (uint32_t)t | bits | ((uint32_t)ssp[t] << 27)
+                                              : VL_UNPACKABLE;
+                win[wn + lane] = make_float4(pj.x + Sx, pj.y + Sy, pj.z + Sz, __uint_as_float(ent));
+              }
+              wn += n;
+              bad = bad || !packable;
+            }
+          }
+        }
+      }
+      if (wn > 0) flush();
+      // per atom of the group (lane k): pad the last group, count, reference
+      if (lane < na) {
+        const int cn = vcnt;
+        const int slot = ag + lane;
+        const size_t d = (size_t)i0 + slot;
+        if ((cn & 3) && cn < cap) {
+          const uint32_t pad = vl_pack(slot, 0, 0, 0, ssp[slot]);
+          for (int e = cn; e & 3; ++e) a.list[(((size_t)(e >> 2) * N + d) << 2) + (e & 3)] = pad;
+        }
+        a.nlist[d] = cn;
+        a.sref[d] = sp[slot];
+        if (cn > cap) atomicAdd(a.counters + 1, 1ull);
+      }
+    }
   }
-  a.nlist[d] = cnt;
-  if (cnt > a.cap) atomicAdd(a.counters + 1, 1ull);
 }
 
 // relative half width of the FP32 cutoff band re-decided in FP64: the FP32
@@ -861,7 +958,7 @@ __global__ void vl_reduce_kernel(VlArgs a) {
 struct VlLayout {
   size_t geom, spos, sref, sidx, scell, sspec, slj, scl, cstart, nlist, list, valid, need, reprc,
       maxd2,
-      counters, partial, bandflag, bandbuf, tile_off, tile4_off, total;
+      counters, partial, bandflag, bandbuf, tile_off, cell_off, total;
   VlLayout(int n_sys, int n_atoms, int cap, int tiles) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
@@ -886,7 +983,7 @@ struct VlLayout {
     bandflag = take(na);
     bandbuf = take(sizeof(uint32_t) * na * VL_NBAND);
     tile_off = take(sizeof(int) * (ns + 1));
-    tile4_off = take(sizeof(int) * (ns + 1));
+    cell_off = take(sizeof(int) * (ns + 1));
     total = off;
   }
 };
@@ -1064,7 +1161,7 @@ extern "C" int kupsmi_pair_forces_verlet(
   a.bandflag = reinterpret_cast<uint8_t*>(b + lay.bandflag);
   a.bandbuf = reinterpret_cast<uint32_t*>(b + lay.bandbuf);
   a.tile_off = reinterpret_cast<int*>(b + lay.tile_off);
-  a.tile4_off = reinterpret_cast<int*>(b + lay.tile4_off);
+  a.cell_off = reinterpret_cast<int*>(b + lay.cell_off);
   a.forces = forces; a.energy = energy; a.virial = virial;
   int ncu = 256;
   KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1076,21 +1173,17 @@ extern "C" int kupsmi_pair_forces_verlet(
   KUPSMI_CHECK_LAUNCH();
   vl_bin_kernel<<<n_sys < ncu ? n_sys : ncu, VL_BIN_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
+  vl_cell_scan_kernel<<<1, VL_PREP_THREADS, 0, st>>>(a);
+  KUPSMI_CHECK_LAUNCH();
   {
-    // stage the systems in LDS when the largest one fits beside its cell starts
-    int lds_max = 65536;
-    KUPSMI_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const size_t lds = sizeof(float4) * (size_t)max_atoms_per_sys + sizeof(int) * (PC_NCMAX + 1);
-    // (a gated call shares the GPU with the other stream's work meanwhile:
-    // the unstaged build's small workgroups fit beside it)
-    if (!v->gate && max_atoms_per_sys <= VL_STAGE_MAX &&
-        lds + sizeof(SysGeom) + 256 <= (size_t)lds_max) {
-      a.stage = max_atoms_per_sys;
-      const dim3 grid4(vl_grid(n_atoms, n_sys, VL_LIST_THREADS));
-      vl_list_kernel<true><<<grid4, VL_LIST_THREADS, lds, st>>>(a);
-    } else {
-      vl_list_kernel<false><<<grid, VL_TILE, 0, st>>>(a);
-    }
+    // persistent list build: VL_LC_WG_PER_CU workgroups per CU, a whole
+    // number of rounds over the dies
+    const unsigned wgs = (unsigned)(VL_XCDS * ((VL_LC_WG_PER_CU * ncu + VL_XCDS - 1) / VL_XCDS));
+    // 32-bit entry offsets: 4 n_atoms a 24-bit factor, cap n_atoms words
+    const bool narrow = 4ull * (unsigned)n_atoms < (1ull << 24) &&
+                        (unsigned long long)v->cap * (unsigned)n_atoms < (1ull << 32);
+    if (narrow) vl_list_kernel<false><<<wgs, VL_LC_THREADS, 0, st>>>(a);
+    else vl_list_kernel<true><<<wgs, VL_LC_THREADS, 0, st>>>(a);
     KUPSMI_CHECK_LAUNCH();
   }
   if (v->gate) {

@@ -104,12 +104,38 @@ def test_verlet_matches_cells_while_atoms_move_and_wrap(gpu):
     assert vl.counters()[0] == rebuilds + 3
 
 
+def test_verlet_rebuilds_only_the_triggered_systems(gpu):
+    """One system of four moved beyond skin/2: only its list is rebuilt (the
+    list build walks the cells of the triggered systems alone) while the
+    others keep theirs, and every system still matches the cell-list path."""
+    b = workloads.nacl_melts(ms=[5, 6, 5, 7], seed=13)
+    c = Case(b, gpu, NACL_LJ, 12.0, alpha=0.3, rc_cl=12.0)
+    skin = 1.0
+    vl = VerletList(skin, neighbor_capacity(b.sizes, b.tril, 12.0, skin))
+    _same(c.verlet(vl), c.cells())
+    assert vl.counters() == (4, 0)
+    off = np.concatenate([[0], np.cumsum(b.sizes)])
+    rng = np.random.default_rng(2)
+    for k, s in enumerate((2, 0, 3)):
+        step = np.zeros((c.n, 3), np.float32)
+        step[off[s]:off[s + 1]] = rng.normal(0, 0.5, (b.sizes[s], 3)).clip(-1.5, 1.5)
+        # the other systems move a little: at most 3 x 0.05 sqrt(3) < skin/2
+        # in all (reuse; wraps near the faces)
+        mask = np.ones(c.n, bool)
+        mask[off[s]:off[s + 1]] = False
+        step[mask] = rng.normal(0, 0.02, (mask.sum(), 3)).clip(-0.05, 0.05)
+        c.pos = (c.pos + torch.as_tensor(step, device=gpu)).contiguous()
+        c.wrap()
+        _same(c.verlet(vl), c.cells())
+        assert vl.counters() == (4 + k + 1, 0)
+
+
 @pytest.mark.parametrize("ms", [[7], [9, 5]])
 def test_verlet_every_tile_of_few_large_systems(gpu, ms):
     """Few systems of many tiles (a single 2744-ion melt: 11 tiles of 256;
     5832 + 1000 ions: 27): every tile of the die-interleaved workgroup order
     is evaluated (a grid sized to the tile count alone left some out), and
-    the staged list build covers every atom."""
+    the list build covers every atom."""
     b = workloads.nacl_melts(ms=ms, seed=12)
     c = Case(b, gpu, NACL_LJ, 12.0, alpha=0.3, rc_cl=12.0)
     vl = VerletList(1.0, neighbor_capacity(b.sizes, b.tril, 12.0, 1.0))
