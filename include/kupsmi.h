@@ -394,8 +394,9 @@ int kupsmi_ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
                        double* structure_factor);
 /* The context's phase event (a hipEvent_t, owned by the context): every
  * kupsmi_ewald_recip on ctx records it on its stream once the structure
- * factors and energies are done, before the force pass.  No counterpart in
- * the reference (a scheduling hook, see kupsmi_verlet_gate). */
+ * factors, energies and force coefficients are done, right before the force
+ * pass.  No counterpart in the reference (a scheduling hook, see
+ * kupsmi_verlet_gate). */
 int kupsmi_ewald_phase_event(kupsmi_ctx* ctx, void** event);
 
 /*

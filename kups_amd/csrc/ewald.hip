@@ -949,9 +949,16 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
   KUPSMI_CHECK_LAUNCH();
   ek_sk_reduce_kernel<<<dim3((a.nk + 255) / 256, n_sys), 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
+  // energies, virial and the force coefficients P(k) S(k) of the force pass
   ek_energy_kernel<<<n_sys, EK_THREADS, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
-  if (ctx && ctx->phase) KUPSMI_HIP(hipEventRecord(ctx->phase, st));
+  // the phase event: all but the force pass is done (recorded just before
+  // it, so work gated on it starts beside the force pass, and the small
+  // kernels before it run on an idle GPU)
+  auto phase = [&]() -> int {
+    if (ctx && ctx->phase) KUPSMI_HIP(hipEventRecord(ctx->phase, st));
+    return KUPSMI_OK;
+  };
   if (ef_mfma) {
     const size_t lds = ef_lds_bytes(ef);
     static std::once_flag lds_attr;
@@ -961,11 +968,13 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
     });
     ek_coef_kernel<<<dim3((ef.mpad + 127) / 128, n_sys), 128, 0, st>>>(a, ef, ef_blocks);
     KUPSMI_CHECK_LAUNCH();
+    if (int rc = phase()) return rc;
     const int per_wg = EF_WAVES * EF_NA * 16 * EF_CHUNKS;
     ek_force_mfma_kernel<<<dim3((max_atoms_per_sys + per_wg - 1) / per_wg, n_sys),
                            EF_WAVES * 64, lds, st>>>(a, ef, ef_blocks);
     KUPSMI_CHECK_LAUNCH();
   } else if (forces && max_atoms_per_sys > 0) {
+    if (int rc = phase()) return rc;
     dim3 g2((max_atoms_per_sys + EK_THREADS - 1) / EK_THREADS, n_sys);
     const size_t lds = sizeof(float) * coef_floats;
     if (lds <= 64 * 1024) {
@@ -975,6 +984,8 @@ static int ewald_recip(kupsmi_ctx* ctx, void* stream, int n_sys, const int32_t* 
       ek_force_kernel<0, false><<<g2, EK_THREADS, 0, st>>>(a);
     }
     KUPSMI_CHECK_LAUNCH();
+  } else if (int rc = phase()) {
+    return rc;
   }
   return KUPSMI_OK;
 }

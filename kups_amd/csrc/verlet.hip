@@ -99,6 +99,11 @@ struct VlArgs {
   // path), and up to VL_NBAND of those entries
   uint8_t* bandflag;
   uint32_t* bandbuf;
+  // tiles with lanes the band pass must walk in full (compact list, its
+  // length, and a flag per (system, tile) for the reduction)
+  int* band_tiles;
+  int* band_count;
+  uint8_t* tile_full;
   float* forces;
   double* energy;
   double* virial;
@@ -187,11 +192,13 @@ __device__ void vl_block_prefix(int ns, int* out, Width width, int* part) {
   if (tid == VL_PREP_THREADS - 1) out[ns] = part[tid];
 }
 
-// One workgroup: clears the per-system displacement maxima and writes the
-// tile prefix (ceil(n_s / VL_TILE) per system).
+// One workgroup: clears the per-system displacement maxima and the band
+// pass's tile count and writes the tile prefix (ceil(n_s / VL_TILE) per
+// system).
 __global__ __launch_bounds__(VL_PREP_THREADS) void vl_prep_kernel(VlArgs a) {
   __shared__ int part[VL_PREP_THREADS];
   for (int k = threadIdx.x; k < a.n_sys; k += VL_PREP_THREADS) a.maxd2[k] = 0u;
+  if (threadIdx.x == 0) *a.band_count = 0;
   vl_block_prefix(a.n_sys, a.tile_off, [&](int k) {
     return (a.sys_off[k + 1] - a.sys_off[k] + VL_TILE - 1) / VL_TILE;
   }, part);
@@ -708,19 +715,22 @@ __device__ inline void vl_walk_main(const VlArgs& a, const PairParams& pp, const
 }
 
 // PASS 0: the pair terms of every list entry below the band of a cutoff, one
-// lane per sorted atom; the lane keeps up to VL_NBAND entries it met inside a
-// band.  PASS 1 (the band pass): each lane that met any adds the band pairs
-// the FP64 decision keeps, from its stored entries, or, past VL_NBAND, by
-// walking its list again.  Partials of PASS 1 go to the second half of
-// a.partial.
+// lane per sorted atom.  A lane that met up to VL_NBAND entries inside a band
+// re-decides them in FP64 on the caller's positions after its walk (the
+// decision of the band pass, in the same launch); a lane that met more, or
+// met band pairs on the stencil path, is left to PASS 1 and its tile goes on
+// a compact list.  PASS 1 (the band pass): a persistent grid over that list,
+// each flagged lane walking its list (or stencil) again and adding the band
+// pairs the FP64 decision keeps.  Partials of PASS 1 go to the second half of
+// a.partial (the reduction reads them for the listed tiles only).
 template <bool EXCL, bool VIR, bool REC, int PASS>
 __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   __shared__ SysGeom g;
   __shared__ float s_sig2[PC_NSPEC * PC_NSPEC], s_e4[PC_NSPEC * PC_NSPEC],
       s_e24[PC_NSPEC * PC_NSPEC];
   __shared__ double red[VL_TILE / 64][PC_NPART];
-  int s, tile;
-  if (!vl_tile_of(a, s, tile)) return;
+  // one tile: the main pass's own, or (PASS 1) each tile of the band list
+  auto tile_work = [&](int s, int tile) {
   const int i0 = a.sys_off[s], n = a.sys_off[s + 1] - i0;
   const int start = tile * VL_TILE;
   const int tid = threadIdx.x;
@@ -729,13 +739,6 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   if (n > PC_NMAX) {
     if (tid < PC_NPART) part[tid] = 0.0;
     return;
-  }
-  if (PASS == 1) {
-    const int d = i0 + start + tid;
-    if (!__syncthreads_or(start + tid < n && a.bandflag[d])) {
-      if (tid < PC_NPART) part[tid] = 0.0;
-      return;
-    }
   }
   if (tid == 0) {
     g = a.geom[s];
@@ -778,9 +781,9 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
       }
     };
     const int cnt = a.nlist[d];
-    if (PASS == 1 && flag <= VL_NBAND) {
-      // the band pass over the entries the main pass stored
-      for (int k = 0; k < flag; ++k) {
+    // the FP64 decision over the band entries the main pass stored
+    auto stored_band = [&](int nb) {
+      for (int k = 0; k < nb; ++k) {
         const uint32_t e = a.bandbuf[(size_t)d * VL_NBAND + k];
         const int t = (int)(e & 0x7fffu);
         int sx, sy, sz;
@@ -791,7 +794,7 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
         const float ex = (rj.x + Sx) - ri.x, ey = (rj.y + Sy) - ri.y, ez = (rj.z + Sz) - ri.z;
         const float r2 = norm2_32(ex, ey, ez);
         int kept, mid;
-        const float gsum = vl_entry_terms<EXCL, PASS>(
+        const float gsum = vl_entry_terms<EXCL, 1>(
             pp, band, g, t, ri, rj, sx, sy, sz, r2, si, a.sspec[i0 + t], xlj,
             EXCL && a.has_lj_excl ? a.slj[i0 + t] : 0, xcl,
             EXCL && a.has_coul_excl ? a.scl[i0 + t] : 0, elj, ecl, kept, mid,
@@ -799,6 +802,9 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
         if (REC && kept) band_record(a.rec, g, i0, t, sx, sy, sz, kept);
         accumulate(gsum, ex, ey, ez);
       }
+    };
+    if (PASS == 1 && flag <= VL_NBAND) {
+      stored_band(flag);
     } else if (cnt <= a.cap && PASS == 0 && !REC && !EXCL) {
       // the hot walk: variants for orthorhombic cells (no cross terms in the
       // image shift) and for equal LJ and Coulomb cutoffs (one band test)
@@ -918,6 +924,10 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
         }
       }
     }
+    if (PASS == 0 && !inband && nband > 0 && nband <= VL_NBAND) {
+      stored_band(nband);
+      nband = 0;  // decided
+    }
     const int gi = i0 + a.sidx[d];
     if (PASS == 1 || a.accumulate) {
       a.forces[3 * gi] += ax; a.forces[3 * gi + 1] += ay; a.forces[3 * gi + 2] += az;
@@ -925,14 +935,34 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
       a.forces[3 * gi] = ax; a.forces[3 * gi + 1] = ay; a.forces[3 * gi + 2] = az;
     }
   }
-  if (PASS == 0 && slot < n)
-    a.bandflag[i0 + slot] = inband || nband > VL_NBAND ? VL_NBAND_FULL : nband;
+  if (PASS == 0) {
+    // lanes left to the band pass, and their tile on its list
+    const bool full = slot < n && (inband || nband > VL_NBAND);
+    if (slot < n) a.bandflag[i0 + slot] = full ? VL_NBAND_FULL : 0;
+    const bool any = __syncthreads_or(full);
+    if (tid == 0) {
+      a.tile_full[(size_t)s * a.tiles + tile] = any;
+      if (any) a.band_tiles[atomicAdd(a.band_count, 1)] = s * a.tiles + tile;
+    }
+  }
   const double v[PC_NPART] = {0.5 * elj, 0.5 * ecl, w00, w01, w02, w11, w12, w22};
   pc_block_partials<VL_TILE>(v, red, part);
+  };
+  if (PASS == 0) {
+    int s, tile;
+    if (vl_tile_of(a, s, tile)) tile_work(s, tile);
+  } else {
+    const int nt = *a.band_count;
+    for (int k = blockIdx.x; k < nt; k += gridDim.x) {
+      const int lin = a.band_tiles[k];
+      tile_work(lin / a.tiles, lin % a.tiles);
+      __syncthreads();
+    }
+  }
 }
 
-// Per system: the main pass's tile partials, then the band pass's, in tile
-// order (the system's own tiles only).
+// Per system: the main pass's tile partials, then the band pass's (tiles on
+// its list), in tile order (the system's own tiles only).
 __global__ void vl_reduce_kernel(VlArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n_sys) return;
@@ -940,8 +970,10 @@ __global__ void vl_reduce_kernel(VlArgs a) {
   double acc[PC_NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int h = 0; h < 2; ++h) {
     const double* p = a.partial + ((size_t)h * a.n_sys + s) * a.tiles * PC_NPART;
-    for (int t = 0; t < nt; ++t)
+    for (int t = 0; t < nt; ++t) {
+      if (h == 1 && !a.tile_full[(size_t)s * a.tiles + t]) continue;  // not on the band list
       for (int k = 0; k < PC_NPART; ++k) acc[k] += p[(size_t)t * PC_NPART + k];
+    }
   }
   if (a.energy) {
     double* e = a.energy + 2 * (size_t)s;
@@ -958,7 +990,8 @@ __global__ void vl_reduce_kernel(VlArgs a) {
 struct VlLayout {
   size_t geom, spos, sref, sidx, scell, sspec, slj, scl, cstart, nlist, list, valid, need, reprc,
       maxd2,
-      counters, partial, bandflag, bandbuf, tile_off, cell_off, total;
+      counters, partial, bandflag, bandbuf, tile_off, cell_off, band_tiles, band_count, tile_full,
+      total;
   VlLayout(int n_sys, int n_atoms, int cap, int tiles) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
@@ -984,6 +1017,9 @@ struct VlLayout {
     bandbuf = take(sizeof(uint32_t) * na * VL_NBAND);
     tile_off = take(sizeof(int) * (ns + 1));
     cell_off = take(sizeof(int) * (ns + 1));
+    band_tiles = take(sizeof(int) * ns * (size_t)tiles);
+    band_count = take(sizeof(int));
+    tile_full = take(ns * (size_t)tiles);
     total = off;
   }
 };
@@ -1162,6 +1198,9 @@ extern "C" int kupsmi_pair_forces_verlet(
   a.bandbuf = reinterpret_cast<uint32_t*>(b + lay.bandbuf);
   a.tile_off = reinterpret_cast<int*>(b + lay.tile_off);
   a.cell_off = reinterpret_cast<int*>(b + lay.cell_off);
+  a.band_tiles = reinterpret_cast<int*>(b + lay.band_tiles);
+  a.band_count = reinterpret_cast<int*>(b + lay.band_count);
+  a.tile_full = reinterpret_cast<uint8_t*>(b + lay.tile_full);
   a.forces = forces; a.energy = energy; a.virial = virial;
   int ncu = 256;
   KUPSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1200,17 +1239,17 @@ extern "C" int kupsmi_pair_forces_verlet(
     record_clear_kernel<<<1, 1, 0, st>>>(a.rec.count);
     KUPSMI_CHECK_LAUNCH();
   }
-#define VL_PASS(P)                                                                 \
-  if (rec) {                                                                       \
-    if (excl) vl_pair_kernel<true, true, true, P><<<grid, VL_TILE, 0, st>>>(a);    \
-    else vl_pair_kernel<false, true, true, P><<<grid, VL_TILE, 0, st>>>(a);        \
-  } else if (excl) {                                                               \
-    if (vir) vl_pair_kernel<true, true, false, P><<<grid, VL_TILE, 0, st>>>(a);    \
-    else vl_pair_kernel<true, false, false, P><<<grid, VL_TILE, 0, st>>>(a);       \
-  } else {                                                                         \
-    if (vir) vl_pair_kernel<false, true, false, P><<<grid, VL_TILE, 0, st>>>(a);   \
-    else vl_pair_kernel<false, false, false, P><<<grid, VL_TILE, 0, st>>>(a);      \
-  }                                                                                \
+#define VL_PASS(P, G)                                                            \
+  if (rec) {                                                                     \
+    if (excl) vl_pair_kernel<true, true, true, P><<<G, VL_TILE, 0, st>>>(a);     \
+    else vl_pair_kernel<false, true, true, P><<<G, VL_TILE, 0, st>>>(a);         \
+  } else if (excl) {                                                             \
+    if (vir) vl_pair_kernel<true, true, false, P><<<G, VL_TILE, 0, st>>>(a);     \
+    else vl_pair_kernel<true, false, false, P><<<G, VL_TILE, 0, st>>>(a);        \
+  } else {                                                                       \
+    if (vir) vl_pair_kernel<false, true, false, P><<<G, VL_TILE, 0, st>>>(a);    \
+    else vl_pair_kernel<false, false, false, P><<<G, VL_TILE, 0, st>>>(a);       \
+  }                                                                              \
   KUPSMI_CHECK_LAUNCH();
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (v->profile) {
@@ -1218,12 +1257,15 @@ extern "C" int kupsmi_pair_forces_verlet(
     KUPSMI_HIP(hipEventCreate(&ev1));
     KUPSMI_HIP(hipEventRecord(ev0, st));
   }
-  VL_PASS(0)
+  VL_PASS(0, grid)
   if (v->profile) {
     KUPSMI_HIP(hipEventRecord(ev1, st));
     v->events.emplace_back(ev0, ev1);
   }
-  VL_PASS(1)
+  // the band pass: persistent, over the tiles the main pass listed (usually
+  // none: a few workgroups read the count and leave)
+  const unsigned band_grid = (unsigned)min(grid.x, (unsigned)(2 * ncu));
+  VL_PASS(1, band_grid)
 #undef VL_PASS
   vl_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();

@@ -201,10 +201,11 @@ class EwaldLJPropagator:
         self.overlap = True
         # with overlap and Verlet lists, True holds the pair passes (not the
         # list upkeep) until the structure factors are done, so the list
-        # upkeep runs beside them and the pair passes beside the force pass;
-        # False lets the pair passes follow the list upkeep at once (measured
-        # ~2 % faster on the 512-melt workload at skins 1.0 and 1.5 A)
-        self.gate_pairs = False
+        # upkeep runs beside them and the pair passes beside the force pass
+        # (measured 1-5 % faster than False on the 512-melt workload at skins
+        # 1.0 to 2.0 A); False lets the pair passes follow the list upkeep at
+        # once
+        self.gate_pairs = True
         self._side = None
         self._f2 = None
         self._vir2 = None
