@@ -20,16 +20,28 @@ __global__ __launch_bounds__(IG_THREADS) void ig_pre_kernel(
     int integrator, int n_sys, const int32_t* sys_off, float* pos, float* mom,
     const float* forces, const float* mass, const double* tril, const double* dt,
     const double* kT, const double* gamma, uint64_t seed, int64_t step, const int64_t* step_dev) {
+  // the system's constants once per workgroup (the inverse cell takes four
+  // FP64 divisions, the O step an FP64 exp and sqrt)
+  __shared__ float s_Lif[6], s_c1, s_c2;
   const int s = blockIdx.y;
+  const double* t = tril + 6 * s;
+  if (threadIdx.x == 0) {
+    double li[6];
+    tril_inverse(t, li);
+    for (int k = 0; k < 6; ++k) s_Lif[k] = (float)li[k];
+    if (integrator == 1) {
+      const double e1 = exp(-gamma[s] * dt[s]);
+      s_c1 = (float)e1;
+      s_c2 = (float)sqrt(kT[s] * (1.0 - e1 * e1));
+    }
+  }
+  __syncthreads();
   const int i0 = sys_off[s], n = sys_off[s + 1] - i0;
   const int i = blockIdx.x * IG_THREADS + threadIdx.x;
   if (i >= n) return;
   const int gi = i0 + i;
-  const double* t = tril + 6 * s;
-  double li[6];
-  tril_inverse(t, li);
-  float Lf[6], Lif[6];
-  for (int k = 0; k < 6; ++k) { Lf[k] = (float)t[k]; Lif[k] = (float)li[k]; }
+  float Lif[6];
+  for (int k = 0; k < 6; ++k) Lif[k] = s_Lif[k];
   const float d = (float)dt[s], h = 0.5f * d;
   const float im = 1.0f / mass[gi];
   float px = mom[3 * gi] + forces[3 * gi] * h;
@@ -41,8 +53,7 @@ __global__ __launch_bounds__(IG_THREADS) void ig_pre_kernel(
   float f0, f1, f2;
   if (integrator == 1) {
     wrap_shift(t, Lif, rx, ry, rz, f0, f1, f2);
-    const double e1 = exp(-gamma[s] * dt[s]);
-    const float c1 = (float)e1, c2 = (float)sqrt(kT[s] * (1.0 - e1 * e1));
+    const float c1 = s_c1, c2 = s_c2;
     float e0, ea, eb;
     normals3(seed, (uint32_t)gi, step_dev ? *step_dev : step, e0, ea, eb);
     const float sm = c2 * sqrtf(mass[gi]);
@@ -56,15 +67,20 @@ __global__ __launch_bounds__(IG_THREADS) void ig_pre_kernel(
 
 // Closing B(dt/2) and the kinetic energy, one workgroup per system (fixed
 // reduction order).
-__global__ __launch_bounds__(IG_THREADS) void ig_post_kernel(
+// One workgroup of IG_POST_THREADS per system (the kinetic energy is one
+// fixed-order FP64 sum per system); wide, so that enough waves are in flight
+// for this memory-bound pass.
+constexpr int IG_POST_THREADS = 1024;
+
+__global__ __launch_bounds__(IG_POST_THREADS) void ig_post_kernel(
     int n_sys, const int32_t* sys_off, float* mom, const float* forces, const float* mass,
     const double* dt, int kick, double* kin) {
-  __shared__ double red[IG_THREADS / 64];
+  __shared__ double red[IG_POST_THREADS / 64];
   const int s = blockIdx.x;
   const int i0 = sys_off[s], n = sys_off[s + 1] - i0;
   const float h = kick ? 0.5f * (float)dt[s] : 0.0f;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += IG_THREADS) {
+  for (int i = threadIdx.x; i < n; i += IG_POST_THREADS) {
     const int gi = i0 + i;
     float px = mom[3 * gi] + forces[3 * gi] * h;
     float py = mom[3 * gi + 1] + forces[3 * gi + 1] * h;
@@ -78,7 +94,7 @@ __global__ __launch_bounds__(IG_THREADS) void ig_post_kernel(
   __syncthreads();
   if (threadIdx.x == 0 && kin) {
     double e = 0.0;
-    for (int w = 0; w < IG_THREADS / 64; ++w) e += red[w];
+    for (int w = 0; w < IG_POST_THREADS / 64; ++w) e += red[w];
     kin[s] = e;
   }
 }
@@ -208,7 +224,7 @@ extern "C" int kupsmi_integrate(void* stream, int phase, int integrator, int n_s
                                             tril, dt, kT, gamma, seed, step, step_dev);
   } else {
     // phase 1: kick + kinetic energy; phase 2: kinetic energy only
-    ig_post_kernel<<<n_sys, IG_THREADS, 0, st>>>(n_sys, sys_off, mom, forces, mass, dt,
+    ig_post_kernel<<<n_sys, IG_POST_THREADS, 0, st>>>(n_sys, sys_off, mom, forces, mass, dt,
                                                  phase == 1, kin_energy);
   }
   KUPSMI_CHECK_LAUNCH();
