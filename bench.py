@@ -55,7 +55,9 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30, help="timed LJ+Ewald steps (headline)")
+    ap.add_argument("--steps", type=int, default=100,
+                    help="timed LJ+Ewald steps (headline; enough to average over the list "
+                         "rebuilds, which come in bursts after the first build)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed LJ+Ewald steps")
     ap.add_argument("--no-lj", action="store_true")
     ap.add_argument("--lj-steps", type=int, default=500)
