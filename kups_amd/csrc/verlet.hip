@@ -961,20 +961,22 @@ __global__ __launch_bounds__(VL_TILE) void vl_pair_kernel(VlArgs a) {
   }
 }
 
-// Per system: the main pass's tile partials, then the band pass's (tiles on
-// its list), in tile order (the system's own tiles only).
-__global__ void vl_reduce_kernel(VlArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n_sys) return;
+// Per system: the main pass's tile partials and the band pass's (tiles on
+// its list), the system's own tiles only, in a fixed order.
+__global__ __launch_bounds__(64) void vl_reduce_kernel(VlArgs a) {
+  // one wave per system: lane l sums the partials j = l, l + 64, ... of the
+  // list (main pass tiles, then band pass tiles), then a fixed butterfly
+  const int s = blockIdx.x, lane = threadIdx.x;
   const int nt = a.tile_off[s + 1] - a.tile_off[s];
   double acc[PC_NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int h = 0; h < 2; ++h) {
-    const double* p = a.partial + ((size_t)h * a.n_sys + s) * a.tiles * PC_NPART;
-    for (int t = 0; t < nt; ++t) {
-      if (h == 1 && !a.tile_full[(size_t)s * a.tiles + t]) continue;  // not on the band list
-      for (int k = 0; k < PC_NPART; ++k) acc[k] += p[(size_t)t * PC_NPART + k];
-    }
+  for (int j = lane; j < 2 * nt; j += 64) {
+    const int h = j >= nt, t = j - h * nt;
+    if (h == 1 && !a.tile_full[(size_t)s * a.tiles + t]) continue;  // not on the band list
+    const double* p = a.partial + (((size_t)h * a.n_sys + s) * a.tiles + t) * PC_NPART;
+    for (int k = 0; k < PC_NPART; ++k) acc[k] += p[k];
   }
+  for (int k = 0; k < PC_NPART; ++k) acc[k] = wave_sum(acc[k]);
+  if (lane != 0) return;
   if (a.energy) {
     double* e = a.energy + 2 * (size_t)s;
     if (a.accumulate) { e[0] += acc[0]; e[1] += acc[1]; } else { e[0] = acc[0]; e[1] = acc[1]; }
@@ -1267,7 +1269,7 @@ extern "C" int kupsmi_pair_forces_verlet(
   const unsigned band_grid = (unsigned)min(grid.x, (unsigned)(2 * ncu));
   VL_PASS(1, band_grid)
 #undef VL_PASS
-  vl_reduce_kernel<<<(n_sys + 255) / 256, 256, 0, st>>>(a);
+  vl_reduce_kernel<<<n_sys, 64, 0, st>>>(a);
   KUPSMI_CHECK_LAUNCH();
   return KUPSMI_OK;
 }
