@@ -344,26 +344,27 @@ __device__ __forceinline__ void vl_list_chunk(uint32_t* blk, unsigned N, int cap
                                               const float (&ry)[VL_LC_AB],
                                               const float (&rz)[VL_LC_AB], float rl2,
                                               int (&cnt)[VL_LC_AB]) {
+  // lanes holding unpackable entries (the whole chunk's, once)
+  const uint64_t ubad = BAD ? __builtin_amdgcn_ballot_w64(ent == VL_UNPACKABLE) : 0ull;
 #pragma unroll
   for (int k = 0; k < VL_LC_AB; ++k) {
     if (k < nl) {
-      bool acc = norm2_32(xj - rx[k], yj - ry[k], zj - rz[k]) < rl2;
-      if (SELF) acc = acc && own != k;
-      const uint64_t m = __builtin_amdgcn_ballot_w64(acc);
+      // the accepted lanes as a mask straight from the compare (d^2 < rl2)
+      uint64_t m = __builtin_amdgcn_fcmpf(norm2_32(xj - rx[k], yj - ry[k], zj - rz[k]), rl2, 4);
+      if (SELF) m &= ~__builtin_amdgcn_ballot_w64(own == k);
       if (m) {
         const int c0 = cnt[k];
         cnt[k] = c0 + (int)__popcll(m);
-        if (BAD && __builtin_amdgcn_ballot_w64(acc && ent == VL_UNPACKABLE)) {
+        if (BAD && (m & ubad)) {
           cnt[k] = max(cnt[k], cap + 1);
           continue;
         }
-        if (c0 < cap) {
-          const unsigned e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)c0));
-          if (acc && e < (unsigned)cap) {
-            if (WIDE) blk[4 * k + ((size_t)(e >> 2) * N << 2) + (e & 3)] = ent;
-            else blk[4 * k + (size_t)(__umul24(e >> 2, N << 2) + (e & 3))] = ent;
-          }
+        // entries past cap (an overflowing atom) are not written
+        const unsigned e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)c0));
+        if (__builtin_amdgcn_inverse_ballot_w64(m) && e < (unsigned)cap) {
+          if (WIDE) blk[4 * k + ((size_t)(e >> 2) * N << 2) + (e & 3)] = ent;
+          else blk[4 * k + (size_t)(__umul24(e >> 2, N << 2) + (e & 3))] = ent;
         }
       }
     }
