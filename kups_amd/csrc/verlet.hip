@@ -298,12 +298,16 @@ __global__ __launch_bounds__(VL_BIN_THREADS) __attribute__((amdgpu_num_vgpr(96))
 
 // Lists of the rebuilt systems, one wave per cell of the compact work list
 // (cell_off).  The wave walks the cell's stencil once per group of up to 64
-// of its atoms: row by row (dz, dy), each row of 2 h0 + 1 cells one
-// contiguous slot range of the sorted system per image shift along x (one
-// piece, two where it wraps), it copies the candidates, moved by their image
-// shift and with their packed entries, into a window of VL_LC_W in LDS (the
-// stencil's pieces back to back, so the chunks below are full whatever the
-// row lengths).  Each time the window fills, and at the end, the group's
+// of its atoms: a row (dz, dy) of 2 h0 + 1 cells is one contiguous slot
+// range of the sorted system per image shift along x (one piece, two where
+// it wraps).  For a batch of rows the lanes (one per row) write the pieces
+// as descriptors with their offsets in the batch's candidate stream (all
+// rows' cell starts loaded at once); the wave then copies the stream, 64
+// positions at a time with each lane finding its piece among the few that
+// overlap (one candidate load per lane), moved by the image shift and with
+// the packed entries, into a window of VL_LC_W in LDS (the stencil's pieces
+// back to back, so the chunks below are full whatever the row lengths).
+// Each time the window fills, and at the end, the group's
 // atoms take it VL_LC_AB at a time (positions and counts uniform) in chunks
 // of 64 candidates; the accepted lanes of a chunk append their entries in
 // lane order (ballot and prefix count), so an atom's entries come in the
@@ -329,7 +333,14 @@ constexpr int VL_LC_THREADS = 256;  // four workers (waves) per workgroup
 constexpr int VL_LC_AB = 8;         // atoms of a cell per pass over the window
 constexpr int VL_LC_W = 512;        // candidates per window (8 KB of LDS per wave)
 constexpr int VL_LC_WG_PER_CU = 4;  // persistent workgroups per CU (measured: 5 is no faster)
+constexpr int VL_LC_ND = 128;       // stencil piece descriptors per batch of rows (2 KB per wave)
 constexpr uint32_t VL_UNPACKABLE = 0xffffffffu;  // entry of an image shift beyond VL_SHIFT_MAX
+
+// A stencil piece of the list build: first slot, offset in the batch's
+// candidate stream, image shift (signed bytes x, y, z) and slot count.
+struct VlPiece {
+  int tb, pos, shifts, len;
+};
 
 // One chunk of 64 window candidates (this lane's: position xj, yj, zj and
 // packed entry ent) against the block's nl atoms: entries appended in lane
@@ -374,11 +385,13 @@ __device__ __forceinline__ void vl_list_chunk(uint32_t* blk, unsigned N, int cap
 template <bool WIDE>
 __global__ __launch_bounds__(VL_LC_THREADS) void vl_list_kernel(VlArgs a) {
   __shared__ float4 vl_win[VL_LC_THREADS / 64][VL_LC_W];
+  __shared__ VlPiece vl_desc[VL_LC_THREADS / 64][VL_LC_ND];
   const int T = a.cell_off[a.n_sys];
   if (T == 0) return;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   float4* win = vl_win[wv];
+  VlPiece* desc = vl_desc[wv];
   const int die = blockIdx.x % VL_XCDS;
   const int wpd = (int)(gridDim.x / VL_XCDS) * (VL_LC_THREADS / 64);  // workers per die
   const int per = (T + VL_XCDS - 1) / VL_XCDS;
@@ -423,6 +436,19 @@ __global__ __launch_bounds__(VL_LC_THREADS) void vl_list_kernel(VlArgs a) {
     };
     const float4* sp = a.spos + i0;
     const int* ssp = a.sspec + i0;
+    const int nry = 2 * h1 + 1, nrows = (2 * h2 + 1) * nry;
+    const int pmax = (2 * h0) / ncx + 2;  // pieces of constant x shift per row, at most
+    if (pmax > VL_LC_ND) {
+      // more image pieces per row than the descriptors hold: no lists, the
+      // pair passes walk the stencil for these atoms
+      for (int k = a0 + lane; k < a1; k += 64) {
+        a.nlist[i0 + k] = cap + 1;
+        a.sref[i0 + k] = sp[k];
+        atomicAdd(a.counters + 1, 1ull);
+      }
+      continue;
+    }
+    const int rows_per_batch = min(64, VL_LC_ND / pmax);
     for (int ag = a0; ag < a1; ag += 64) {
       const int na = min(64, a1 - ag);
       int vcnt = 0;          // lane k: entries of atom ag + k so far
@@ -472,53 +498,113 @@ __global__ __launch_bounds__(VL_LC_THREADS) void vl_list_kernel(VlArgs a) {
         wn = 0;
         bad = false;
       };
-      for (int dz = -h2; dz <= h2; ++dz) {
-        const float gz = prune ? gap2(dz, wz) : 0.0f;
-        if (gz > lim) continue;
-        int zz = cz + dz;
-        const int sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
-        zz -= sz * ncz;
-        for (int dy = -h1; dy <= h1; ++dy) {
-          const float gyz = gz + (prune ? gap2(dy, wy) : 0.0f);
-          if (gyz > lim) continue;
-          int yy = cy + dy;
-          const int sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
-          yy -= sy * ncy;
-          int dxa = -h0, dxb = h0;
-          if (prune) {
-            while (dxa < 0 && gyz + gap2(dxa, wx) > lim) ++dxa;
-            while (dxb > 0 && gyz + gap2(dxb, wx) > lim) --dxb;
-          }
-          const int* row = cs + (zz * ncy + yy) * ncx;
-          // pieces of constant image shift along x, in dx order
-          for (int xs = cx + dxa; xs <= cx + dxb;) {
-            const int sx = xs >= 0 ? xs / ncx : -((-xs + ncx - 1) / ncx);
-            const int xe = min(cx + dxb, (sx + 1) * ncx - 1);
-            const int tb = row[xs - sx * ncx], te = row[xe - sx * ncx + 1];
-            xs = xe + 1;
-            if (tb == te) continue;
-            float Sx, Sy, Sz;
-            shift_vec32(Lf, sx, sy, sz, Sx, Sy, Sz);
-            const bool packable = sx >= -VL_SHIFT_MAX && sx <= VL_SHIFT_MAX && sy >= -VL_SHIFT_MAX &&
-                                  sy <= VL_SHIFT_MAX && sz >= -VL_SHIFT_MAX && sz <= VL_SHIFT_MAX;
-            const uint32_t bits = vl_pack(0, sx, sy, sz, 0);
-            // the group's atoms themselves: in their own cell at zero shift
-            if (sx == 0 && sy == 0 && sz == 0 && tb <= ag && ag < te) selfp = G + wn + (ag - tb);
-            for (int t0 = tb; t0 < te; t0 += 64) {
-              const int n = min(64, te - t0);
-              if (wn + n > VL_LC_W) flush();
-              if (lane < n) {
-                const int t = t0 + lane;
-                const float4 pj = sp[t];
-                const uint32_t ent = packable ? (uint32_t)t | bits | ((uint32_t)ssp[t] << 27)
-                                              : VL_UNPACKABLE;
-                win[wn + lane] = make_float4(pj.x + Sx, pj.y + Sy, pj.z + Sz, __uint_as_float(ent));
-              }
-              wn += n;
-              bad = bad || !packable;
+      // rows (dz, dy) of the stencil in batches whose pieces fit the
+      // descriptors; per batch, one lane per row writes its pieces (one
+      // contiguous slot range per image shift along x, in dx order) with
+      // their offsets in the batch's candidate stream (the cell starts of
+      // all rows loaded together), then the window is filled 64 stream
+      // positions at a time (one candidate load per lane)
+      for (int r0 = 0; r0 < nrows; r0 += rows_per_batch) {
+        const int r = r0 + lane;
+        int npl = 0, zz = 0, yy = 0, sy = 0, sz = 0, dxa = 0, dxb = -1;
+        bool own_row = false;
+        if (lane < rows_per_batch && r < nrows) {
+          const int dz = r / nry - h2, dy = r % nry - h1;
+          own_row = dz == 0 && dy == 0;
+          const float gyz = prune ? gap2(dz, wz) + gap2(dy, wy) : 0.0f;
+          if (gyz <= lim) {
+            zz = cz + dz;
+            sz = zz >= 0 ? zz / ncz : -((-zz + ncz - 1) / ncz);
+            zz -= sz * ncz;
+            yy = cy + dy;
+            sy = yy >= 0 ? yy / ncy : -((-yy + ncy - 1) / ncy);
+            yy -= sy * ncy;
+            dxa = -h0;
+            dxb = h0;
+            if (prune) {
+              while (dxa < 0 && gyz + gap2(dxa, wx) > lim) ++dxa;
+              while (dxb > 0 && gyz + gap2(dxb, wx) > lim) --dxb;
             }
+            const int xa = cx + dxa, xb = cx + dxb;
+            const int fa = xa >= 0 ? xa / ncx : -((-xa + ncx - 1) / ncx);
+            const int fb = xb >= 0 ? xb / ncx : -((-xb + ncx - 1) / ncx);
+            npl = fb - fa + 1;
           }
         }
+        const int* rowp = cs + (zz * ncy + yy) * ncx;
+        // the lane's pieces in dx order: pass 1 sums their lengths, pass 2
+        // (cached loads) writes the descriptors
+        auto pieces = [&](auto&& use) {
+          int k = 0;
+          for (int xs = cx + dxa; xs <= cx + dxb; ++k) {
+            const int sx = xs >= 0 ? xs / ncx : -((-xs + ncx - 1) / ncx);
+            const int xe = min(cx + dxb, (sx + 1) * ncx - 1);
+            const int tb = rowp[xs - sx * ncx];
+            use(k, tb, rowp[xe - sx * ncx + 1] - tb, sx);
+            xs = xe + 1;
+          }
+        };
+        int lsum = 0;
+        pieces([&](int, int, int len, int) { lsum += len; });
+        int incl = lsum, inp = npl;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(incl, o, 64), z = __shfl_up(inp, o, 64);
+          if (lane >= o) { incl += y; inp += z; }
+        }
+        const int bn = __builtin_amdgcn_readlane(incl, 63);  // candidates of the batch
+        const int np = __builtin_amdgcn_readlane(inp, 63);   // pieces of the batch
+        int self_off = -1;
+        {
+          int pos = incl - lsum;
+          const int k0 = inp - npl;
+          pieces([&](int k, int tb, int len, int sx) {
+            desc[k0 + k] = VlPiece{tb, pos, (sx & 255) | ((sy & 255) << 8) | ((sz & 255) << 16), len};
+            // the group's atoms themselves: in their own cell at zero shift
+            if (own_row && sx == 0 && sy == 0 && sz == 0 && tb <= ag && ag < tb + len)
+              self_off = pos + (ag - tb);
+            pos += len;
+          });
+        }
+        const uint64_t has_self = __builtin_amdgcn_ballot_w64(self_off >= 0);
+        if (has_self)
+          selfp = G + wn + __builtin_amdgcn_readlane(self_off, (int)__builtin_ctzll(has_self));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // fill the window: stream positions c .. c + 63 of the batch; lane j
+        // takes the piece holding c + j (pieces kc .. of the batch overlap)
+        int kc = 0;
+        for (int c = 0; c < bn; c += 64) {
+          const int n = min(64, bn - c);
+          if (wn + n > VL_LC_W) flush();
+          const int p = c + lane;
+          int t = 0, psx = 0, psy = 0, psz = 0;
+          for (int k = kc; k < np; ++k) {
+            const VlPiece d = desc[k];
+            if (d.pos >= c + 64) break;
+            if (p >= d.pos && p < d.pos + d.len) {
+              t = d.tb + (p - d.pos);
+              psx = (int)(signed char)d.shifts;
+              psy = (int)(signed char)(d.shifts >> 8);
+              psz = (int)(signed char)(d.shifts >> 16);
+            }
+          }
+          while (kc < np && desc[kc].pos + desc[kc].len <= c + 64) ++kc;
+          const bool packable = psx >= -VL_SHIFT_MAX && psx <= VL_SHIFT_MAX && psy >= -VL_SHIFT_MAX &&
+                                psy <= VL_SHIFT_MAX && psz >= -VL_SHIFT_MAX && psz <= VL_SHIFT_MAX;
+          if (lane < n) {
+            const float4 pj = sp[t];
+            float Sx, Sy, Sz;
+            shift_vec32(Lf, psx, psy, psz, Sx, Sy, Sz);
+            const uint32_t ent = packable ? (uint32_t)t | vl_pack(0, psx, psy, psz, 0) |
+                                                ((uint32_t)ssp[t] << 27)
+                                          : VL_UNPACKABLE;
+            win[wn + lane] = make_float4(pj.x + Sx, pj.y + Sy, pj.z + Sz, __uint_as_float(ent));
+          }
+          bad = bad || __builtin_amdgcn_ballot_w64(lane < n && !packable) != 0;
+          wn += n;
+        }
+        __builtin_amdgcn_wave_barrier();
       }
       if (wn > 0) flush();
       // per atom of the group (lane k): pad the last group, count, reference
