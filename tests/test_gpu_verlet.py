@@ -146,6 +146,24 @@ def test_verlet_every_tile_of_few_large_systems(gpu, ms):
     assert vl.counters() == (len(ms), 0)
 
 
+def test_verlet_cells_of_more_than_64_atoms(gpu):
+    """A long cutoff (24 A, skin 2 A) on an 8000-ion melt: cells of ~13 A hold
+    ~120 atoms, so the list build walks each cell's stencil once per group of
+    64 of its atoms and refills its candidate window many times per group;
+    lists of ~3300 entries per atom."""
+    b = workloads.nacl_melts(ms=[10], seed=14)
+    lj = LennardJonesParameters.from_dict(24.0, {"Na": (2.5, 0.01), "Cl": (4.4, 0.01)})
+    c = Case(b, gpu, lj, 24.0, alpha=0.2, rc_cl=24.0)
+    skin = 2.0
+    vl = VerletList(skin, neighbor_capacity(b.sizes, b.tril, 24.0, skin))
+    _same(c.verlet(vl), c.cells())
+    assert vl.counters() == (1, 0)
+    c.pos = (c.pos + 0.4).contiguous()
+    c.wrap()
+    _same(c.verlet(vl), c.cells())
+    assert vl.counters() == (1, 0)
+
+
 def test_verlet_overflowed_lists_use_the_stencil(gpu):
     b = workloads.nacl_melts(ms=[5], seed=9)
     c = Case(b, gpu, NACL_LJ, 12.0, alpha=0.3, rc_cl=12.0)
